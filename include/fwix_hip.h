@@ -1,0 +1,131 @@
+/*
+ * fwix_hip.h -- C ABI of the MI355X (gfx950) one-way depth-stepping library
+ * (libpyfwix_hip.so).
+ *
+ * Every entry point replaces one method of the reference's device operator
+ * API: CudaOperator<M,D>::cu_forward / cu_adjoint (operator/src/CudaOperator.h:61-72)
+ * and the concrete operators in operator/src and propagator/src.  Shapes are
+ * passed by value, buffers are plain device pointers owned by the caller
+ * (an operator borrows them, as the reference's operators borrow the
+ * complex_vector* they are given, CudaOperator.h:29-53), and streams are
+ * explicit hipStream_t values (0 = the default stream).
+ *
+ * Conventions
+ *   - complex64 samples, interleaved (re, im), layout x fastest:
+ *     wavefield (NS, NW, NY, NX), labels (nz, NW, NY, NX) uint8,
+ *     reference slownesses (nz, nref, NW), slowness / model (nz, NW, ny, nx),
+ *     traces (ntrace, NW) -- operator/src/complex_vector.h:10-20,
+ *     propagator/src/injection.cu:26-27.
+ *   - add = 0 overwrites the output, add != 0 accumulates into it
+ *     (CudaOperator.h:74-120).
+ *   - every function returns 0 on success and a negative value on error;
+ *     fwix_last_error() then returns a message (thread-local).  Nothing
+ *     calls exit() (the reference's gpuAssert does, complex_vector.h:22-30).
+ *   - calls on one operator handle must not overlap in time (each handle owns
+ *     one set of scratch buffers).
+ */
+#ifndef FWIX_HIP_H
+#define FWIX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fwix_c64 { float re, im; } fwix_c64;
+typedef void* fwix_stream;               /* a hipStream_t */
+typedef struct fwix_op_s* fwix_op;       /* opaque operator handle */
+
+/* Wavefield hypercube: the padded (x, y, w, s) domain of the step operators
+ * (propagator/src/Propagator.cpp:61-73: NX = nx + padx, NY = ny + pady, the
+ * padding appended at the high side, origin and spacing of the slowness). */
+typedef struct fwix_dims {
+  int NX, NY, NW, NS;       /* padded lateral grid, frequencies, shots */
+  int nx, ny;               /* unpadded model grid, nx <= NX, ny <= NY */
+  float ox, dx, oy, dy;     /* lateral axes (origin, spacing) */
+  float ow, dw;             /* frequency axis in Hz: f = ow + iw*dw */
+  float oz, dz;             /* depth axis */
+  int nz;                   /* depth steps */
+} fwix_dims;
+
+/* Operator parameters: the reference's paramObj keys and defaults
+ * (nref 1 OneStep.h:64, taperx/tapery 0 Taper.h:16-17, eps 0 OneStep.h:66,
+ * eps_scat 0.04 Scatter.cpp:12, taylor 1 Scatter.cpp:14). */
+typedef struct fwix_params {
+  int nref;
+  int taperx, tapery;
+  float eps;
+  float eps_scat;
+  int taylor;
+  int planes_per_chunk;     /* planes per fused launch; 0 = automatic */
+} fwix_params;
+
+enum { FWIX_PSPI = 0, FWIX_NSPS = 1 };
+
+/* ---------------------------------------------------------------- library */
+int fwix_version(void);
+const char* fwix_last_error(void);
+void fwix_params_default(fwix_params* p);
+/* 1 if an n-point lateral FFT is supported (2^a or 3*2^a, 8 <= n <= 1024) */
+int fwix_fft_size_supported(int n);
+/* release any operator handle */
+int fwix_op_destroy(fwix_op op);
+
+/* ---------------------------------------------------------------- cuFFT2d
+ * Unitary batched 2-D C2C transform over (y, x), e^{-i} forward, adjoint =
+ * inverse.  Replaces cuFFT2d (operator/src/FFT.h:10-37, FFT.cpp:5-69 with
+ * the rsqrt(NX*NY) store callback, fft_callback.cu:7-22).  model == data is
+ * allowed (the in-place cu_forward(data) / cu_adjoint(data), FFT.cpp:57-69). */
+int fwix_fft2d_create(const fwix_dims* d, fwix_op* out);
+int fwix_fft2d_forward(fwix_op op, int add, const fwix_c64* model, fwix_c64* data, fwix_stream s);
+int fwix_fft2d_adjoint(fwix_op op, int add, fwix_c64* model, const fwix_c64* data, fwix_stream s);
+
+/* ---------------------------------------------------------------- PhaseShift
+ * PhaseShift(domain, dz, eps) (propagator/src/PhaseShift.h:11-72,
+ * PhaseShift.cpp:6-48, phase_shift.cu).  set_slow copies the NW reference
+ * slownesses of one reference from host memory (PhaseShift.h:21-23). */
+int fwix_phase_shift_create(const fwix_dims* d, float dz, float eps, fwix_op* out);
+int fwix_phase_shift_set_slow(fwix_op op, const fwix_c64* sref_host);
+int fwix_phase_shift_forward(fwix_op op, int add, const fwix_c64* model, fwix_c64* data, fwix_stream s);
+int fwix_phase_shift_adjoint(fwix_op op, int add, fwix_c64* model, const fwix_c64* data, fwix_stream s);
+int fwix_phase_shift_inverse(fwix_op op, int add, fwix_c64* model, const fwix_c64* data, fwix_stream s);
+
+/* ---------------------------------------------------------------- Selector
+ * Selector(domain) + set_labels / set_value (propagator/src/Selector.h:8-58,
+ * selector.cu:7-37).  Labels are int32 (NW, NY, NX) host memory, as the
+ * reference's RefSampler hands them over (RefSampler.h:33-36). */
+int fwix_selector_create(const fwix_dims* d, fwix_op* out);
+int fwix_selector_set_labels(fwix_op op, const int32_t* labels_host);
+int fwix_selector_set_value(fwix_op op, int value);
+int fwix_selector_forward(fwix_op op, int add, const fwix_c64* model, fwix_c64* data, fwix_stream s);
+int fwix_selector_adjoint(fwix_op op, int add, fwix_c64* model, const fwix_c64* data, fwix_stream s);
+
+/* ---------------------------------------------------------------- Taper
+ * Taper(domain, par) (propagator/src/Taper.h:8-42, taper.cu:7-77). */
+int fwix_taper_create(const fwix_dims* d, int taperx, int tapery, fwix_op* out);
+int fwix_taper_forward(fwix_op op, int add, const fwix_c64* model, fwix_c64* data, fwix_stream s);
+int fwix_taper_adjoint(fwix_op op, int add, fwix_c64* model, const fwix_c64* data, fwix_stream s);
+
+/* ---------------------------------------------------------------- OneStep: PSPI / NSPS
+ * PSPI / NSPS(domain, slow_hyper, par) (propagator/src/OneStep.h:11-111,
+ * PSPI.cpp, NSPS.cpp).  set_refs borrows device arrays for all depths:
+ * sref (nz, nref, NW) complex64 and labels (nz, NW, NY, NX) uint8 -- the
+ * reference's RefSampler output (RefSampler.h:29-36) resident in HBM instead
+ * of copied host-to-device at every set_depth (OneStep.h:42-45).
+ * forward: data (+)= P_z model; adjoint: model (+)= P_z^H data.  model == data
+ * runs the step in place (PSPI::cu_forward(model) / cu_adjoint(data),
+ * PSPI.cpp:66-103).  The PSPI adjoint always includes the taper (the
+ * reference's non-in-place adjoint omits it, PSPI.cpp:26-44) and the NSPS
+ * adjoint is the true adjoint (NSPS.cpp:16 passes swapped arguments). */
+int fwix_onestep_create(int method, const fwix_dims* d, const fwix_params* p, fwix_op* out);
+int fwix_onestep_set_refs(fwix_op op, const fwix_c64* sref_dev, const uint8_t* labels_dev);
+int fwix_onestep_set_depth(fwix_op op, int iz);
+int fwix_onestep_forward(fwix_op op, int add, const fwix_c64* model, fwix_c64* data, fwix_stream s);
+int fwix_onestep_adjoint(fwix_op op, int add, fwix_c64* model, const fwix_c64* data, fwix_stream s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FWIX_HIP_H */

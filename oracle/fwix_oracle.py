@@ -1,0 +1,517 @@
+"""FP64 numpy restatement of pyfwix's one-way depth-stepping arithmetic.
+
+TEST INFRASTRUCTURE ONLY -- imported by ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py``; never by the product package.
+
+Array layouts (C order, x fastest -- the reference's ``ND_TO_FLAT`` layout,
+``operator/src/complex_vector.h:10-20``):
+
+* wavefield        ``(NS, NW, NY, NX)``  padded lateral grid, NX = nx + padx
+* reference slow.  ``(nref, NW)`` per depth, ``(nz, nref, NW)`` for a model
+* labels           ``(NW, NY, NX)`` per depth (int), ``(nz, NW, NY, NX)``
+* slowness slice   ``(NW, ny, nx)`` unpadded, ``(nz, NW, ny, nx)`` for a model
+* extended image   ``(NW, ny, nx)`` per depth, ``(nz, NW, ny, nx)`` for a model
+* traces           ``(ntrace, NW)``  (frequency fastest, ``injection.cu:26-27``)
+
+Every function takes ``dtype``; float64/complex128 is the oracle, complex64
+with ``workers>1`` is the CPU baseline timed by the benchmark.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import scipy.fft as sfft
+
+__all__ = [
+    "Grid", "kgrid", "w2grid", "wgrid", "fft2", "ifft2", "dot",
+    "phase_shift_mult", "phase_shift", "taper_weights", "taper", "select",
+    "pspi_forward", "pspi_adjoint", "nsps_forward", "nsps_adjoint",
+    "pad_slow", "scatter_forward", "scatter_adjoint", "ic_forward", "ic_adjoint",
+    "injection_weights", "injection_forward", "injection_adjoint",
+    "downward_forward", "downward_adjoint", "upward_forward", "upward_adjoint",
+    "model_background", "born_forward", "born_adjoint", "migrate",
+    "SCATTER_COEF", "set_fft_workers",
+]
+
+_WORKERS = 1
+
+
+def set_fft_workers(n: int) -> None:
+    """Number of threads ``scipy.fft`` uses (1 for the oracle, all cores for the baseline)."""
+    global _WORKERS
+    _WORKERS = int(n)
+
+
+@dataclass
+class Grid:
+    """Wavefield hypercube: padded lateral grid, frequencies, shots, depth axis.
+
+    Mirrors the wavefield hypercube the reference builds from the slowness
+    axes plus padding (``propagator/src/Propagator.cpp:61-73``: axes 1-2 keep
+    the slowness origin/spacing, ``n += pad`` at the high side).
+    """
+    NX: int
+    NY: int
+    NW: int
+    NS: int
+    nx: int
+    ny: int
+    ox: float = 0.0
+    dx: float = 1.0
+    oy: float = 0.0
+    dy: float = 1.0
+    ow: float = 1.0
+    dw: float = 1.0
+    oz: float = 0.0
+    dz: float = 1.0
+    nz: int = 1
+
+
+# --------------------------------------------------------------------------- tables
+def kgrid(n: int, d: float) -> np.ndarray:
+    """Wavenumbers of an n-point axis with spacing d (``propagator/src/PhaseShift.h:43-57``).
+
+    k[i] = i*dk for i <= n/2, (i-n)*dk otherwise, dk = 2*pi/(d*n).
+    """
+    dk = 2.0 * math.pi / (d * n)
+    i = np.arange(n, dtype=np.float64)
+    return np.where(i <= n // 2, i, i - n) * dk
+
+
+def w2grid(nw: int, ow: float, dw: float) -> np.ndarray:
+    """Squared angular frequencies (2*pi*f)^2, f = ow + i*dw (``PhaseShift.h:59-70``)."""
+    f = ow + np.arange(nw, dtype=np.float64) * dw
+    return (2.0 * math.pi * f) ** 2
+
+
+def wgrid(nw: int, ow: float, dw: float) -> np.ndarray:
+    """Angular frequencies 2*pi*f; zero frequency is an error (``propagator/src/Scatter.h:91-103``)."""
+    f = ow + np.arange(nw, dtype=np.float64) * dw
+    if np.any(f == 0.0):
+        raise RuntimeError("Frequency is zero in the scattering operator!")
+    return 2.0 * math.pi * f
+
+
+# --------------------------------------------------------------------------- FFT
+def fft2(u: np.ndarray) -> np.ndarray:
+    """Unitary 2-D DFT over (y, x) with e^{-i} kernel.
+
+    ``operator/src/FFT.cpp:22-28`` (cufftPlanMany rank 2, dims {NY, NX}) with the
+    store callback scaling by rsqrt(NX*NY) (``operator/src/fft_callback.cu:7-11``).
+    """
+    return sfft.fft2(u, axes=(-2, -1), norm="ortho", workers=_WORKERS)
+
+
+def ifft2(u: np.ndarray) -> np.ndarray:
+    """Inverse (= adjoint) of :func:`fft2` (``FFT.cpp:45-55,62-69``, CUFFT_INVERSE + rsqrt scaling)."""
+    return sfft.ifft2(u, axes=(-2, -1), norm="ortho", workers=_WORKERS)
+
+
+def dot(a: np.ndarray, b: np.ndarray) -> complex:
+    """<a, b> = sum conj(a) * b in float64 (the build's convention for SEPlib ``dot``)."""
+    return complex(np.vdot(a.astype(np.complex128).ravel(), b.astype(np.complex128).ravel()))
+
+
+# --------------------------------------------------------------------------- phase shift
+def phase_shift_mult(sref_w: np.ndarray, kx: np.ndarray, ky: np.ndarray, w2: np.ndarray,
+                     dz: float, eps: float, mode: str = "fwd", dtype=np.complex128) -> np.ndarray:
+    """Per-(w, ky, kx) phase-shift multiplier for one reference slowness.
+
+    ``propagator/src/phase_shift.cu``:
+      a = w2*Re(s) - (kx^2+ky^2), b = w2*(Im(s) - eps*Re(s)), c = sqrt(a^2+b^2),
+      kr = sqrt((c+a)/2), ki = sqrt((c-a)/2)
+    * ``fwd`` (``:7-61``): exp(-ki*dz) * exp(-i*kr*dz)  (no branch on b)
+    * ``adj`` (``:125-184``): kr -> -kr when b > 0;  exp(-ki*dz) * exp(+i*kr*dz)
+    * ``inv`` (``:248-305``): kr -> -kr when b > 0;  exp(+ki*dz) * exp(+i*kr*dz)
+    Returns an array of shape (NW, NY, NX).
+    """
+    rdt = np.float64 if dtype == np.complex128 else np.float32
+    s = np.asarray(sref_w).astype(dtype)
+    sre = s.real.astype(rdt)[:, None, None]
+    sim = s.imag.astype(rdt)[:, None, None]
+    w2 = np.asarray(w2, dtype=rdt)[:, None, None]
+    k2 = (np.asarray(kx, rdt)[None, None, :] ** 2 + np.asarray(ky, rdt)[None, :, None] ** 2)
+    a = w2 * sre - k2
+    b = w2 * (sim - rdt(eps) * sre)
+    c = np.sqrt(a * a + b * b)
+    kr = np.sqrt(np.maximum((c + a) / 2, 0))
+    ki = np.sqrt(np.maximum((c - a) / 2, 0))
+    dz = rdt(dz)
+    if mode == "fwd":
+        return (np.exp(-ki * dz) * np.exp(-1j * kr * dz)).astype(dtype)
+    krs = np.where(b <= 0, kr, -kr)
+    if mode == "adj":
+        return (np.exp(-ki * dz) * np.exp(1j * krs * dz)).astype(dtype)
+    if mode == "inv":
+        return (np.exp(ki * dz) * np.exp(1j * krs * dz)).astype(dtype)
+    raise ValueError(mode)
+
+
+def phase_shift(m: np.ndarray, sref_w, kx, ky, w2, dz, eps, mode="fwd", d=None, add=False):
+    """PhaseShift operator: ``d (+)= Phi * m`` broadcast over shots (``propagator/src/PhaseShift.cpp:32-47``)."""
+    out = phase_shift_mult(sref_w, kx, ky, w2, dz, eps, mode, m.dtype)[None] * m
+    return out + d if (add and d is not None) else out
+
+
+# --------------------------------------------------------------------------- taper / select
+def taper_weights(NX: int, NY: int, tx: int, ty: int, dtype=np.float64) -> np.ndarray:
+    """Cosine edge taper weights (NY, NX) (``propagator/src/taper.cu:7-77``).
+
+    w(i) = 0.5(1-cos(pi*i/t)) for i < t; 0.5(1+cos(pi*(i-N+t)/t)) for i >= N-t; else 1.
+    Both edges of the padded grid; the product wx*wy is applied.
+    """
+    def w1(n, t):
+        i = np.arange(n, dtype=np.float64)
+        w = np.ones(n)
+        if t > 0:
+            lo = i < t
+            hi = (~lo) & (i >= n - t)
+            w[lo] = 0.5 * (1 - np.cos(np.pi * i[lo] / t))
+            w[hi] = 0.5 * (1 + np.cos(np.pi * (i[hi] - n + t) / t))
+        return w
+    return (w1(NY, ty)[:, None] * w1(NX, tx)[None, :]).astype(dtype)
+
+
+def taper(m: np.ndarray, tx: int, ty: int) -> np.ndarray:
+    """Taper operator (self-adjoint), ``propagator/src/Taper.h:30-37``."""
+    rdt = np.float64 if m.dtype == np.complex128 else np.float32
+    return m * taper_weights(m.shape[-1], m.shape[-2], tx, ty, rdt)
+
+
+def select(m: np.ndarray, labels: np.ndarray, r: int) -> np.ndarray:
+    """Selector: ``m * [labels == r]`` broadcast over shots (``propagator/src/selector.cu:7-37``)."""
+    return m * (labels == r)[None]
+
+
+# --------------------------------------------------------------------------- one step
+def _tables(g: Grid, dtype):
+    return kgrid(g.NX, g.dx), kgrid(g.NY, g.dy), w2grid(g.NW, g.ow, g.dw)
+
+
+def pspi_forward(m, sref, labels, g: Grid, dz, eps=0.0, tx=0, ty=0, d=None, add=False):
+    """PSPI forward step ``d (+)= sum_r M_r F^-1 [Phi_r F (T m)]``.
+
+    ``propagator/src/PSPI.cpp:6-24`` (non-in-place) and ``:66-83`` (in place,
+    ``d = m``): taper, FFT, then per reference: phase shift (forward), inverse FFT,
+    select-accumulate.  ``sref`` is (nref, NW), ``labels`` (NW, NY, NX).
+    """
+    kx, ky, w2 = _tables(g, m.dtype)
+    k = fft2(taper(m, tx, ty))
+    out = np.zeros_like(m)
+    for r in range(sref.shape[0]):
+        phi = phase_shift_mult(sref[r], kx, ky, w2, dz, eps, "fwd", m.dtype)
+        out += select(ifft2(phi[None] * k), labels, r)
+    return out + d if (add and d is not None) else out
+
+
+def pspi_adjoint(d, sref, labels, g: Grid, dz, eps=0.0, tx=0, ty=0, m=None, add=False,
+                 taper_adjoint=True):
+    """PSPI adjoint step ``m (+)= T F^-1 [sum_r Phi_r^adj F (M_r d)]``.
+
+    ``propagator/src/PSPI.cpp:85-103`` (in place: ends with the taper).  The
+    reference's non-in-place adjoint (``PSPI.cpp:26-44``) omits the taper, so it
+    is the adjoint of the forward only when taperx = tapery = 0;
+    ``taper_adjoint=False`` restates that variant.
+    """
+    kx, ky, w2 = _tables(g, d.dtype)
+    acc = np.zeros_like(d)
+    for r in range(sref.shape[0]):
+        phi = phase_shift_mult(sref[r], kx, ky, w2, dz, eps, "adj", d.dtype)
+        acc += phi[None] * fft2(select(d, labels, r))
+    out = ifft2(acc)
+    if taper_adjoint:
+        out = taper(out, tx, ty)
+    return out + m if (add and m is not None) else out
+
+
+def nsps_forward(m, sref, labels, g: Grid, dz, eps=0.0, d=None, add=False):
+    """NSPS forward ``d (+)= F^-1 [sum_r Phi_r F (M_r m)]`` (``propagator/src/NSPS.cpp:27-45``; no taper)."""
+    kx, ky, w2 = _tables(g, m.dtype)
+    acc = np.zeros_like(m)
+    for r in range(sref.shape[0]):
+        phi = phase_shift_mult(sref[r], kx, ky, w2, dz, eps, "fwd", m.dtype)
+        acc += phi[None] * fft2(select(m, labels, r))
+    out = ifft2(acc)
+    return out + d if (add and d is not None) else out
+
+
+def nsps_adjoint(d, sref, labels, g: Grid, dz, eps=0.0, m=None, add=False):
+    """Adjoint of :func:`nsps_forward`: ``m (+)= sum_r M_r F^-1 [Phi_r^adj F d]``.
+
+    The reference's ``NSPS::cu_adjoint`` (``propagator/src/NSPS.cpp:6-25``) passes
+    swapped arguments to ``ps->cu_adjoint`` (``:16``) and is not an adjoint; this is
+    the corrected operator (pinned by the dot-product test).
+    """
+    kx, ky, w2 = _tables(g, d.dtype)
+    k = fft2(d)
+    out = np.zeros_like(d)
+    for r in range(sref.shape[0]):
+        phi = phase_shift_mult(sref[r], kx, ky, w2, dz, eps, "adj", d.dtype)
+        out += select(ifft2(phi[None] * k), labels, r)
+    return out + m if (add and m is not None) else out
+
+
+# --------------------------------------------------------------------------- scatter
+SCATTER_COEF = (1.0, 1.0 / 2.0, 3.0 / 8.0, 5.0 / 16.0, 35.0 / 128.0)  # Scatter.h:73
+
+
+def pad_slow(s: np.ndarray, NX: int, NY: int) -> np.ndarray:
+    """Edge-replicating pad (NW, ny, nx) -> (NW, NY, NX) (``propagator/src/scatter.cu:240-277``)."""
+    nw, ny, nx = s.shape
+    iy = np.minimum(np.arange(NY), ny - 1)
+    ix = np.minimum(np.arange(NX), nx - 1)
+    return s[:, iy][:, :, ix]
+
+
+def _powc(z: np.ndarray, p: float) -> np.ndarray:
+    """z^p = |z|^p e^{i p atan2(Im z, Re z)} (``propagator/src/scatter.cu:6-21``)."""
+    return np.abs(z) ** p * np.exp(1j * p * np.arctan2(z.imag, z.real))
+
+
+def _scatter_terms(slow_z, g: Grid, eps_s, ntaylor, dtype):
+    kx, ky = kgrid(g.NX, g.dx), kgrid(g.NY, g.dy)
+    w = wgrid(g.NW, g.ow, g.dw)
+    kk = np.sqrt(kx[None, None, :] ** 2 + ky[None, :, None] ** 2)
+    sp = pad_slow(np.asarray(slow_z).astype(np.complex128), g.NX, g.NY)
+    st = sp.real + 1j * (sp.imag - eps_s * sp.real)
+    terms = []
+    for it in range(ntaylor + 1):
+        kfac = (kk / w[:, None, None]) ** (2 * it)           # scatter.cu:23-59
+        sfac = SCATTER_COEF[it] * _powc(st, -0.5 - it)        # scatter.cu:61-106
+        terms.append((kfac.astype(dtype), sfac.astype(dtype)))
+    return terms, w
+
+
+def scatter_forward(m, slow_z, g: Grid, dz, eps_s=0.04, ntaylor=1, d=None, add=False):
+    """Born scattering ``d (+)= -i(w dz/2) sum_it c_it s~^(-1/2-it) F^-1[(|k|/w)^(2it) F m]``.
+
+    ``propagator/src/Scatter.cpp:90-102`` with kernels ``scatter.cu:23-195``;
+    s~ = Re s + i(Im s - eps_s Re s) of the edge-padded slice (``Scatter.cpp:68-74``).
+    """
+    terms, w = _scatter_terms(slow_z, g, eps_s, ntaylor, m.dtype)
+    fm = fft2(m)
+    out = np.zeros_like(m)
+    iw = (-0.5j * w * dz).astype(m.dtype)[None, :, None, None]   # scale_by_iw_fwd
+    for kfac, sfac in terms:
+        out += iw * (sfac[None] * ifft2(kfac[None] * fm))
+    return out + d if (add and d is not None) else out
+
+
+def scatter_adjoint(d, slow_z, g: Grid, dz, eps_s=0.04, ntaylor=1, m=None, add=False):
+    """Adjoint of :func:`scatter_forward` (``propagator/src/Scatter.cpp:104-116``; ``scatter.cu:108-153,197-238``)."""
+    terms, w = _scatter_terms(slow_z, g, eps_s, ntaylor, d.dtype)
+    iw = (0.5j * w * dz).astype(d.dtype)[None, :, None, None]    # scale_by_iw_adj
+    out = np.zeros_like(d)
+    v = iw * d
+    for kfac, sfac in terms:
+        out += ifft2(kfac[None] * fft2(np.conj(sfac)[None] * v))
+    return out + m if (add and m is not None) else out
+
+
+# --------------------------------------------------------------------------- imaging condition
+def ic_forward(img, U, d=None, add=False):
+    """``D[s,w,y,x] (+)= U[s,w,y,x] * img[w,y,x]`` on the unpadded x<nx, y<ny region.
+
+    ``propagator/src/imaging_condition.cu:9-55``; the padding of D is left untouched
+    (zero when add is false).
+    """
+    nw, ny, nx = img.shape
+    out = np.zeros_like(U) if not (add and d is not None) else d.copy()
+    out[:, :, :ny, :nx] += U[:, :, :ny, :nx] * img[None]
+    return out
+
+
+def ic_adjoint(D, U, nx, ny, img=None, add=False):
+    """``img[w,y,x] (+)= sum_s conj(U) * D`` (``propagator/src/imaging_condition.cu:57-108``)."""
+    out = np.sum(np.conj(U[:, :, :ny, :nx]) * D[:, :, :ny, :nx], axis=0)
+    return out + img if (add and img is not None) else out
+
+
+# --------------------------------------------------------------------------- injection
+def injection_weights(cx, cy, cz, ids, g: Grid, iz_inj: int):
+    """Per-trace bilinear-in-(x,y) x linear-in-z weights at depth step ``iz_inj``.
+
+    ``propagator/src/injection.cu:38-75``.  Index and fraction arithmetic is done in
+    float32, like the kernel, so that both pick the same cells.  Returns a list
+    of (trace, shot, iy, ix, [w00, w01, w10, w11]) for traces active at this step
+    (corners (iy,ix), (iy,ix+1), (iy+1,ix), (iy+1,ix+1)).
+    """
+    f32 = np.float32
+    out = []
+    for t in range(len(cx)):
+        fy = (f32(cy[t]) - f32(g.oy)) / f32(g.dy)
+        iy = int(np.floor(fy))
+        ly = f32(1) - (f32(cy[t]) - (f32(g.oy) + f32(iy) * f32(g.dy))) / f32(g.dy)
+        fx = (f32(cx[t]) - f32(g.ox)) / f32(g.dx)
+        ix = int(np.floor(fx))
+        lx = f32(1) - (f32(cx[t]) - (f32(g.ox) + f32(ix) * f32(g.dx))) / f32(g.dx)
+        fz = (f32(cz[t]) - f32(g.oz)) / f32(g.dz)
+        izc = int(np.floor(fz))
+        lz = f32(1) - (f32(cz[t]) - (f32(g.oz) + f32(izc) * f32(g.dz))) / f32(g.dz)
+        if ix < 0 or ix >= g.NX - 1 or iy < 0 or iy >= g.NY - 1:
+            continue
+        if iz_inj == izc:
+            f = lz
+        elif iz_inj == izc + 1:
+            f = f32(1) - lz
+        else:
+            continue
+        w = [f * lx * ly, f * (f32(1) - lx) * ly, f * lx * (f32(1) - ly),
+             f * (f32(1) - lx) * (f32(1) - ly)]
+        out.append((t, int(ids[t]), iy, ix, [float(v) for v in w]))
+    return out
+
+
+def injection_forward(traces, cx, cy, cz, ids, g: Grid, iz_inj, wfld=None, add=False, shape=None):
+    """Scatter traces (ntrace, NW) into the wavefield (``propagator/src/injection.cu:7-96``).
+
+    The reference kernel's read-modify-write races when traces share nodes
+    (``injection.cu:89-92``); the restatement is the race-free sum.
+    """
+    out = np.zeros(shape, traces.dtype) if not (add and wfld is not None) else wfld.copy()
+    for t, s, iy, ix, w in injection_weights(cx, cy, cz, ids, g, iz_inj):
+        out[s, :, iy, ix] += w[0] * traces[t]
+        out[s, :, iy, ix + 1] += w[1] * traces[t]
+        out[s, :, iy + 1, ix] += w[2] * traces[t]
+        out[s, :, iy + 1, ix + 1] += w[3] * traces[t]
+    return out
+
+
+def injection_adjoint(wfld, cx, cy, cz, ids, g: Grid, iz_inj, traces=None, add=False):
+    """Gather the wavefield into traces (``propagator/src/injection.cu:98-187``)."""
+    out = (np.zeros((len(cx), wfld.shape[1]), wfld.dtype) if not (add and traces is not None)
+           else traces.copy())
+    for t, s, iy, ix, w in injection_weights(cx, cy, cz, ids, g, iz_inj):
+        out[t] += (w[0] * wfld[s, :, iy, ix] + w[1] * wfld[s, :, iy, ix + 1]
+                   + w[2] * wfld[s, :, iy + 1, ix] + w[3] * wfld[s, :, iy + 1, ix + 1])
+    return out
+
+
+# --------------------------------------------------------------------------- recursions
+def _step_fwd(u, z, sref, labels, g, dz, eps, tx, ty, method):
+    if method == "nsps":
+        return nsps_forward(u, sref[z], labels[z], g, dz, eps)
+    return pspi_forward(u, sref[z], labels[z], g, dz, eps, tx, ty)
+
+
+def _step_adj(u, z, sref, labels, g, dz, eps, tx, ty, method):
+    if method == "nsps":
+        return nsps_adjoint(u, sref[z], labels[z], g, dz, eps)
+    return pspi_adjoint(u, sref[z], labels[z], g, dz, eps, tx, ty)
+
+
+def downward_forward(m, sref, labels, g, dz, eps=0.0, tx=0, ty=0, method="pspi", checkpoints=None):
+    """``Downward::cu_forward`` (``propagator/src/OneWay.cpp:22-36``): u <- P_{nz-1} ... P_0 m.
+
+    ``checkpoints`` (a list) receives U_z before each step, as ``compress_slice``
+    stores it.  (The reference's 1-argument host wrapper aliases model and data
+    and returns 2*P^nz u, ``operator/src/CudaOperator.h:141-151``; this is P^nz u.)
+    """
+    u = m.copy()
+    for z in range(sref.shape[0]):
+        if checkpoints is not None:
+            checkpoints.append(u.copy())
+        u = _step_fwd(u, z, sref, labels, g, dz, eps, tx, ty, method)
+    return u
+
+
+def downward_adjoint(d, sref, labels, g, dz, eps=0.0, tx=0, ty=0, method="pspi"):
+    """``Downward::cu_adjoint`` (``OneWay.cpp:38-52``): u <- P_0^adj ... P_{nz-1}^adj d."""
+    u = d.copy()
+    for z in range(sref.shape[0] - 1, -1, -1):
+        u = _step_adj(u, z, sref, labels, g, dz, eps, tx, ty, method)
+    return u
+
+
+def upward_forward(m, sref, labels, g, dz, eps=0.0, tx=0, ty=0, method="pspi"):
+    """``Upward::cu_forward`` (``OneWay.cpp:54-67``): steps z = nz-1 ... 0."""
+    u = m.copy()
+    for z in range(sref.shape[0] - 1, -1, -1):
+        u = _step_fwd(u, z, sref, labels, g, dz, eps, tx, ty, method)
+    return u
+
+
+def upward_adjoint(d, sref, labels, g, dz, eps=0.0, tx=0, ty=0, method="pspi"):
+    """``Upward::cu_adjoint`` (``OneWay.cpp:69-79``): adjoint steps z = 0 ... nz-1."""
+    u = d.copy()
+    for z in range(sref.shape[0]):
+        u = _step_adj(u, z, sref, labels, g, dz, eps, tx, ty, method)
+    return u
+
+
+def model_background(wavelet, src, rec, sref, labels, g: Grid, dz, eps=0.0, tx=0, ty=0):
+    """Background down-going modeling loop (``propagator/src/Propagator.cpp:155-180``).
+
+    For z = 0..nz-1: U += Inj_z(wavelet); d += Rec_z^T U; checkpoint U_z; U <- P_z U.
+    ``src``/``rec`` are (cx, cy, cz, ids) tuples.  Returns (data, [U_z]).
+    """
+    nz = sref.shape[0]
+    shape = (g.NS, g.NW, g.NY, g.NX)
+    U = np.zeros(shape, wavelet.dtype)
+    data = np.zeros((len(rec[0]), g.NW), wavelet.dtype)
+    ck = []
+    for z in range(nz):
+        U = injection_forward(wavelet, *src, g, z, wfld=U, add=True, shape=shape)
+        data = injection_adjoint(U, *rec, g, z, traces=data, add=True)
+        ck.append(U.copy())
+        U = pspi_forward(U, sref[z], labels[z], g, dz, eps, tx, ty)
+    return data, ck
+
+
+def born_forward(dslow, bg, slow, rec, sref, labels, g: Grid, dz, eps=0.0, tx=0, ty=0,
+                 eps_s=0.04, ntaylor=1):
+    """Born down-scattering channel (``propagator/src/ExtendedBorn.cpp:152-178``).
+
+    dU_0 = 0; for z: d += Rec_z^T dU_z; dU_{z+1} = P_z dU_z + P_z S_z(U_z * ds_z)
+    (``Scattering.cpp:43-52``: IC forward, in-place scatter, non-in-place PSPI).
+    ``dslow``/``slow`` are (nz, NW, ny, nx); ``bg`` the list of U_z.
+    """
+    nz = sref.shape[0]
+    dU = np.zeros_like(bg[0])
+    data = np.zeros((len(rec[0]), g.NW), dU.dtype)
+    for z in range(nz):
+        data = injection_adjoint(dU, *rec, g, z, traces=data, add=True)
+        sc = scatter_forward(ic_forward(dslow[z], bg[z]), slow[z], g, dz, eps_s, ntaylor)
+        dU = (pspi_forward(dU, sref[z], labels[z], g, dz, eps, tx, ty)
+              + pspi_forward(sc, sref[z], labels[z], g, dz, eps, tx, ty))
+    return data
+
+
+def born_adjoint(data, bg, slow, rec, sref, labels, g: Grid, dz, eps=0.0, tx=0, ty=0,
+                 eps_s=0.04, ntaylor=1, taper_adjoint=True):
+    """Adjoint of :func:`born_forward` (``propagator/src/ExtendedBorn.cpp:180-208``).
+
+    lam = 0; for z = nz-1..0: ds_z = IC^adj S_z^adj P~_z^adj lam; lam <- P_z^adj lam;
+    lam += Inj_z(d).  ``taper_adjoint=False`` restates the reference's non-in-place
+    PSPI adjoint without taper inside P~ (``Scattering.cpp:54-61``, ``PSPI.cpp:26-44``).
+    """
+    nz = sref.shape[0]
+    lam = np.zeros_like(bg[0])
+    shape = lam.shape
+    out = np.zeros((nz, g.NW, g.ny, g.nx), lam.dtype)
+    for z in range(nz - 1, -1, -1):
+        mu = pspi_adjoint(lam, sref[z], labels[z], g, dz, eps, tx, ty, taper_adjoint=taper_adjoint)
+        out[z] = ic_adjoint(scatter_adjoint(mu, slow[z], g, dz, eps_s, ntaylor), bg[z], g.nx, g.ny)
+        lam = pspi_adjoint(lam, sref[z], labels[z], g, dz, eps, tx, ty)
+        lam = injection_forward(data, *rec, g, z, wfld=lam, add=True, shape=shape)
+    return out
+
+
+def migrate(data, bg, rec, sref, labels, g: Grid, dz, eps=0.0, tx=0, ty=0):
+    """Shot-profile migration image ``I_z = sum_w sum_s conj(U_z) * P_z^adj lam_{z+1}``.
+
+    The imaging-condition core of the Born adjoint (``imaging_condition.cu:57-108``
+    summed over frequency) with the receiver wavefield back-propagated as in
+    ``ExtendedBorn.cpp:180-208``.  Returns (nz, ny, nx).
+    """
+    nz = sref.shape[0]
+    lam = np.zeros_like(bg[0])
+    shape = lam.shape
+    img = np.zeros((nz, g.ny, g.nx), lam.dtype)
+    for z in range(nz - 1, -1, -1):
+        lam = pspi_adjoint(lam, sref[z], labels[z], g, dz, eps, tx, ty)
+        img[z] = ic_adjoint(lam, bg[z], g.nx, g.ny).sum(axis=0)
+        lam = injection_forward(data, *rec, g, z, wfld=lam, add=True, shape=shape)
+    return img

@@ -1,0 +1,133 @@
+"""ctypes binding of libpyfwix_hip.so (the C ABI declared in include/fwix_hip.h).
+
+The product path has no CPU fallback: importing this module when the shared
+library is missing raises ImportError, and every failing call raises
+RuntimeError with the library's own message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+_LIB_PATH = Path(os.environ.get("FWIX_LIB", Path(__file__).resolve().parent / "libpyfwix_hip.so"))
+
+if not _LIB_PATH.exists():
+    raise ImportError(
+        f"{_LIB_PATH} is missing: build it with `python -m pyfwix_amd.build` "
+        "(there is no CPU fallback for the depth-stepping operators)")
+
+lib = C.CDLL(str(_LIB_PATH))
+
+FWIX_PSPI = 0
+FWIX_NSPS = 1
+
+
+class fwix_dims(C.Structure):
+    _fields_ = [("NX", C.c_int), ("NY", C.c_int), ("NW", C.c_int), ("NS", C.c_int),
+                ("nx", C.c_int), ("ny", C.c_int),
+                ("ox", C.c_float), ("dx", C.c_float), ("oy", C.c_float), ("dy", C.c_float),
+                ("ow", C.c_float), ("dw", C.c_float), ("oz", C.c_float), ("dz", C.c_float),
+                ("nz", C.c_int)]
+
+
+class fwix_params(C.Structure):
+    _fields_ = [("nref", C.c_int), ("taperx", C.c_int), ("tapery", C.c_int),
+                ("eps", C.c_float), ("eps_scat", C.c_float), ("taylor", C.c_int),
+                ("planes_per_chunk", C.c_int)]
+
+
+_P = C.c_void_p
+_I = C.c_int
+_F = C.c_float
+_OPP = C.POINTER(C.c_void_p)
+_D = C.POINTER(fwix_dims)
+_PP = C.POINTER(fwix_params)
+
+# name -> (argtypes); all return int status except where noted
+_SIGS = {
+    "fwix_op_destroy": [_P],
+    "fwix_fft2d_create": [_D, _OPP],
+    "fwix_fft2d_forward": [_P, _I, _P, _P, _P],
+    "fwix_fft2d_adjoint": [_P, _I, _P, _P, _P],
+    "fwix_phase_shift_create": [_D, _F, _F, _OPP],
+    "fwix_phase_shift_set_slow": [_P, _P],
+    "fwix_phase_shift_forward": [_P, _I, _P, _P, _P],
+    "fwix_phase_shift_adjoint": [_P, _I, _P, _P, _P],
+    "fwix_phase_shift_inverse": [_P, _I, _P, _P, _P],
+    "fwix_selector_create": [_D, _OPP],
+    "fwix_selector_set_labels": [_P, _P],
+    "fwix_selector_set_value": [_P, _I],
+    "fwix_selector_forward": [_P, _I, _P, _P, _P],
+    "fwix_selector_adjoint": [_P, _I, _P, _P, _P],
+    "fwix_taper_create": [_D, _I, _I, _OPP],
+    "fwix_taper_forward": [_P, _I, _P, _P, _P],
+    "fwix_taper_adjoint": [_P, _I, _P, _P, _P],
+    "fwix_onestep_create": [_I, _D, _PP, _OPP],
+    "fwix_onestep_set_refs": [_P, _P, _P],
+    "fwix_onestep_set_depth": [_P, _I],
+    "fwix_onestep_forward": [_P, _I, _P, _P, _P],
+    "fwix_onestep_adjoint": [_P, _I, _P, _P, _P],
+}
+
+EXPORTS = ["fwix_version", "fwix_last_error", "fwix_params_default", "fwix_fft_size_supported",
+           *_SIGS.keys()]
+
+for _name, _args in _SIGS.items():
+    _fn = getattr(lib, _name, None)
+    if _fn is not None:
+        _fn.argtypes = _args
+        _fn.restype = C.c_int
+
+lib.fwix_version.restype = C.c_int
+lib.fwix_last_error.restype = C.c_char_p
+lib.fwix_params_default.argtypes = [_PP]
+lib.fwix_params_default.restype = None
+lib.fwix_fft_size_supported.argtypes = [_I]
+lib.fwix_fft_size_supported.restype = C.c_int
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise RuntimeError(lib.fwix_last_error().decode(errors="replace"))
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib, name)(*args))
+
+
+def create(name: str, *args) -> C.c_void_p:
+    h = C.c_void_p()
+    check(getattr(lib, name)(*args, C.byref(h)))
+    return h
+
+
+class Handle:
+    """Owns one operator handle; destroyed with the Python object."""
+
+    def __init__(self, h: C.c_void_p):
+        self.h = h
+
+    def __del__(self):
+        h, self.h = getattr(self, "h", None), None
+        if h and lib is not None:
+            lib.fwix_op_destroy(h)
+
+
+def default_params(**kw) -> fwix_params:
+    p = fwix_params()
+    lib.fwix_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def make_dims(NX, NY, NW, NS, nx=None, ny=None, ox=0.0, dx=1.0, oy=0.0, dy=1.0,
+              ow=1.0, dw=1.0, oz=0.0, dz=1.0, nz=1) -> fwix_dims:
+    return fwix_dims(int(NX), int(NY), int(NW), int(NS), int(nx or NX), int(ny or NY),
+                     float(ox), float(dx), float(oy), float(dy), float(ow), float(dw),
+                     float(oz), float(dz), int(nz))
+
+
+def fft_size_supported(n: int) -> bool:
+    return bool(lib.fwix_fft_size_supported(int(n)))

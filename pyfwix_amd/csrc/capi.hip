@@ -1,0 +1,469 @@
+// C ABI of libpyfwix_hip.so: operator handles over the HIP kernels.
+// Declarations and reference citations: include/fwix_hip.h.
+#include "fwix_hip.h"
+#include "fwix_launch.h"
+#include "capi_internal.h"
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace fwix;
+
+namespace fwix {
+
+thread_local std::string g_err;
+
+void set_error(const std::string& s) { g_err = s; }
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void DevBuf::alloc(size_t bytes) {
+  release();
+  if (bytes == 0) return;
+  hip_check(hipMalloc(&p, bytes), "hipMalloc");
+  n = bytes;
+}
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  n = 0;
+}
+
+static void upload_bytes(DevBuf& b, const void* host, size_t bytes) {
+  b.alloc(bytes);
+  if (bytes) hip_check(hipMemcpy(b.p, host, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+static std::vector<float> k_table(int n, double d) {  // PhaseShift.h:43-57
+  std::vector<float> k(n);
+  const double dk = 2.0 * M_PI / (d * n);
+  for (int i = 0; i < n; ++i) k[i] = (float)((i <= n / 2 ? i : i - n) * dk);
+  return k;
+}
+
+static std::vector<float> taper_table(int n, int t) {  // taper.cu:43-55
+  std::vector<float> w(n, 1.f);
+  if (t > 0)
+    for (int i = 0; i < n; ++i) {
+      if (i < t) w[i] = (float)(0.5 * (1.0 - std::cos(M_PI * i / t)));
+      else if (i >= n - t) w[i] = (float)(0.5 * (1.0 + std::cos(M_PI * (i - n + t) / t)));
+    }
+  return w;
+}
+
+static std::vector<float2> twiddle_table(int n) {  // exp(-2 pi i k / n) in double, rounded once
+  std::vector<float2> t(n);
+  for (int k = 0; k < n; ++k) {
+    const double a = -2.0 * M_PI * (double)k / (double)n;
+    t[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  return t;
+}
+
+void validate_dims(const fwix_dims* d, bool need_fft) {
+  if (!d) throw Error("null dims");
+  if (d->NX < 1 || d->NY < 1 || d->NW < 1 || d->NS < 1) throw Error("dims: NX, NY, NW, NS must be >= 1");
+  if (d->nx < 1 || d->ny < 1 || d->nx > d->NX || d->ny > d->NY)
+    throw Error("dims: need 1 <= nx <= NX and 1 <= ny <= NY");
+  if (!(d->dx > 0.f) || !(d->dy > 0.f)) throw Error("dims: dx and dy must be > 0");
+  if (need_fft && (!fft_size_supported(d->NX) || !fft_size_supported(d->NY)))
+    throw Error("dims: NX=" + std::to_string(d->NX) + ", NY=" + std::to_string(d->NY) +
+                ": lateral FFT sizes must be 2^a or 3*2^a between 8 and 1024");
+  const double pts = (double)d->NX * d->NY * d->NW * d->NS;
+  if (pts > 9.0e18 / 8) throw Error("dims: wavefield too large");
+}
+
+void validate_params(const fwix_params* p) {
+  if (!p) throw Error("null params");
+  if (p->nref < 1 || p->nref > 255) throw Error("params: nref must be in [1, 255]");
+  if (p->taperx < 0 || p->tapery < 0) throw Error("params: taperx/tapery must be >= 0");
+  if (p->taylor < 0 || p->taylor > 4) throw Error("params: taylor must be in [0, 4]");
+  if (p->planes_per_chunk < 0) throw Error("params: planes_per_chunk must be >= 0");
+}
+
+void Tables::init(const fwix_dims& d, int tx, int ty) {
+  if (tx > d.NX || ty > d.NY) throw Error("taper wider than the padded grid");
+  auto kx = k_table(d.NX, d.dx), ky = k_table(d.NY, d.dy);
+  std::vector<float> w2(d.NW), w(d.NW);
+  for (int i = 0; i < d.NW; ++i) {  // PhaseShift.h:59-70, Scatter.h:91-103
+    const double f = 2.0 * M_PI * ((double)d.ow + (double)i * d.dw);
+    w2[i] = (float)(f * f);
+    w[i] = (float)f;
+  }
+  auto tx_ = taper_table(d.NX, tx), ty_ = taper_table(d.NY, ty);
+  upload_bytes(b_kx, kx.data(), kx.size() * 4);
+  upload_bytes(b_ky, ky.data(), ky.size() * 4);
+  upload_bytes(b_w2, w2.data(), w2.size() * 4);
+  upload_bytes(b_w, w.data(), w.size() * 4);
+  upload_bytes(b_tapx, tx_.data(), tx_.size() * 4);
+  upload_bytes(b_tapy, ty_.data(), ty_.size() * 4);
+  if (fft_size_supported(d.NX) && fft_size_supported(d.NY)) {
+    auto twx = twiddle_table(d.NX), twy = twiddle_table(d.NY);
+    upload_bytes(b_twx, twx.data(), twx.size() * 8);
+    upload_bytes(b_twy, twy.data(), twy.size() * 8);
+  }
+  dims = d;
+}
+
+KArgs Tables::kargs() const {
+  KArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.kx = (const float*)b_kx.p;
+  a.ky = (const float*)b_ky.p;
+  a.w2 = (const float*)b_w2.p;
+  a.w = (const float*)b_w.p;
+  a.tapx = (const float*)b_tapx.p;
+  a.tapy = (const float*)b_tapy.p;
+  a.twx = (const float2*)b_twx.p;
+  a.twy = (const float2*)b_twy.p;
+  a.NX = dims.NX; a.NY = dims.NY; a.NW = dims.NW; a.NS = dims.NS;
+  a.nx = dims.nx; a.ny = dims.ny;
+  a.dz = dims.dz;
+  return a;
+}
+
+PwArgs Tables::pwargs() const {
+  PwArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.NX = dims.NX; a.NY = dims.NY; a.NW = dims.NW; a.NS = dims.NS;
+  a.kx = (const float*)b_kx.p;
+  a.ky = (const float*)b_ky.p;
+  a.w2 = (const float*)b_w2.p;
+  a.tapx = (const float*)b_tapx.p;
+  a.tapy = (const float*)b_tapy.p;
+  return a;
+}
+
+int auto_chunk(const fwix_dims& d, int nterm, int requested) {
+  const int planes = d.NW * d.NS;
+  if (requested > 0) return requested < planes ? requested : planes;
+  // keep the chunk's intermediates (spectrum + nterm fan-out planes) around
+  // 128 MiB so that they stay in the 256 MB Infinity Cache between passes
+  const double per_plane = (double)(1 + nterm) * d.NX * d.NY * 8.0;
+  int c = (int)(128.0 * 1024 * 1024 / per_plane);
+  if (c < 1) c = 1;
+  return c < planes ? c : planes;
+}
+
+// ------------------------------------------------------------------ one step engine
+void StepEngine::init(int method_, const fwix_dims& d, const fwix_params& p) {
+  method = method_;
+  params = p;
+  tables.init(d, method == FWIX_PSPI ? p.taperx : 0, method == FWIX_PSPI ? p.tapery : 0);
+  chunk = auto_chunk(d, p.nref, p.planes_per_chunk);
+  const size_t plane = (size_t)d.NX * d.NY;
+  K.alloc(plane * chunk * sizeof(float2));
+  W.alloc(plane * chunk * p.nref * sizeof(float2));
+}
+
+void StepEngine::check_ready(int iz) const {
+  if (!sref || !labels) throw Error("reference slownesses / labels not set (call set_refs)");
+  if (iz < 0 || iz >= tables.dims.nz) throw Error("depth index out of range");
+}
+
+KArgs StepEngine::base(int iz, int c0, int np) const {
+  const fwix_dims& d = tables.dims;
+  KArgs a = tables.kargs();
+  a.K = (float2*)K.p;
+  a.W = (float2*)W.p;
+  a.labels = labels + (size_t)iz * d.NW * d.NY * d.NX;
+  a.sref = sref + (size_t)iz * params.nref * d.NW;
+  a.nref = params.nref;
+  a.nterm = params.nref;
+  a.p0 = c0;
+  a.np = np;
+  a.wstride = (long long)np * d.NX * d.NY;
+  a.eps = params.eps;
+  a.eps_s = params.eps_scat;
+  return a;
+}
+
+// out (+)= P_z in     (forward) or    out (+)= P_z^H in    (adjoint)
+void StepEngine::apply(bool adjoint, int iz, int add, const float2* in, float2* out, hipStream_t st) {
+  check_ready(iz);
+  const fwix_dims& d = tables.dims;
+  const size_t plane = (size_t)d.NX * d.NY;
+  const int planes = d.NW * d.NS;
+  const float sx = (float)(1.0 / std::sqrt((double)d.NX));
+  const float sy2 = (float)(1.0 / (double)d.NY);
+  const bool pspi = method == FWIX_PSPI;
+  const bool taper = pspi && (params.taperx > 0 || params.tapery > 0);
+  for (int c0 = 0; c0 < planes; c0 += chunk) {
+    const int np = (planes - c0) < chunk ? (planes - c0) : chunk;
+    KArgs a = base(iz, c0, np);
+    const float2* cin = in + (size_t)c0 * plane;
+    float2* cout = out + (size_t)c0 * plane;
+    // "shared spectrum" shape: PSPI forward and NSPS adjoint
+    if ((pspi && !adjoint) || (!pspi && adjoint)) {
+      a.in = cin; a.out = a.K; a.add = 0; a.scale = sx;
+      hip_check(launch_rows(taper ? 2 : 0, a, st), "rows pass");
+      a.scale = sy2;
+      hip_check(launch_cols_fanout(adjoint ? 1 : 0, a, st), "column fan-out pass");
+      a.out = cout; a.add = add; a.scale = sx;
+      hip_check(launch_rows_select(a, st), "rows select pass");
+    } else {  // "masked inputs" shape: PSPI adjoint and NSPS forward
+      a.in = cin; a.scale = sx;
+      hip_check(launch_rows_fanout(a, st), "rows fan-out pass");
+      a.scale = sy2;
+      hip_check(launch_cols_fanin(adjoint ? 1 : 0, a, st), "column fan-in pass");
+      a.in = a.K; a.out = cout; a.add = add; a.scale = sx;
+      hip_check(launch_rows(taper ? 3 : 1, a, st), "rows inverse pass");
+    }
+  }
+}
+
+}  // namespace fwix
+
+// =================================================================== C entry points
+namespace {
+
+struct FftOp : fwix_op_s {
+  Tables t;
+  DevBuf K;
+  int chunk = 1;
+  FftOp() { kind = KIND_FFT; }
+  void run(bool inv, int add, const float2* in, float2* out, hipStream_t st) {
+    const fwix_dims& d = t.dims;
+    const size_t plane = (size_t)d.NX * d.NY;
+    const int planes = d.NW * d.NS;
+    for (int c0 = 0; c0 < planes; c0 += chunk) {
+      const int np = (planes - c0) < chunk ? (planes - c0) : chunk;
+      KArgs a = t.kargs();
+      a.p0 = c0; a.np = np;
+      a.in = in + (size_t)c0 * plane; a.out = (float2*)K.p; a.add = 0;
+      a.scale = (float)(1.0 / std::sqrt((double)d.NX));
+      hip_check(launch_rows(inv ? 1 : 0, a, st), "fft rows");
+      a.in = (const float2*)K.p; a.out = out + (size_t)c0 * plane; a.add = add;
+      a.scale = (float)(1.0 / std::sqrt((double)d.NY));
+      hip_check(launch_cols(inv, a, st), "fft columns");
+    }
+  }
+};
+
+struct PhaseShiftOp : fwix_op_s {
+  Tables t;
+  DevBuf sref;
+  float dz = 0.f, eps = 0.f;
+  PhaseShiftOp() { kind = KIND_PHASE_SHIFT; }
+  void run(int mode, int add, const float2* in, float2* out, hipStream_t st) {
+    PwArgs a = t.pwargs();
+    a.sref = (const float2*)sref.p;
+    a.dz = dz; a.eps = eps; a.add = add;
+    hip_check(launch_phase_shift(mode, in, out, a, st), "phase shift");
+  }
+};
+
+struct SelectorOp : fwix_op_s {
+  Tables t;
+  DevBuf labels;
+  int value = 0;
+  bool has_labels = false;
+  SelectorOp() { kind = KIND_SELECTOR; }
+  void run(int add, const float2* in, float2* out, hipStream_t st) {
+    if (!has_labels) throw Error("selector labels not set");
+    PwArgs a = t.pwargs();
+    a.labels = (const uint8_t*)labels.p;
+    a.value = value; a.add = add;
+    hip_check(launch_select(in, out, a, st), "select");
+  }
+};
+
+struct TaperOp : fwix_op_s {
+  Tables t;
+  TaperOp() { kind = KIND_TAPER; }
+  void run(int add, const float2* in, float2* out, hipStream_t st) {
+    PwArgs a = t.pwargs();
+    a.add = add;
+    hip_check(launch_taper(in, out, a, st), "taper");
+  }
+};
+
+struct OneStepOp : fwix_op_s {
+  StepEngine eng;
+  int iz = 0;
+  OneStepOp() { kind = KIND_ONESTEP; }
+};
+
+template <class T>
+T* as(fwix_op op, int kind) {
+  if (!op) throw Error("null operator handle");
+  if (op->kind != kind) throw Error("operator handle of the wrong kind");
+  return static_cast<T*>(op);
+}
+
+inline hipStream_t S(fwix_stream s) { return (hipStream_t)s; }
+
+}  // namespace
+
+extern "C" {
+
+int fwix_version(void) { return 100; }
+const char* fwix_last_error(void) { return fwix::g_err.c_str(); }
+
+void fwix_params_default(fwix_params* p) {
+  if (!p) return;
+  p->nref = 1; p->taperx = 0; p->tapery = 0; p->eps = 0.f; p->eps_scat = 0.04f;
+  p->taylor = 1; p->planes_per_chunk = 0;
+}
+
+int fwix_fft_size_supported(int n) { return fft_size_supported(n) ? 1 : 0; }
+
+int fwix_op_destroy(fwix_op op) {
+  return guard([&] { delete op; });
+}
+
+// ---------------------------------------------------------------- FFT
+int fwix_fft2d_create(const fwix_dims* d, fwix_op* out) {
+  return guard([&] {
+    validate_dims(d, true);
+    if (!out) throw Error("null output handle");
+    auto* op = new FftOp();
+    try {
+      op->t.init(*d, 0, 0);
+      op->chunk = auto_chunk(*d, 0, 0);
+      op->K.alloc((size_t)d->NX * d->NY * op->chunk * sizeof(float2));
+    } catch (...) { delete op; throw; }
+    *out = op;
+  });
+}
+int fwix_fft2d_forward(fwix_op op, int add, const fwix_c64* m, fwix_c64* d, fwix_stream s) {
+  return guard([&] { as<FftOp>(op, KIND_FFT)->run(false, add, (const float2*)m, (float2*)d, S(s)); });
+}
+int fwix_fft2d_adjoint(fwix_op op, int add, fwix_c64* m, const fwix_c64* d, fwix_stream s) {
+  return guard([&] { as<FftOp>(op, KIND_FFT)->run(true, add, (const float2*)d, (float2*)m, S(s)); });
+}
+
+// ---------------------------------------------------------------- PhaseShift
+int fwix_phase_shift_create(const fwix_dims* d, float dz, float eps, fwix_op* out) {
+  return guard([&] {
+    validate_dims(d, false);
+    if (!out) throw Error("null output handle");
+    auto* op = new PhaseShiftOp();
+    try {
+      op->t.init(*d, 0, 0);
+      op->dz = dz; op->eps = eps;
+      std::vector<float2> ones(d->NW, make_float2(1.f, 0.f));
+      upload_bytes(op->sref, ones.data(), ones.size() * 8);
+    } catch (...) { delete op; throw; }
+    *out = op;
+  });
+}
+int fwix_phase_shift_set_slow(fwix_op op, const fwix_c64* sref) {
+  return guard([&] {
+    auto* p = as<PhaseShiftOp>(op, KIND_PHASE_SHIFT);
+    if (!sref) throw Error("null slowness");
+    hip_check(hipMemcpy(p->sref.p, sref, (size_t)p->t.dims.NW * 8, hipMemcpyHostToDevice), "set_slow");
+  });
+}
+int fwix_phase_shift_forward(fwix_op op, int add, const fwix_c64* m, fwix_c64* d, fwix_stream s) {
+  return guard([&] { as<PhaseShiftOp>(op, KIND_PHASE_SHIFT)->run(0, add, (const float2*)m, (float2*)d, S(s)); });
+}
+int fwix_phase_shift_adjoint(fwix_op op, int add, fwix_c64* m, const fwix_c64* d, fwix_stream s) {
+  return guard([&] { as<PhaseShiftOp>(op, KIND_PHASE_SHIFT)->run(1, add, (const float2*)d, (float2*)m, S(s)); });
+}
+int fwix_phase_shift_inverse(fwix_op op, int add, fwix_c64* m, const fwix_c64* d, fwix_stream s) {
+  return guard([&] { as<PhaseShiftOp>(op, KIND_PHASE_SHIFT)->run(2, add, (const float2*)d, (float2*)m, S(s)); });
+}
+
+// ---------------------------------------------------------------- Selector
+int fwix_selector_create(const fwix_dims* d, fwix_op* out) {
+  return guard([&] {
+    validate_dims(d, false);
+    if (!out) throw Error("null output handle");
+    auto* op = new SelectorOp();
+    try {
+      op->t.init(*d, 0, 0);
+      op->labels.alloc((size_t)d->NX * d->NY * d->NW);
+    } catch (...) { delete op; throw; }
+    *out = op;
+  });
+}
+int fwix_selector_set_labels(fwix_op op, const int32_t* labels) {
+  return guard([&] {
+    auto* p = as<SelectorOp>(op, KIND_SELECTOR);
+    if (!labels) throw Error("null labels");
+    const fwix_dims& d = p->t.dims;
+    const size_t n = (size_t)d.NX * d.NY * d.NW;
+    std::vector<uint8_t> h(n);
+    for (size_t i = 0; i < n; ++i) {
+      if (labels[i] < 0 || labels[i] > 254) throw Error("labels must be in [0, 254]");
+      h[i] = (uint8_t)labels[i];
+    }
+    hip_check(hipMemcpy(p->labels.p, h.data(), n, hipMemcpyHostToDevice), "set_labels");
+    p->has_labels = true;
+  });
+}
+int fwix_selector_set_value(fwix_op op, int value) {
+  return guard([&] { as<SelectorOp>(op, KIND_SELECTOR)->value = value; });
+}
+int fwix_selector_forward(fwix_op op, int add, const fwix_c64* m, fwix_c64* d, fwix_stream s) {
+  return guard([&] { as<SelectorOp>(op, KIND_SELECTOR)->run(add, (const float2*)m, (float2*)d, S(s)); });
+}
+int fwix_selector_adjoint(fwix_op op, int add, fwix_c64* m, const fwix_c64* d, fwix_stream s) {
+  return guard([&] { as<SelectorOp>(op, KIND_SELECTOR)->run(add, (const float2*)d, (float2*)m, S(s)); });
+}
+
+// ---------------------------------------------------------------- Taper
+int fwix_taper_create(const fwix_dims* d, int tx, int ty, fwix_op* out) {
+  return guard([&] {
+    validate_dims(d, false);
+    if (!out) throw Error("null output handle");
+    if (tx < 0 || ty < 0) throw Error("taper widths must be >= 0");
+    auto* op = new TaperOp();
+    try { op->t.init(*d, tx, ty); } catch (...) { delete op; throw; }
+    *out = op;
+  });
+}
+int fwix_taper_forward(fwix_op op, int add, const fwix_c64* m, fwix_c64* d, fwix_stream s) {
+  return guard([&] { as<TaperOp>(op, KIND_TAPER)->run(add, (const float2*)m, (float2*)d, S(s)); });
+}
+int fwix_taper_adjoint(fwix_op op, int add, fwix_c64* m, const fwix_c64* d, fwix_stream s) {
+  return guard([&] { as<TaperOp>(op, KIND_TAPER)->run(add, (const float2*)d, (float2*)m, S(s)); });
+}
+
+// ---------------------------------------------------------------- OneStep
+int fwix_onestep_create(int method, const fwix_dims* d, const fwix_params* p, fwix_op* out) {
+  return guard([&] {
+    validate_dims(d, true);
+    validate_params(p);
+    if (method != FWIX_PSPI && method != FWIX_NSPS) throw Error("method must be FWIX_PSPI or FWIX_NSPS");
+    if (d->nz < 1) throw Error("dims: nz must be >= 1");
+    if (!out) throw Error("null output handle");
+    auto* op = new OneStepOp();
+    try { op->eng.init(method, *d, *p); } catch (...) { delete op; throw; }
+    *out = op;
+  });
+}
+int fwix_onestep_set_refs(fwix_op op, const fwix_c64* sref, const uint8_t* labels) {
+  return guard([&] {
+    auto* p = as<OneStepOp>(op, KIND_ONESTEP);
+    if (!sref || !labels) throw Error("null reference slownesses or labels");
+    p->eng.sref = (const float2*)sref;
+    p->eng.labels = labels;
+  });
+}
+int fwix_onestep_set_depth(fwix_op op, int iz) {
+  return guard([&] {
+    auto* p = as<OneStepOp>(op, KIND_ONESTEP);
+    if (iz < 0 || iz >= p->eng.tables.dims.nz) throw Error("depth index out of range");
+    p->iz = iz;
+  });
+}
+int fwix_onestep_forward(fwix_op op, int add, const fwix_c64* m, fwix_c64* d, fwix_stream s) {
+  return guard([&] {
+    auto* p = as<OneStepOp>(op, KIND_ONESTEP);
+    p->eng.apply(false, p->iz, add, (const float2*)m, (float2*)d, S(s));
+  });
+}
+int fwix_onestep_adjoint(fwix_op op, int add, fwix_c64* m, const fwix_c64* d, fwix_stream s) {
+  return guard([&] {
+    auto* p = as<OneStepOp>(op, KIND_ONESTEP);
+    p->eng.apply(true, p->iz, add, (const float2*)d, (float2*)m, S(s));
+  });
+}
+
+}  // extern "C"

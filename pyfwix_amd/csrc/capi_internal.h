@@ -1,0 +1,95 @@
+// Internal host-side types of libpyfwix_hip.so (handles, device tables,
+// scratch buffers, error plumbing).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdexcept>
+#include <string>
+
+#include "fwix_hip.h"
+#include "fwix_launch.h"
+
+enum {
+  KIND_FFT = 1,
+  KIND_PHASE_SHIFT,
+  KIND_SELECTOR,
+  KIND_TAPER,
+  KIND_ONESTEP,
+  KIND_ONEWAY,
+  KIND_INJECTION,
+  KIND_SCATTER,
+  KIND_IC,
+  KIND_BORN,
+};
+
+struct fwix_op_s {
+  int kind = 0;
+  virtual ~fwix_op_s() = default;
+};
+
+namespace fwix {
+
+struct Error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+extern thread_local std::string g_err;
+void set_error(const std::string& s);
+void hip_check(hipError_t e, const char* what);
+
+// Runs f and converts any exception into the C ABI's error convention.
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return -1;
+  } catch (...) {
+    set_error("unknown error");
+    return -1;
+  }
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void alloc(size_t bytes);
+  void release();
+};
+
+// Device tables of one wavefield hypercube: wavenumbers, frequencies, taper
+// weights and FFT twiddles.
+struct Tables {
+  fwix_dims dims{};
+  DevBuf b_kx, b_ky, b_w2, b_w, b_tapx, b_tapy, b_twx, b_twy;
+  void init(const fwix_dims& d, int tx, int ty);
+  KArgs kargs() const;
+  PwArgs pwargs() const;
+};
+
+void validate_dims(const fwix_dims* d, bool need_fft);
+void validate_params(const fwix_params* p);
+int auto_chunk(const fwix_dims& d, int nterm, int requested);
+
+// One depth step (PSPI or NSPS) over a whole wavefield, forward or adjoint,
+// run as chunks of planes through the three fused passes.
+struct StepEngine {
+  int method = FWIX_PSPI;
+  fwix_params params{};
+  Tables tables;
+  const float2* sref = nullptr;     // borrowed (nz, nref, NW)
+  const uint8_t* labels = nullptr;  // borrowed (nz, NW, NY, NX)
+  DevBuf K, W;
+  int chunk = 1;
+  void init(int method, const fwix_dims& d, const fwix_params& p);
+  void check_ready(int iz) const;
+  KArgs base(int iz, int c0, int np) const;
+  void apply(bool adjoint, int iz, int add, const float2* in, float2* out, hipStream_t st);
+};
+
+}  // namespace fwix

@@ -1,0 +1,97 @@
+// Plain per-operator kernels: the reference's operator granularity
+// (PhaseShift, Selector, Taper, add/scale), used by the standalone operator
+// entry points and by the plain per-shot comparator path.  One thread per
+// complex sample, 64-bit flat indices, x fastest.
+#include "fwix_kernels.h"
+#include "fwix_launch.h"
+
+namespace fwix {
+
+static inline dim3 flat_grid(size_t n) {
+  size_t g = (n + kThreads - 1) / kThreads;
+  if (g > 1u << 30) g = 1u << 30;
+  return dim3((unsigned)g);
+}
+
+// d = (add ? d : 0) + Phi * m  (propagator/src/phase_shift.cu:7-61,125-184,248-305)
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_phase_shift(const float2* __restrict__ m, float2* d, PwArgs a) {
+  const size_t n = (size_t)a.NX * a.NY * a.NW * a.NS;
+  for (size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x; idx < n; idx += (size_t)gridDim.x * kThreads) {
+    const int ix = (int)(idx % a.NX);
+    const size_t t = idx / a.NX;
+    const int iy = (int)(t % a.NY);
+    const int iw = (int)((t / a.NY) % a.NW);
+    const float2 s = a.sref[iw];
+    const float k2 = fmaf(a.kx[ix], a.kx[ix], a.ky[iy] * a.ky[iy]);
+    float2 y = cmul(m[idx], ps_mult<MODE>(a.w2[iw], s.x, s.y, k2, a.dz, a.eps));
+    if (a.add) y = cadd(y, d[idx]);
+    d[idx] = y;
+  }
+}
+
+// d = (add ? d : 0) + [labels == value] * m  (propagator/src/selector.cu:7-37)
+__global__ __launch_bounds__(kThreads) void k_select(const float2* __restrict__ m, float2* d, PwArgs a) {
+  const size_t n = (size_t)a.NX * a.NY * a.NW * a.NS;
+  const size_t plane3 = (size_t)a.NX * a.NY * a.NW;
+  for (size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x; idx < n; idx += (size_t)gridDim.x * kThreads) {
+    float2 y = a.labels[idx % plane3] == a.value ? m[idx] : czero();
+    if (a.add) y = cadd(y, d[idx]);
+    d[idx] = y;
+  }
+}
+
+// d = (add ? d : 0) + wx(ix) wy(iy) m  (propagator/src/taper.cu:7-77)
+__global__ __launch_bounds__(kThreads) void k_taper(const float2* __restrict__ m, float2* d, PwArgs a) {
+  const size_t n = (size_t)a.NX * a.NY * a.NW * a.NS;
+  for (size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x; idx < n; idx += (size_t)gridDim.x * kThreads) {
+    const int ix = (int)(idx % a.NX);
+    const int iy = (int)((idx / a.NX) % a.NY);
+    float2 y = cscale(m[idx], a.tapx[ix] * a.tapy[iy]);
+    if (a.add) y = cadd(y, d[idx]);
+    d[idx] = y;
+  }
+}
+
+// d = (add ? d : 0) + alpha * m  (operator/src/complex_vector.cu:4-29 add / scale)
+__global__ __launch_bounds__(kThreads) void k_axpy(const float2* __restrict__ m, float2* d, size_t n, float alpha, int add) {
+  for (size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x; idx < n; idx += (size_t)gridDim.x * kThreads) {
+    float2 y = cscale(m[idx], alpha);
+    if (add) y = cadd(y, d[idx]);
+    d[idx] = y;
+  }
+}
+
+// zero fill (used instead of hipMemsetAsync so that captured graphs replay correctly)
+__global__ __launch_bounds__(kThreads) void k_zero(float2* d, size_t n) {
+  for (size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x; idx < n; idx += (size_t)gridDim.x * kThreads)
+    d[idx] = czero();
+}
+
+hipError_t launch_phase_shift(int mode, const float2* m, float2* d, const PwArgs& a, hipStream_t st) {
+  const dim3 g = flat_grid((size_t)a.NX * a.NY * a.NW * a.NS);
+  if (mode == 0) hipLaunchKernelGGL(k_phase_shift<0>, g, dim3(kThreads), 0, st, m, d, a);
+  else if (mode == 1) hipLaunchKernelGGL(k_phase_shift<1>, g, dim3(kThreads), 0, st, m, d, a);
+  else hipLaunchKernelGGL(k_phase_shift<2>, g, dim3(kThreads), 0, st, m, d, a);
+  return hipGetLastError();
+}
+hipError_t launch_select(const float2* m, float2* d, const PwArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_select, flat_grid((size_t)a.NX * a.NY * a.NW * a.NS), dim3(kThreads), 0, st, m, d, a);
+  return hipGetLastError();
+}
+hipError_t launch_taper(const float2* m, float2* d, const PwArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_taper, flat_grid((size_t)a.NX * a.NY * a.NW * a.NS), dim3(kThreads), 0, st, m, d, a);
+  return hipGetLastError();
+}
+hipError_t launch_axpy(const float2* m, float2* d, size_t n, float alpha, int add, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_axpy, flat_grid(n), dim3(kThreads), 0, st, m, d, n, alpha, add);
+  return hipGetLastError();
+}
+hipError_t launch_zero(float2* d, size_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero, flat_grid(n), dim3(kThreads), 0, st, d, n);
+  return hipGetLastError();
+}
+
+}  // namespace fwix

@@ -1,0 +1,435 @@
+// Fused one-way depth-step kernels: three passes per PSPI / NSPS step.
+//
+//   PSPI forward   (propagator/src/PSPI.cpp:6-24, :66-83)
+//     rows_fwd(pre-taper)  ->  cols_fanout(Phi_r)  ->  rows_select
+//   PSPI adjoint   (PSPI.cpp:85-103)
+//     rows_fanout(mask)    ->  cols_fanin(Phi_r^adj) -> rows_inv(post-taper)
+//   NSPS forward   (propagator/src/NSPS.cpp:27-45)
+//     rows_fanout(mask)    ->  cols_fanin(Phi_r)    -> rows_inv
+//   NSPS adjoint   (corrected; NSPS.cpp:6-25 passes swapped arguments)
+//     rows_fwd             ->  cols_fanout(Phi_r^adj) -> rows_select
+//
+// Rows are transformed one wavefront per row (or per few short rows), loaded
+// and stored straight from/to HBM in the positions the first/last Stockham
+// stage needs.  Columns are staged through a [NY][CW+1] LDS tile so global
+// accesses stay 128-byte coalesced.  The per-reference fan-out keeps the
+// spectrum of a column tile in registers and runs all nref inverse
+// transforms from it; the fan-in accumulates all nref spectra in registers
+// before a single inverse transform.
+#include "fwix_kernels.h"
+#include "fwix_launch.h"
+
+#include <type_traits>
+
+namespace fwix {
+
+template <int N>
+struct RowCfg {
+  static constexpr int NB = row_nb(N);
+  static constexpr int V = NB * N / 64;
+  static constexpr int LDSW = NB * N;
+};
+template <int N>
+struct ColCfg {
+  static constexpr int CW = col_cw(N);
+  static constexpr int NB = CW / kWaves;
+  static constexpr int V = NB * N / 64;
+  static constexpr int CWP = CW + 1;
+};
+
+// ------------------------------------------------------------------ row passes
+// out = (add ? out : 0) + post(scale * F_x(pre * in))   along x, one row per wave
+template <int N, bool INV, bool PRE_TAPER, bool POST_TAPER>
+__global__ __launch_bounds__(kThreads) void k_rows(KArgs a) {
+  using C = RowCfg<N>;
+  using F = WaveFft<N, C::NB, false, INV, RowLds<N>>;
+  __shared__ float2 lds[kWaves * C::LDSW];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nrows = a.np * a.NY;
+  const int rb = (blockIdx.x * kWaves + wave) * C::NB;
+  if (rb >= nrows) return;
+  float2* wl = lds + wave * C::LDSW;
+  float2 v[C::V];
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) {
+    int b, i;
+    edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
+    const int lr = rb + b;
+    float2 x = czero();
+    if (lr < nrows) {
+      x = a.in[(size_t)lr * N + i];
+      if (PRE_TAPER) x = cscale(x, a.tapx[i] * a.tapy[lr % a.NY]);
+    }
+    v[e] = x;
+  }
+  F::run(v, wl, a.twx, lane);
+  const float s = a.scale;  // row normalisation 1/sqrt(NX)
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) {
+    int b, i;
+    edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
+    const int lr = rb + b;
+    if (lr < nrows) {
+      const size_t idx = (size_t)lr * N + i;
+      float2 y = cscale(v[e], s);
+      if (POST_TAPER) y = cscale(y, a.tapx[i] * a.tapy[lr % a.NY]);
+      if (a.add) y = cadd(y, a.out[idx]);
+      a.out[idx] = y;
+    }
+  }
+}
+
+// rows_fanout: W_r = scale * F_x([labels == r] * in) for every reference r
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
+  using C = RowCfg<N>;
+  using F = WaveFft<N, C::NB, false, false, RowLds<N>>;
+  __shared__ float2 lds[kWaves * C::LDSW];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nrows = a.np * a.NY;
+  const int rb = (blockIdx.x * kWaves + wave) * C::NB;
+  if (rb >= nrows) return;
+  float2* wl = lds + wave * C::LDSW;
+  float2 u[C::V];
+  uint8_t lab[C::V];
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) {
+    int b, i;
+    edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
+    const int lr = rb + b;
+    u[e] = czero();
+    lab[e] = 255;
+    if (lr < nrows) {
+      u[e] = a.in[(size_t)lr * N + i];
+      const int p = lr / a.NY, iy = lr - p * a.NY;
+      lab[e] = a.labels[((size_t)plane_iw(a, p) * a.NY + iy) * N + i];
+    }
+  }
+  const float s = a.scale;
+  for (int r = 0; r < a.nref; ++r) {
+    float2 x[C::V];
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) x[e] = (lab[e] == r) ? u[e] : czero();
+    F::run(x, wl, a.twx, lane);
+    float2* W = a.W + (size_t)r * a.wstride;
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      int b, i;
+      edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
+      const int lr = rb + b;
+      if (lr < nrows) W[(size_t)lr * N + i] = cscale(x[e], s);
+    }
+  }
+}
+
+// rows_select: out = (add ? out : 0) + sum_r [labels == r] * scale * F_x^-1(W_r)
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
+  using C = RowCfg<N>;
+  using F = WaveFft<N, C::NB, false, true, RowLds<N>>;
+  __shared__ float2 lds[kWaves * C::LDSW];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nrows = a.np * a.NY;
+  const int rb = (blockIdx.x * kWaves + wave) * C::NB;
+  if (rb >= nrows) return;
+  float2* wl = lds + wave * C::LDSW;
+  uint8_t lab[C::V];
+  float2 acc[C::V];
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) {
+    int b, i;
+    edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
+    const int lr = rb + b;
+    acc[e] = czero();
+    lab[e] = 255;
+    if (lr < nrows) {
+      const int p = lr / a.NY, iy = lr - p * a.NY;
+      lab[e] = a.labels[((size_t)plane_iw(a, p) * a.NY + iy) * N + i];
+    }
+  }
+  for (int r = 0; r < a.nref; ++r) {
+    const float2* W = a.W + (size_t)r * a.wstride;
+    float2 x[C::V];
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      int b, i;
+      edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
+      const int lr = rb + b;
+      x[e] = lr < nrows ? W[(size_t)lr * N + i] : czero();
+    }
+    F::run(x, wl, a.twx, lane);
+#pragma unroll
+    for (int e = 0; e < C::V; ++e)
+      if (lab[e] == r) acc[e] = cadd(acc[e], x[e]);
+  }
+  const float s = a.scale;
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) {
+    int b, i;
+    edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
+    const int lr = rb + b;
+    if (lr < nrows) {
+      const size_t idx = (size_t)lr * N + i;
+      float2 y = cscale(acc[e], s);
+      if (a.add) y = cadd(y, a.out[idx]);
+      a.out[idx] = y;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ column passes
+template <int N, int CW, int CWP>
+__device__ __forceinline__ void tile_load(float2* tile, const float2* __restrict__ src, int NX, int c0) {
+  for (int e = threadIdx.x; e < N * CW; e += kThreads) {
+    const int i = e / CW, c = e % CW, col = c0 + c;
+    tile[i * CWP + c] = col < NX ? src[(size_t)i * NX + col] : czero();
+  }
+}
+template <int N, int CW, int CWP>
+__device__ __forceinline__ void tile_store(const float2* tile, float2* __restrict__ dst, int NX, int c0, int add) {
+  for (int e = threadIdx.x; e < N * CW; e += kThreads) {
+    const int i = e / CW, c = e % CW, col = c0 + c;
+    if (col < NX) {
+      const size_t idx = (size_t)i * NX + col;
+      float2 y = tile[i * CWP + c];
+      if (add) y = cadd(y, dst[idx]);
+      dst[idx] = y;
+    }
+  }
+}
+
+// out = (add ? out : 0) + scale * F_y(in) on each plane (plain 2-D FFT column pass)
+template <int N, bool INV>
+__global__ __launch_bounds__(kThreads) void k_cols(KArgs a) {
+  using C = ColCfg<N>;
+  using F = WaveFft<N, C::NB, true, INV, ColLds<C::CWP>>;
+  __shared__ float2 tile[N * C::CWP];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int p = blockIdx.y, c0 = blockIdx.x * C::CW;
+  const size_t pb = (size_t)p * a.NX * N;
+  tile_load<N, C::CW, C::CWP>(tile, a.in + pb, a.NX, c0);
+  __syncthreads();
+  float2* wl = tile + wave * C::NB;
+  float2 v[C::V];
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) {
+    int b, i;
+    edge_pos<N, C::NB, F::P::first(), true>(lane, e, b, i);
+    v[e] = wl[i * C::CWP + b];
+  }
+  F::run(v, wl, a.twy, lane);
+  const float s = a.scale;  // column normalisation 1/sqrt(NY)
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) {
+    int b, i;
+    edge_pos<N, C::NB, F::P::last(), true>(lane, e, b, i);
+    wl[i * C::CWP + b] = cscale(v[e], s);
+  }
+  __syncthreads();
+  tile_store<N, C::CW, C::CWP>(tile, a.out + pb, a.NX, c0, a.add);
+}
+
+// cols_fanout: W_r = F_y^-1[Phi_r * F_y(K)] / NY for every reference r.
+// MODE 0: forward phase shift, 1: adjoint phase shift.
+template <int N, int MODE>
+__global__ __launch_bounds__(kThreads) void k_cols_fanout(KArgs a) {
+  using C = ColCfg<N>;
+  using F = WaveFft<N, C::NB, true, false, ColLds<C::CWP>>;
+  using Fi = WaveFft<N, C::NB, true, true, ColLds<C::CWP>>;
+  static_assert(F::P::last() == Fi::P::first(), "register hand-off needs mirrored plans");
+  __shared__ float2 tile[N * C::CWP];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int p = blockIdx.y, c0 = blockIdx.x * C::CW;
+  const size_t pb = (size_t)p * a.NX * N;
+  tile_load<N, C::CW, C::CWP>(tile, a.K + pb, a.NX, c0);
+  __syncthreads();
+  float2* wl = tile + wave * C::NB;
+  float2 spec[C::V];
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) {
+    int b, i;
+    edge_pos<N, C::NB, F::P::first(), true>(lane, e, b, i);
+    spec[e] = wl[i * C::CWP + b];
+  }
+  F::run(spec, wl, a.twy, lane);
+  const int iw = plane_iw(a, p);
+  const float w2 = a.w2[iw];
+  const float s2 = a.scale;  // 1/NY: forward and inverse column normalisation together
+  const int colw = c0 + wave * C::NB;
+  for (int r = 0; r < a.nref; ++r) {
+    const float2 sr = a.sref[r * a.NW + iw];
+    float2 x[C::V];
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      int b, i;
+      edge_pos<N, C::NB, F::P::last(), true>(lane, e, b, i);
+      const int col = colw + b;
+      const float kxv = a.kx[col < a.NX ? col : 0], kyv = a.ky[i];
+      const float2 ph = ps_mult<MODE>(w2, sr.x, sr.y, fmaf(kxv, kxv, kyv * kyv), a.dz, a.eps);
+      x[e] = cscale(cmul(spec[e], ph), s2);
+    }
+    Fi::run(x, wl, a.twy, lane);
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      int b, i;
+      edge_pos<N, C::NB, Fi::P::last(), true>(lane, e, b, i);
+      wl[i * C::CWP + b] = x[e];
+    }
+    __syncthreads();
+    tile_store<N, C::CW, C::CWP>(tile, a.W + (size_t)r * a.wstride + pb, a.NX, c0, 0);
+    __syncthreads();
+  }
+}
+
+// cols_fanin: K = F_y^-1[sum_r Phi_r * F_y(W_r)] / NY.
+template <int N, int MODE>
+__global__ __launch_bounds__(kThreads) void k_cols_fanin(KArgs a) {
+  using C = ColCfg<N>;
+  using F = WaveFft<N, C::NB, true, false, ColLds<C::CWP>>;
+  using Fi = WaveFft<N, C::NB, true, true, ColLds<C::CWP>>;
+  static_assert(F::P::last() == Fi::P::first(), "register hand-off needs mirrored plans");
+  __shared__ float2 tile[N * C::CWP];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int p = blockIdx.y, c0 = blockIdx.x * C::CW;
+  const size_t pb = (size_t)p * a.NX * N;
+  float2* wl = tile + wave * C::NB;
+  const int iw = plane_iw(a, p);
+  const float w2 = a.w2[iw];
+  const int colw = c0 + wave * C::NB;
+  float2 acc[C::V];
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) acc[e] = czero();
+  for (int r = 0; r < a.nref; ++r) {
+    tile_load<N, C::CW, C::CWP>(tile, a.W + (size_t)r * a.wstride + pb, a.NX, c0);
+    __syncthreads();
+    float2 v[C::V];
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      int b, i;
+      edge_pos<N, C::NB, F::P::first(), true>(lane, e, b, i);
+      v[e] = wl[i * C::CWP + b];
+    }
+    F::run(v, wl, a.twy, lane);
+    const float2 sr = a.sref[r * a.NW + iw];
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      int b, i;
+      edge_pos<N, C::NB, F::P::last(), true>(lane, e, b, i);
+      const int col = colw + b;
+      const float kxv = a.kx[col < a.NX ? col : 0], kyv = a.ky[i];
+      const float2 ph = ps_mult<MODE>(w2, sr.x, sr.y, fmaf(kxv, kxv, kyv * kyv), a.dz, a.eps);
+      acc[e] = cadd(acc[e], cmul(v[e], ph));
+    }
+    __syncthreads();
+  }
+  const float s2 = a.scale;
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) acc[e] = cscale(acc[e], s2);
+  Fi::run(acc, wl, a.twy, lane);
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) {
+    int b, i;
+    edge_pos<N, C::NB, Fi::P::last(), true>(lane, e, b, i);
+    wl[i * C::CWP + b] = acc[e];
+  }
+  __syncthreads();
+  tile_store<N, C::CW, C::CWP>(tile, a.K + pb, a.NX, c0, 0);
+}
+
+// ------------------------------------------------------------------ launchers
+#define FWIX_SIZES(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
+
+bool fft_size_supported(int n) {
+  switch (n) {
+#define CASE(S) case S:
+    FWIX_SIZES(CASE)
+#undef CASE
+    return true;
+    default: return false;
+  }
+}
+
+template <class Fn>
+static hipError_t dispatch_n(int n, Fn&& fn) {
+  switch (n) {
+#define CASE(S) case S: return fn(std::integral_constant<int, S>{});
+    FWIX_SIZES(CASE)
+#undef CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+static dim3 row_grid(int N, int nrows) {
+  const int nb = row_nb(N);
+  const int waves = (nrows + nb - 1) / nb;
+  return dim3((waves + kWaves - 1) / kWaves);
+}
+static dim3 col_grid(int N, int NX, int np) {
+  const int cw = col_cw(N);
+  return dim3((NX + cw - 1) / cw, np);
+}
+
+// kind: 0 plain forward, 1 plain inverse, 2 forward with pre-taper,
+//       3 inverse with post-taper
+hipError_t launch_rows(int kind, const KArgs& a, hipStream_t st) {
+  return dispatch_n(a.NX, [&](auto nc) -> hipError_t {
+    constexpr int N = decltype(nc)::value;
+    const dim3 g = row_grid(N, a.np * a.NY);
+    switch (kind) {
+      case 0: hipLaunchKernelGGL((k_rows<N, false, false, false>), g, dim3(kThreads), 0, st, a); break;
+      case 1: hipLaunchKernelGGL((k_rows<N, true, false, false>), g, dim3(kThreads), 0, st, a); break;
+      case 2: hipLaunchKernelGGL((k_rows<N, false, true, false>), g, dim3(kThreads), 0, st, a); break;
+      case 3: hipLaunchKernelGGL((k_rows<N, true, false, true>), g, dim3(kThreads), 0, st, a); break;
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_rows_fanout(const KArgs& a, hipStream_t st) {
+  return dispatch_n(a.NX, [&](auto nc) -> hipError_t {
+    constexpr int N = decltype(nc)::value;
+    hipLaunchKernelGGL((k_rows_fanout<N>), row_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_rows_select(const KArgs& a, hipStream_t st) {
+  return dispatch_n(a.NX, [&](auto nc) -> hipError_t {
+    constexpr int N = decltype(nc)::value;
+    hipLaunchKernelGGL((k_rows_select<N>), row_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_cols(bool inv, const KArgs& a, hipStream_t st) {
+  return dispatch_n(a.NY, [&](auto nc) -> hipError_t {
+    constexpr int N = decltype(nc)::value;
+    const dim3 g = col_grid(N, a.NX, a.np);
+    if (inv) hipLaunchKernelGGL((k_cols<N, true>), g, dim3(kThreads), 0, st, a);
+    else hipLaunchKernelGGL((k_cols<N, false>), g, dim3(kThreads), 0, st, a);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_cols_fanout(int mode, const KArgs& a, hipStream_t st) {
+  return dispatch_n(a.NY, [&](auto nc) -> hipError_t {
+    constexpr int N = decltype(nc)::value;
+    const dim3 g = col_grid(N, a.NX, a.np);
+    if (mode == 0) hipLaunchKernelGGL((k_cols_fanout<N, 0>), g, dim3(kThreads), 0, st, a);
+    else hipLaunchKernelGGL((k_cols_fanout<N, 1>), g, dim3(kThreads), 0, st, a);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_cols_fanin(int mode, const KArgs& a, hipStream_t st) {
+  return dispatch_n(a.NY, [&](auto nc) -> hipError_t {
+    constexpr int N = decltype(nc)::value;
+    const dim3 g = col_grid(N, a.NX, a.np);
+    if (mode == 0) hipLaunchKernelGGL((k_cols_fanin<N, 0>), g, dim3(kThreads), 0, st, a);
+    else hipLaunchKernelGGL((k_cols_fanin<N, 1>), g, dim3(kThreads), 0, st, a);
+    return hipGetLastError();
+  });
+}
+
+}  // namespace fwix
