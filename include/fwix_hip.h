@@ -125,6 +125,81 @@ int fwix_onestep_set_depth(fwix_op op, int iz);
 int fwix_onestep_forward(fwix_op op, int add, const fwix_c64* model, fwix_c64* data, fwix_stream s);
 int fwix_onestep_adjoint(fwix_op op, int add, fwix_c64* model, const fwix_c64* data, fwix_stream s);
 
+/* ---------------------------------------------------------------- Injection
+ * Injection(domain = traces (ntrace, NW), range = wavefield, oz, dz, cx, cy,
+ * cz, ids) + set_coords / set_depth (propagator/src/Injection.h:10-55,
+ * injection.cu).  Coordinates and shot ids are host arrays of ntrace values.
+ * forward scatters traces into the wavefield at depth step iz (bilinear in
+ * x, y, linear in z); adjoint records.  The forward is race-free (each node
+ * sums its contributions in a fixed order; the reference's kernel races,
+ * injection.cu:89-92). */
+int fwix_injection_create(const fwix_dims* d, int ntrace, fwix_op* out);
+int fwix_injection_set_coords(fwix_op op, const float* cx, const float* cy, const float* cz, const int* ids);
+int fwix_injection_set_depth(fwix_op op, int iz);
+int fwix_injection_forward(fwix_op op, int add, const fwix_c64* traces, fwix_c64* wfld, fwix_stream s);
+int fwix_injection_adjoint(fwix_op op, int add, fwix_c64* traces, const fwix_c64* wfld, fwix_stream s);
+
+/* ---------------------------------------------------------------- Scatter
+ * Scatter(domain, slow, par) + set_depth (propagator/src/Scatter.h:12-104,
+ * Scatter.cpp, scatter.cu): Born scattering, Taylor series of dkz/ds with
+ * params.taylor terms and params.eps_scat.  set_slow borrows the unpadded
+ * background slowness (nz, NW, ny, nx) in device memory (the reference copies
+ * one host slice per set_depth, Scatter.cpp:68-74). */
+int fwix_scatter_create(const fwix_dims* d, const fwix_params* p, fwix_op* out);
+int fwix_scatter_set_slow(fwix_op op, const fwix_c64* slow_dev);
+int fwix_scatter_set_depth(fwix_op op, int iz);
+int fwix_scatter_forward(fwix_op op, int add, const fwix_c64* model, fwix_c64* data, fwix_stream s);
+int fwix_scatter_adjoint(fwix_op op, int add, fwix_c64* model, const fwix_c64* data, fwix_stream s);
+
+/* ---------------------------------------------------------------- ImagingCondition
+ * ImagingCondition(domain = image (NW, ny, nx), range = wavefield)
+ * (propagator/src/ImagingCondition.h:11-43, imaging_condition.cu).
+ * set_background borrows the background wavefield slice U_z (device).
+ * forward: wfld (+)= U_z * img on the model region; adjoint: img (+)=
+ * sum_shots conj(U_z) * wfld, deterministic (no float atomics). */
+int fwix_ic_create(const fwix_dims* d, fwix_op* out);
+int fwix_ic_set_background(fwix_op op, const fwix_c64* bg_dev);
+int fwix_ic_forward(fwix_op op, int add, const fwix_c64* img, fwix_c64* wfld, fwix_stream s);
+int fwix_ic_adjoint(fwix_op op, int add, fwix_c64* img, const fwix_c64* wfld, fwix_stream s);
+
+/* ---------------------------------------------------------------- Downward / Upward
+ * Downward (direction 0) / Upward (direction 1) depth recursions
+ * (propagator/src/OneWay.h:14-245, OneWay.cpp:6-79) over params.nref-reference
+ * PSPI (method FWIX_PSPI, as the reference, OneWay.h:24-25) or NSPS steps.
+ * forward: data (+)= P_{nz-1}...P_0 model (Downward) or P_0...P_{nz-1} model
+ * (Upward); model is left untouched (the reference steps it in place).
+ * ckpt (nullable, nz wavefields) receives the wavefield before every step
+ * (OneWay::compress_slice, OneWay.h:52-58). */
+int fwix_oneway_create(int direction, int method, const fwix_dims* d, const fwix_params* p, fwix_op* out);
+int fwix_oneway_set_refs(fwix_op op, const fwix_c64* sref_dev, const uint8_t* labels_dev);
+int fwix_oneway_forward(fwix_op op, int add, const fwix_c64* model, fwix_c64* data, fwix_c64* ckpt, fwix_stream s);
+int fwix_oneway_adjoint(fwix_op op, int add, fwix_c64* model, const fwix_c64* data, fwix_stream s);
+
+/* ---------------------------------------------------------------- Born (ExtendedBorn, down-scattering channel)
+ * The background down-going modeling of Propagator::forward
+ * (propagator/src/Propagator.cpp:155-180) and the linearized Born operator
+ * of ExtendedBorn::downward_scattering_fwd/adj (ExtendedBorn.cpp:152-208)
+ * with its migration image.  Background wavefields stay in HBM: every depth
+ * (ckpt_every <= 1) or every ckpt_every-th depth with recomputation in the
+ * adjoint -- lossless, replacing the ZFP WavefieldPool (WavefieldPool.cpp).
+ *   sources / receivers: host coordinates (n values) and shot ids
+ *   wavelet (nsrc, NW), data (nrec, NW), model ds (nz, NW, ny, nx),
+ *   image (nz, ny, nx) = sum_w sum_s conj(U_z) P_z^H lam_{z+1}
+ * fwix_born_model must run (after set_refs / set_sources / set_wavelet)
+ * before forward / adjoint / migrate; it optionally records the direct data. */
+int fwix_born_create(const fwix_dims* d, const fwix_params* p, int ckpt_every, fwix_op* out);
+int fwix_born_set_refs(fwix_op op, const fwix_c64* sref_dev, const uint8_t* labels_dev);
+int fwix_born_set_slow(fwix_op op, const fwix_c64* slow_dev);
+int fwix_born_set_sources(fwix_op op, int n, const float* x, const float* y, const float* z, const int* ids);
+int fwix_born_set_receivers(fwix_op op, int n, const float* x, const float* y, const float* z, const int* ids);
+int fwix_born_set_wavelet(fwix_op op, const fwix_c64* wavelet_dev);
+int fwix_born_model(fwix_op op, int add, fwix_c64* data, fwix_stream s);
+int fwix_born_forward(fwix_op op, int add, const fwix_c64* dslow, fwix_c64* data, fwix_stream s);
+int fwix_born_adjoint(fwix_op op, int add, fwix_c64* dslow, const fwix_c64* data, fwix_stream s);
+int fwix_born_migrate(fwix_op op, int add, fwix_c64* image, const fwix_c64* data, fwix_stream s);
+/* depth steps (PSPI applications of one wavefield) executed so far, -1 on a bad handle */
+long long fwix_born_steps(fwix_op op);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,10 +68,38 @@ _SIGS = {
     "fwix_onestep_set_depth": [_P, _I],
     "fwix_onestep_forward": [_P, _I, _P, _P, _P],
     "fwix_onestep_adjoint": [_P, _I, _P, _P, _P],
+    "fwix_injection_create": [_D, _I, _OPP],
+    "fwix_injection_set_coords": [_P, _P, _P, _P, _P],
+    "fwix_injection_set_depth": [_P, _I],
+    "fwix_injection_forward": [_P, _I, _P, _P, _P],
+    "fwix_injection_adjoint": [_P, _I, _P, _P, _P],
+    "fwix_scatter_create": [_D, _PP, _OPP],
+    "fwix_scatter_set_slow": [_P, _P],
+    "fwix_scatter_set_depth": [_P, _I],
+    "fwix_scatter_forward": [_P, _I, _P, _P, _P],
+    "fwix_scatter_adjoint": [_P, _I, _P, _P, _P],
+    "fwix_ic_create": [_D, _OPP],
+    "fwix_ic_set_background": [_P, _P],
+    "fwix_ic_forward": [_P, _I, _P, _P, _P],
+    "fwix_ic_adjoint": [_P, _I, _P, _P, _P],
+    "fwix_oneway_create": [_I, _I, _D, _PP, _OPP],
+    "fwix_oneway_set_refs": [_P, _P, _P],
+    "fwix_oneway_forward": [_P, _I, _P, _P, _P, _P],
+    "fwix_oneway_adjoint": [_P, _I, _P, _P, _P],
+    "fwix_born_create": [_D, _PP, _I, _OPP],
+    "fwix_born_set_refs": [_P, _P, _P],
+    "fwix_born_set_slow": [_P, _P],
+    "fwix_born_set_sources": [_P, _I, _P, _P, _P, _P],
+    "fwix_born_set_receivers": [_P, _I, _P, _P, _P, _P],
+    "fwix_born_set_wavelet": [_P, _P],
+    "fwix_born_model": [_P, _I, _P, _P],
+    "fwix_born_forward": [_P, _I, _P, _P, _P],
+    "fwix_born_adjoint": [_P, _I, _P, _P, _P],
+    "fwix_born_migrate": [_P, _I, _P, _P, _P],
 }
 
 EXPORTS = ["fwix_version", "fwix_last_error", "fwix_params_default", "fwix_fft_size_supported",
-           *_SIGS.keys()]
+           "fwix_born_steps", *_SIGS.keys()]
 
 for _name, _args in _SIGS.items():
     _fn = getattr(lib, _name, None)
@@ -85,6 +113,8 @@ lib.fwix_params_default.argtypes = [_PP]
 lib.fwix_params_default.restype = None
 lib.fwix_fft_size_supported.argtypes = [_I]
 lib.fwix_fft_size_supported.restype = C.c_int
+lib.fwix_born_steps.argtypes = [_P]
+lib.fwix_born_steps.restype = C.c_longlong
 
 
 def check(rc: int) -> None:

@@ -1,0 +1,634 @@
+// C ABI of the operators around the step: Injection, Scatter,
+// ImagingCondition, Downward / Upward recursions and the Born operator
+// (background modeling, linearized forward, adjoint, migration).
+// Declarations and reference citations: include/fwix_hip.h.
+#include "fwix_hip.h"
+#include "fwix_launch.h"
+#include "capi_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+using namespace fwix;
+
+namespace fwix {
+
+template <class T>
+static void upload_vec(DevBuf& b, const std::vector<T>& v) {
+  b.alloc(v.size() * sizeof(T));
+  if (!v.empty()) hip_check(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+// ------------------------------------------------------------------ injection plan
+void InjPlan::build(const fwix_dims& d, int nlev_, int ntrace_, const float* cx, const float* cy, const float* cz,
+                    const int* ids) {
+  if (ntrace_ < 0) throw Error("ntrace must be >= 0");
+  if (ntrace_ > 0 && (!cx || !cy || !cz || !ids)) throw Error("null coordinates");
+  ntrace = ntrace_;
+  nlev = nlev_;
+  NW = d.NW;
+  NX = d.NX;
+  plane = (long long)d.NX * d.NY;
+  const long long vol = plane * d.NW;  // one shot
+  struct Ent { int lev; long long node; int trace; float w; };
+  struct Act { int lev; int trace; long long base; float4 w; };
+  std::vector<Ent> ents;
+  std::vector<Act> acts;
+  for (int t = 0; t < ntrace; ++t) {
+    if (ids[t] < 0 || ids[t] >= d.NS) throw Error("trace shot id out of range [0, NS)");
+    // propagator/src/injection.cu:38-48, float arithmetic as in the kernel
+    const int iy = (int)floorf((cy[t] - d.oy) / d.dy);
+    const float y = d.oy + iy * d.dy;
+    const float ly = 1.f - (cy[t] - y) / d.dy;
+    const int ix = (int)floorf((cx[t] - d.ox) / d.dx);
+    const float x = d.ox + ix * d.dx;
+    const float lx = 1.f - (cx[t] - x) / d.dx;
+    const int iz = (int)floorf((cz[t] - d.oz) / d.dz);
+    const float z = d.oz + iz * d.dz;
+    const float lz = 1.f - (cz[t] - z) / d.dz;
+    if (ix < 0 || ix >= d.NX - 1 || iy < 0 || iy >= d.NY - 1) continue;  // injection.cu:50-52
+    for (int k = 0; k < 2; ++k) {
+      const int lev = iz + k;
+      if (lev < 0 || lev >= nlev) continue;
+      const float f = k == 0 ? lz : 1 - lz;  // injection.cu:60-75
+      float4 w;
+      w.x = f * lx * ly;
+      w.y = f * (1 - lx) * ly;
+      w.z = f * lx * (1 - ly);
+      w.w = f * (1 - lx) * (1 - ly);
+      const long long base = (long long)ids[t] * vol + (long long)iy * d.NX + ix;
+      acts.push_back({lev, t, base, w});
+      ents.push_back({lev, base, t, w.x});
+      ents.push_back({lev, base + 1, t, w.y});
+      ents.push_back({lev, base + d.NX, t, w.z});
+      ents.push_back({lev, base + d.NX + 1, t, w.w});
+    }
+  }
+  std::stable_sort(acts.begin(), acts.end(), [](const Act& a, const Act& b) { return a.lev < b.lev; });
+  std::stable_sort(ents.begin(), ents.end(), [](const Ent& a, const Ent& b) {
+    if (a.lev != b.lev) return a.lev < b.lev;
+    if (a.node != b.node) return a.node < b.node;
+    return a.trace < b.trace;
+  });
+  act_lvl.assign(nlev + 1, 0);
+  seg_lvl.assign(nlev + 1, 0);
+  std::vector<int> at(acts.size());
+  std::vector<long long> ab(acts.size());
+  std::vector<float4> aw(acts.size());
+  for (size_t k = 0; k < acts.size(); ++k) {
+    at[k] = acts[k].trace;
+    ab[k] = acts[k].base;
+    aw[k] = acts[k].w;
+    act_lvl[acts[k].lev + 1]++;
+  }
+  std::vector<long long> sb;
+  std::vector<int> so, et;
+  std::vector<float> ew;
+  for (size_t k = 0; k < ents.size(); ++k) {
+    if (k == 0 || ents[k].lev != ents[k - 1].lev || ents[k].node != ents[k - 1].node) {
+      sb.push_back(ents[k].node);
+      so.push_back((int)et.size());
+      seg_lvl[ents[k].lev + 1]++;
+    }
+    et.push_back(ents[k].trace);
+    ew.push_back(ents[k].w);
+  }
+  so.push_back((int)et.size());
+  for (int l = 0; l < nlev; ++l) {
+    act_lvl[l + 1] += act_lvl[l];
+    seg_lvl[l + 1] += seg_lvl[l];
+  }
+  upload_vec(act_trace, at);
+  upload_vec(act_base, ab);
+  upload_vec(act_w, aw);
+  upload_vec(seg_base, sb);
+  upload_vec(seg_off, so);
+  upload_vec(ent_trace, et);
+  upload_vec(ent_w, ew);
+}
+
+void InjPlan::forward(int iz, const float2* traces, float2* wfld, hipStream_t st) const {
+  if (iz < 0 || iz >= nlev) return;
+  hip_check(launch_inj_fwd(traces, wfld, (const long long*)seg_base.p, (const int*)seg_off.p,
+                           (const int*)ent_trace.p, (const float*)ent_w.p, seg_lvl[iz],
+                           seg_lvl[iz + 1] - seg_lvl[iz], NW, plane, st),
+            "injection forward");
+}
+
+void InjPlan::adjoint(int iz, float2* traces, const float2* wfld, hipStream_t st) const {
+  if (iz < 0 || iz >= nlev) return;
+  hip_check(launch_inj_adj(traces, wfld, (const int*)act_trace.p, (const long long*)act_base.p,
+                           (const float4*)act_w.p, act_lvl[iz], act_lvl[iz + 1] - act_lvl[iz], NW, NX, plane, st),
+            "injection adjoint");
+}
+
+}  // namespace fwix
+
+namespace {
+
+inline hipStream_t S(fwix_stream s) { return (hipStream_t)s; }
+
+template <class T>
+T* as(fwix_op op, int kind) {
+  if (!op) throw Error("null operator handle");
+  if (op->kind != kind) throw Error("operator handle of the wrong kind");
+  return static_cast<T*>(op);
+}
+
+size_t wsize(const fwix_dims& d) { return (size_t)d.NX * d.NY * d.NW * d.NS; }
+
+// ------------------------------------------------------------------ Injection
+struct InjectionOp : fwix_op_s {
+  fwix_dims d{};
+  int ntrace = 0, iz = 0;
+  InjPlan plan;
+  bool has_coords = false;
+  InjectionOp() { kind = KIND_INJECTION; }
+};
+
+// ------------------------------------------------------------------ Scatter
+struct ScatterOp : fwix_op_s {
+  StepEngine eng;  // tables + K/W scratch (labels unused)
+  const float2* slow = nullptr;
+  int iz = 0;
+  ScatterOp() { kind = KIND_SCATTER; }
+  void run(bool adjoint, int add, const float2* in, float2* out, hipStream_t st) {
+    if (!slow) throw Error("scatter slowness not set (call set_slow)");
+    const fwix_dims& d = eng.tables.dims;
+    const size_t plane = (size_t)d.NX * d.NY;
+    const int planes = d.NW * d.NS;
+    const float sx = (float)(1.0 / std::sqrt((double)d.NX));
+    const float sy2 = (float)(1.0 / (double)d.NY);
+    for (int c0 = 0; c0 < planes; c0 += eng.chunk) {
+      const int np = std::min(eng.chunk, planes - c0);
+      KArgs a = eng.tables.kargs();
+      a.K = (float2*)eng.K.p;
+      a.W = (float2*)eng.W.p;
+      a.p0 = c0; a.np = np;
+      a.wstride = (long long)np * plane;
+      a.nterm = eng.params.taylor + 1;
+      a.eps_s = eng.params.eps_scat;
+      a.slow = slow + (size_t)iz * d.NW * d.ny * d.nx;
+      const float2* cin = in + (size_t)c0 * plane;
+      float2* cout = out + (size_t)c0 * plane;
+      if (!adjoint) {  // Scatter.cpp:90-102
+        a.in = cin; a.out = a.K; a.add = 0; a.scale = sx;
+        hip_check(launch_rows(0, a, st), "scatter rows");
+        a.scale = sy2;
+        hip_check(launch_cols_fanout(2, a, st), "scatter columns");
+        a.out = cout; a.add = add; a.scale = sx;
+        hip_check(launch_rows_scatter(false, a, st), "scatter combine");
+      } else {  // Scatter.cpp:104-116
+        a.in = cin; a.scale = sx;
+        hip_check(launch_rows_scatter_adj(a, st), "scatter adjoint rows");
+        a.scale = sy2;
+        hip_check(launch_cols_fanin(2, a, st), "scatter adjoint columns");
+        a.in = a.K; a.out = cout; a.add = add; a.scale = sx;
+        hip_check(launch_rows(1, a, st), "scatter adjoint inverse");
+      }
+    }
+  }
+};
+
+// ------------------------------------------------------------------ ImagingCondition
+struct ICOp : fwix_op_s {
+  fwix_dims d{};
+  const float2* bg = nullptr;
+  ICOp() { kind = KIND_IC; }
+};
+
+// ------------------------------------------------------------------ Downward / Upward
+struct OneWayOp : fwix_op_s {
+  int direction = 0;  // 0 down, 1 up
+  StepEngine eng;
+  DevBuf u;
+  OneWayOp() { kind = KIND_ONEWAY; }
+  // d (+)= P_{last} ... P_{first} m  (OneWay.cpp:22-79)
+  void run(bool adjoint, int add, const float2* in, float2* out, float2* ckpt, hipStream_t st) {
+    const fwix_dims& d = eng.tables.dims;
+    const size_t n = wsize(d);
+    float2* w = (float2*)u.p;
+    hip_check(launch_axpy(in, w, n, 1.f, 0, st), "copy");
+    const int nz = d.nz;
+    // Downward forward and Upward adjoint step z = 0..nz-1; the others z = nz-1..0
+    const bool ascending = (direction == 0) != adjoint;
+    for (int k = 0; k < nz; ++k) {
+      const int z = ascending ? k : nz - 1 - k;
+      if (ckpt) hip_check(launch_axpy(w, ckpt + (size_t)z * n, n, 1.f, 0, st), "checkpoint");
+      eng.apply(adjoint, z, 0, w, w, st);
+    }
+    hip_check(launch_axpy(w, out, n, 1.f, add, st), "accumulate");
+  }
+};
+
+// ------------------------------------------------------------------ Born
+struct BornOp : fwix_op_s {
+  StepEngine eng;
+  fwix_dims d{};
+  fwix_params par{};
+  const float2* slow = nullptr;     // (nz, NW, ny, nx) background slowness
+  const float2* wavelet = nullptr;  // (nsrc, NW)
+  InjPlan src, rec;
+  int nsrc = 0, nrec = 0;
+  int ckpt = 1;
+  DevBuf store, seg, U, dU, lam;
+  int seg_first = -1;  // first depth of the recomputed segment held in seg
+  bool have_bg = false;
+  long long steps = 0;  // depth steps executed (statistics)
+  BornOp() { kind = KIND_BORN; }
+
+  size_t P() const { return wsize(d); }
+  float2* bufU() { return (float2*)U.p; }
+
+  // wavefield U_z of the background down-going recursion (Propagator.cpp:155-180)
+  const float2* background(int z, hipStream_t st) {
+    if (!have_bg) throw Error("background wavefield not modeled (call fwix_born_model)");
+    const size_t n = P();
+    if (ckpt == 1) return (const float2*)store.p + (size_t)z * n;
+    const int s0 = (z / ckpt) * ckpt;
+    if (seg_first != s0) {
+      // recompute U_{s0..s1-1} from the checkpoint at s0
+      const int s1 = std::min(s0 + ckpt, d.nz);
+      float2* sg = (float2*)seg.p;
+      hip_check(launch_axpy((const float2*)store.p + (size_t)(s0 / ckpt) * n, sg, n, 1.f, 0, st), "restore");
+      for (int zz = s0 + 1; zz < s1; ++zz) {
+        float2* prev = sg + (size_t)(zz - 1 - s0) * n;
+        float2* cur = sg + (size_t)(zz - s0) * n;
+        eng.apply(false, zz - 1, 0, prev, cur, st);
+        src.forward(zz, wavelet, cur, st);
+        ++steps;
+      }
+      seg_first = s0;
+    }
+    return (const float2*)seg.p + (size_t)(z - s0) * n;
+  }
+
+  void model(float2* data, int add, hipStream_t st) {
+    if (!wavelet) throw Error("wavelet not set (call fwix_born_set_wavelet)");
+    if (!eng.sref) throw Error("reference slownesses / labels not set");
+    const size_t n = P();
+    if (data && !add) hip_check(launch_zero(data, (size_t)nrec * d.NW, st), "zero data");
+    float2* u = bufU();
+    hip_check(launch_zero(u, n, st), "zero wavefield");
+    for (int z = 0; z < d.nz; ++z) {
+      src.forward(z, wavelet, u, st);
+      if (data) rec.adjoint(z, data, u, st);
+      if (z % ckpt == 0) hip_check(launch_axpy(u, (float2*)store.p + (size_t)(z / ckpt) * n, n, 1.f, 0, st), "store");
+      if (z + 1 < d.nz) {
+        eng.apply(false, z, 0, u, u, st);
+        ++steps;
+      }
+    }
+    seg_first = -1;
+    have_bg = true;
+  }
+
+  KArgs born_args(int z, int p0, int np) const {
+    KArgs a = eng.base(z, p0, np);
+    a.nterm = par.taylor + 1;
+    a.slow = slow + (size_t)z * d.NW * d.ny * d.nx;
+    a.eps_s = par.eps_scat;
+    return a;
+  }
+
+  // dU_{z+1} = P_z [ dU_z + S_z(U_z * ds_z) ]   (ExtendedBorn.cpp:152-178)
+  void forward(int add, const float2* dslow, float2* data, hipStream_t st) {
+    if (!slow) throw Error("background slowness not set (call fwix_born_set_slow)");
+    const size_t n = P(), plane = (size_t)d.NX * d.NY;
+    if (!add) hip_check(launch_zero(data, (size_t)nrec * d.NW, st), "zero data");
+    float2* du = (float2*)dU.p;
+    hip_check(launch_zero(du, n, st), "zero dU");
+    const float sx = (float)(1.0 / std::sqrt((double)d.NX)), sy2 = (float)(1.0 / (double)d.NY);
+    const int cw = std::min(eng.chunk, d.NW);
+    for (int z = 0; z < d.nz; ++z) {
+      if (z > 0) rec.adjoint(z, data, du, st);
+      if (z + 1 == d.nz) break;  // dU_nz is never recorded
+      const float2* bg = background(z, st);
+      for (int is = 0; is < d.NS; ++is)
+        for (int w0 = 0; w0 < d.NW; w0 += cw) {
+          const int p0 = is * d.NW + w0, np = std::min(cw, d.NW - w0);
+          KArgs a = born_args(z, p0, np);
+          a.dslow = dslow + (size_t)z * d.NW * d.ny * d.nx;
+          a.bg = bg + (size_t)p0 * plane;
+          a.scale = sx;
+          hip_check(launch_rows_ic_fwd(a, st), "born ic");
+          a.wstride = (long long)np * plane;
+          a.scale = sy2;
+          hip_check(launch_cols_fanout(2, a, st), "born scatter columns");
+          a.addin = du + (size_t)p0 * plane;
+          a.scale = sx;
+          hip_check(launch_rows_scatter(true, a, st), "born scatter rows");
+          a.nterm = par.nref;
+          a.scale = sy2;
+          hip_check(launch_cols_fanout(0, a, st), "born pspi columns");
+          a.out = du + (size_t)p0 * plane;
+          a.add = 0;
+          a.scale = sx;
+          hip_check(launch_rows_select(a, st), "born pspi rows");
+        }
+      ++steps;
+    }
+  }
+
+  // adjoint (ExtendedBorn.cpp:180-208) or, with image != null, the migration
+  // image I_z = sum_{w,s} conj(U_z) P_z^H lam_{z+1}
+  void adjoint(int add, float2* dslow, float2* image, const float2* data, hipStream_t st) {
+    if (!slow && !image) throw Error("background slowness not set (call fwix_born_set_slow)");
+    const size_t n = P(), plane = (size_t)d.NX * d.NY;
+    const size_t mz = (size_t)d.NW * d.ny * d.nx;
+    if (!add) {
+      if (dslow) hip_check(launch_zero(dslow, mz * d.nz, st), "zero model");
+      if (image) hip_check(launch_zero(image, (size_t)d.ny * d.nx * d.nz, st), "zero image");
+    }
+    float2* l = (float2*)lam.p;
+    hip_check(launch_zero(l, n, st), "zero lambda");
+    const float sx = (float)(1.0 / std::sqrt((double)d.NX)), sy2 = (float)(1.0 / (double)d.NY);
+    const int cw = std::min(eng.chunk, d.NW);
+    for (int z = d.nz - 1; z >= 0; --z) {
+      if (z + 1 < d.nz) {  // lam_{nz} = 0: nothing to propagate or image at the bottom
+        const float2* bg = background(z, st);
+        if (image) {
+          eng.apply(true, z, 0, l, l, st);
+          hip_check(launch_image(bg, l, image + (size_t)z * d.ny * d.nx, d.NX, d.NY, d.NW, d.NS, d.nx, d.ny, 1, st),
+                    "image");
+        } else {
+          for (int is = 0; is < d.NS; ++is)
+            for (int w0 = 0; w0 < d.NW; w0 += cw) {
+              const int p0 = is * d.NW + w0, np = std::min(cw, d.NW - w0);
+              float2* lc = l + (size_t)p0 * plane;
+              KArgs a = born_args(z, p0, np);
+              a.wstride = (long long)np * plane;
+              // mu = P_z^H lam (PSPI adjoint, in place)
+              a.in = lc; a.scale = sx;
+              hip_check(launch_rows_fanout(a, st), "born adjoint pspi rows");
+              a.nterm = par.nref; a.scale = sy2;
+              hip_check(launch_cols_fanin(1, a, st), "born adjoint pspi columns");
+              a.in = a.K; a.out = lc; a.add = 0; a.scale = sx;
+              hip_check(launch_rows(par.taperx > 0 || par.tapery > 0 ? 3 : 1, a, st), "born adjoint pspi inverse");
+              // ds_z += conj(U_z) S_z^H mu
+              a.in = lc; a.nterm = par.taylor + 1; a.scale = sx;
+              hip_check(launch_rows_scatter_adj(a, st), "born adjoint scatter rows");
+              a.scale = sy2;
+              hip_check(launch_cols_fanin(2, a, st), "born adjoint scatter columns");
+              a.bg = bg + (size_t)p0 * plane;
+              a.aux = dslow + (size_t)z * mz;
+              a.add = 1; a.scale = sx;
+              hip_check(launch_rows_ic_adj(a, st), "born adjoint imaging");
+            }
+        }
+        ++steps;
+      }
+      rec.forward(z, data, l, st);  // adjoint source (ExtendedBorn.cpp:195-196)
+    }
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------- Injection
+int fwix_injection_create(const fwix_dims* d, int ntrace, fwix_op* out) {
+  return guard([&] {
+    validate_dims(d, false);
+    if (ntrace < 0) throw Error("ntrace must be >= 0");
+    if (!out) throw Error("null output handle");
+    auto* op = new InjectionOp();
+    op->d = *d;
+    op->ntrace = ntrace;
+    *out = op;
+  });
+}
+int fwix_injection_set_coords(fwix_op op, const float* cx, const float* cy, const float* cz, const int* ids) {
+  return guard([&] {
+    auto* p = as<InjectionOp>(op, KIND_INJECTION);
+    int maxlev = std::max(p->d.nz, 1);
+    for (int t = 0; t < p->ntrace; ++t) {
+      const int iz = (int)floorf((cz[t] - p->d.oz) / p->d.dz);
+      maxlev = std::max(maxlev, iz + 2);
+    }
+    p->plan.build(p->d, maxlev, p->ntrace, cx, cy, cz, ids);
+    p->has_coords = true;
+  });
+}
+int fwix_injection_set_depth(fwix_op op, int iz) {
+  return guard([&] { as<InjectionOp>(op, KIND_INJECTION)->iz = iz; });
+}
+int fwix_injection_forward(fwix_op op, int add, const fwix_c64* traces, fwix_c64* wfld, fwix_stream s) {
+  return guard([&] {
+    auto* p = as<InjectionOp>(op, KIND_INJECTION);
+    if (!p->has_coords) throw Error("injection coordinates not set");
+    if (!add) hip_check(launch_zero((float2*)wfld, wsize(p->d), S(s)), "zero");
+    p->plan.forward(p->iz, (const float2*)traces, (float2*)wfld, S(s));
+  });
+}
+int fwix_injection_adjoint(fwix_op op, int add, fwix_c64* traces, const fwix_c64* wfld, fwix_stream s) {
+  return guard([&] {
+    auto* p = as<InjectionOp>(op, KIND_INJECTION);
+    if (!p->has_coords) throw Error("injection coordinates not set");
+    if (!add) hip_check(launch_zero((float2*)traces, (size_t)p->ntrace * p->d.NW, S(s)), "zero");
+    p->plan.adjoint(p->iz, (float2*)traces, (const float2*)wfld, S(s));
+  });
+}
+
+// ---------------------------------------------------------------- Scatter
+int fwix_scatter_create(const fwix_dims* d, const fwix_params* par, fwix_op* out) {
+  return guard([&] {
+    validate_dims(d, true);
+    validate_params(par);
+    if (d->nz < 1) throw Error("dims: nz must be >= 1");
+    for (int i = 0; i < d->NW; ++i)
+      if ((double)d->ow + (double)i * d->dw == 0.0) throw Error("Frequency is zero in the scattering operator!");
+    if (!out) throw Error("null output handle");
+    auto* op = new ScatterOp();
+    try {
+      fwix_params p = *par;
+      op->eng.init(FWIX_PSPI, *d, p, p.taylor + 1);
+    } catch (...) { delete op; throw; }
+    *out = op;
+  });
+}
+int fwix_scatter_set_slow(fwix_op op, const fwix_c64* slow_dev) {
+  return guard([&] {
+    if (!slow_dev) throw Error("null slowness");
+    as<ScatterOp>(op, KIND_SCATTER)->slow = (const float2*)slow_dev;
+  });
+}
+int fwix_scatter_set_depth(fwix_op op, int iz) {
+  return guard([&] {
+    auto* p = as<ScatterOp>(op, KIND_SCATTER);
+    if (iz < 0 || iz >= p->eng.tables.dims.nz) throw Error("depth index out of range");
+    p->iz = iz;
+  });
+}
+int fwix_scatter_forward(fwix_op op, int add, const fwix_c64* m, fwix_c64* d, fwix_stream s) {
+  return guard([&] { as<ScatterOp>(op, KIND_SCATTER)->run(false, add, (const float2*)m, (float2*)d, S(s)); });
+}
+int fwix_scatter_adjoint(fwix_op op, int add, fwix_c64* m, const fwix_c64* d, fwix_stream s) {
+  return guard([&] { as<ScatterOp>(op, KIND_SCATTER)->run(true, add, (const float2*)d, (float2*)m, S(s)); });
+}
+
+// ---------------------------------------------------------------- ImagingCondition
+int fwix_ic_create(const fwix_dims* d, fwix_op* out) {
+  return guard([&] {
+    validate_dims(d, false);
+    if (!out) throw Error("null output handle");
+    auto* op = new ICOp();
+    op->d = *d;
+    *out = op;
+  });
+}
+int fwix_ic_set_background(fwix_op op, const fwix_c64* bg_dev) {
+  return guard([&] {
+    if (!bg_dev) throw Error("null background wavefield");
+    as<ICOp>(op, KIND_IC)->bg = (const float2*)bg_dev;
+  });
+}
+int fwix_ic_forward(fwix_op op, int add, const fwix_c64* img, fwix_c64* wfld, fwix_stream s) {
+  return guard([&] {
+    auto* p = as<ICOp>(op, KIND_IC);
+    if (!p->bg) throw Error("background wavefield not set");
+    const fwix_dims& d = p->d;
+    hip_check(launch_ic_fwd((const float2*)img, (float2*)wfld, p->bg, d.NX, d.NY, d.NW, d.NS, d.nx, d.ny, add, S(s)),
+              "ic forward");
+  });
+}
+int fwix_ic_adjoint(fwix_op op, int add, fwix_c64* img, const fwix_c64* wfld, fwix_stream s) {
+  return guard([&] {
+    auto* p = as<ICOp>(op, KIND_IC);
+    if (!p->bg) throw Error("background wavefield not set");
+    const fwix_dims& d = p->d;
+    hip_check(launch_ic_adj((float2*)img, (const float2*)wfld, p->bg, d.NX, d.NY, d.NW, d.NS, d.nx, d.ny, add, S(s)),
+              "ic adjoint");
+  });
+}
+
+// ---------------------------------------------------------------- Downward / Upward
+int fwix_oneway_create(int direction, int method, const fwix_dims* d, const fwix_params* par, fwix_op* out) {
+  return guard([&] {
+    validate_dims(d, true);
+    validate_params(par);
+    if (direction != 0 && direction != 1) throw Error("direction must be 0 (down) or 1 (up)");
+    if (method != FWIX_PSPI && method != FWIX_NSPS) throw Error("method must be FWIX_PSPI or FWIX_NSPS");
+    if (d->nz < 1) throw Error("dims: nz must be >= 1");
+    if (!out) throw Error("null output handle");
+    auto* op = new OneWayOp();
+    try {
+      op->direction = direction;
+      op->eng.init(method, *d, *par);
+      op->u.alloc(wsize(*d) * sizeof(float2));
+    } catch (...) { delete op; throw; }
+    *out = op;
+  });
+}
+int fwix_oneway_set_refs(fwix_op op, const fwix_c64* sref, const uint8_t* labels) {
+  return guard([&] {
+    auto* p = as<OneWayOp>(op, KIND_ONEWAY);
+    if (!sref || !labels) throw Error("null reference slownesses or labels");
+    p->eng.sref = (const float2*)sref;
+    p->eng.labels = labels;
+  });
+}
+int fwix_oneway_forward(fwix_op op, int add, const fwix_c64* m, fwix_c64* d, fwix_c64* ckpt, fwix_stream s) {
+  return guard([&] {
+    as<OneWayOp>(op, KIND_ONEWAY)->run(false, add, (const float2*)m, (float2*)d, (float2*)ckpt, S(s));
+  });
+}
+int fwix_oneway_adjoint(fwix_op op, int add, fwix_c64* m, const fwix_c64* d, fwix_stream s) {
+  return guard([&] {
+    as<OneWayOp>(op, KIND_ONEWAY)->run(true, add, (const float2*)d, (float2*)m, nullptr, S(s));
+  });
+}
+
+// ---------------------------------------------------------------- Born
+int fwix_born_create(const fwix_dims* d, const fwix_params* par, int ckpt_every, fwix_op* out) {
+  return guard([&] {
+    validate_dims(d, true);
+    validate_params(par);
+    if (d->nz < 1) throw Error("dims: nz must be >= 1");
+    if (ckpt_every < 0) throw Error("ckpt_every must be >= 0");
+    for (int i = 0; i < d->NW; ++i)
+      if ((double)d->ow + (double)i * d->dw == 0.0) throw Error("Frequency is zero in the scattering operator!");
+    if (!out) throw Error("null output handle");
+    auto* op = new BornOp();
+    try {
+      op->d = *d;
+      op->par = *par;
+      op->ckpt = ckpt_every <= 1 ? 1 : std::min(ckpt_every, d->nz);
+      op->eng.init(FWIX_PSPI, *d, *par, par->taylor + 1);
+      const size_t n = wsize(*d) * sizeof(float2);
+      const size_t nck = (size_t)((d->nz + op->ckpt - 1) / op->ckpt);
+      op->store.alloc(n * nck);
+      if (op->ckpt > 1) op->seg.alloc(n * op->ckpt);
+      op->U.alloc(n);
+      op->dU.alloc(n);
+      op->lam.alloc(n);
+    } catch (...) { delete op; throw; }
+    *out = op;
+  });
+}
+int fwix_born_set_refs(fwix_op op, const fwix_c64* sref, const uint8_t* labels) {
+  return guard([&] {
+    auto* p = as<BornOp>(op, KIND_BORN);
+    if (!sref || !labels) throw Error("null reference slownesses or labels");
+    p->eng.sref = (const float2*)sref;
+    p->eng.labels = labels;
+    p->have_bg = false;
+  });
+}
+int fwix_born_set_slow(fwix_op op, const fwix_c64* slow_dev) {
+  return guard([&] {
+    if (!slow_dev) throw Error("null slowness");
+    as<BornOp>(op, KIND_BORN)->slow = (const float2*)slow_dev;
+  });
+}
+int fwix_born_set_sources(fwix_op op, int n, const float* x, const float* y, const float* z, const int* ids) {
+  return guard([&] {
+    auto* p = as<BornOp>(op, KIND_BORN);
+    p->src.build(p->d, p->d.nz, n, x, y, z, ids);
+    p->nsrc = n;
+    p->have_bg = false;
+  });
+}
+int fwix_born_set_receivers(fwix_op op, int n, const float* x, const float* y, const float* z, const int* ids) {
+  return guard([&] {
+    auto* p = as<BornOp>(op, KIND_BORN);
+    p->rec.build(p->d, p->d.nz, n, x, y, z, ids);
+    p->nrec = n;
+  });
+}
+int fwix_born_set_wavelet(fwix_op op, const fwix_c64* wavelet_dev) {
+  return guard([&] {
+    if (!wavelet_dev) throw Error("null wavelet");
+    auto* p = as<BornOp>(op, KIND_BORN);
+    p->wavelet = (const float2*)wavelet_dev;
+    p->have_bg = false;
+  });
+}
+int fwix_born_model(fwix_op op, int add, fwix_c64* data, fwix_stream s) {
+  return guard([&] { as<BornOp>(op, KIND_BORN)->model((float2*)data, add, S(s)); });
+}
+int fwix_born_forward(fwix_op op, int add, const fwix_c64* dslow, fwix_c64* data, fwix_stream s) {
+  return guard([&] { as<BornOp>(op, KIND_BORN)->forward(add, (const float2*)dslow, (float2*)data, S(s)); });
+}
+int fwix_born_adjoint(fwix_op op, int add, fwix_c64* dslow, const fwix_c64* data, fwix_stream s) {
+  return guard([&] {
+    if (!dslow) throw Error("null model");
+    as<BornOp>(op, KIND_BORN)->adjoint(add, (float2*)dslow, nullptr, (const float2*)data, S(s));
+  });
+}
+int fwix_born_migrate(fwix_op op, int add, fwix_c64* image, const fwix_c64* data, fwix_stream s) {
+  return guard([&] {
+    if (!image) throw Error("null image");
+    as<BornOp>(op, KIND_BORN)->adjoint(add, nullptr, (float2*)image, (const float2*)data, S(s));
+  });
+}
+long long fwix_born_steps(fwix_op op) {
+  if (!op || op->kind != KIND_BORN) return -1;
+  return static_cast<BornOp*>(op)->steps;
+}
+
+}  // extern "C"

@@ -1,0 +1,448 @@
+// Born scattering, imaging condition and injection kernels.
+//
+// Born forward step (propagator/src/ExtendedBorn.cpp:152-178 with
+// ForwardScattering, Scattering.cpp:43-52):
+//     dU_{z+1} = P_z [ dU_z + S_z(U_z * ds_z) ]
+//   rows_ic_fwd       v = U_z * ds_z (zero in the padding), F_x        -> K
+//   cols_fanout<2>    F_y, (k^2/w^2)^it, F_y^-1 for it = 0..T           -> W_it
+//   rows_scatter      F_x^-1 W_it, * c_it s~^(-1/2-it), sum, * -i w dz/2,
+//                     + dU_z, taper, F_x                                  -> K
+//   cols_fanout<0>, rows_select: the PSPI step                           -> dU_{z+1}
+// Born adjoint step (ExtendedBorn.cpp:180-208, Scattering.cpp:54-61):
+//   mu = P_z^H lam (three PSPI passes), then
+//   rows_scatter_adj  * +i w dz/2, conj(c_it s~^(-1/2-it)), F_x          -> W_it
+//   cols_fanin<2>     F_y, (k^2/w^2)^it, sum, F_y^-1                     -> K
+//   rows_ic_adj       F_x^-1, ds_z += conj(U_z) * (.) on the model region
+#include "fwix_kernels.h"
+#include "fwix_launch.h"
+
+#include <type_traits>
+
+namespace fwix {
+
+template <int N>
+struct BRowCfg {
+  static constexpr int NB = row_nb(N);
+  static constexpr int V = NB * N / 64;
+  static constexpr int LDSW = NB * padded(N);
+};
+
+// c_it * s~^(-1/2-it), s~ = Re s + i (Im s - eps_s Re s), from the edge-padded
+// slowness slice (propagator/src/scatter.cu:61-106, pad_fwd :240-277,
+// Taylor coefficients Scatter.h:73).  s^(-1/2) is the reciprocal of the
+// principal square root, identical to |s|^(-1/2) e^{-i arg(s)/2}
+// (scatter.cu:6-21); higher terms multiply by 1/s.
+__device__ __forceinline__ void scatter_factors(const KArgs& a, int iw, int iy, int ix, float2& rsq, float2& inv) {
+  const int yy = iy < a.ny ? iy : a.ny - 1;
+  const int xx = ix < a.nx ? ix : a.nx - 1;
+  const float2 s = a.slow[((size_t)iw * a.ny + yy) * a.nx + xx];
+  const float sr = s.x, si = s.y - a.eps_s * s.x;
+  // principal sqrt of (sr, si)
+  const float r = sqrtf(fmaf(sr, sr, si * si));
+  float qr, qi;
+  if (sr >= 0.f) {
+    qr = sqrtf(0.5f * (r + sr));
+    qi = qr > 0.f ? 0.5f * si / qr : 0.f;
+  } else {
+    const float t = sqrtf(0.5f * (r - sr));
+    qr = 0.5f * fabsf(si) / t;
+    qi = copysignf(t, si);
+  }
+  const float iq = 1.f / fmaf(qr, qr, qi * qi);
+  rsq = make_float2(qr * iq, -qi * iq);                // 1/sqrt(s~)
+  const float is2 = 1.f / fmaf(sr, sr, si * si);
+  inv = make_float2(sr * is2, -si * is2);              // 1/s~
+}
+__device__ __forceinline__ float taylor_coef(int it) {  // Scatter.h:73
+  return it == 0 ? 1.f : it == 1 ? 0.5f : it == 2 ? 0.375f : it == 3 ? 0.3125f : 0.2734375f;
+}
+
+// K = scale * F_x( U * ds ) on the model region (x < nx, y < ny), zero elsewhere
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_rows_ic_fwd(KArgs a) {
+  using C = BRowCfg<N>;
+  using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
+  __shared__ float2 lds[kWaves * C::LDSW];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nrows = a.np * a.NY;
+  const int rb = (blockIdx.x * kWaves + wave) * C::NB;
+  if (rb >= nrows) return;
+  float2 v[C::V];
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) {
+    int b, i;
+    edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
+    const int lr = rb + b;
+    v[e] = czero();
+    if (lr < nrows) {
+      const int p = lr / a.NY, iy = lr - p * a.NY;
+      if (iy < a.ny && i < a.nx) {
+        const float2 u = a.bg[(size_t)lr * N + i];
+        const float2 m = a.dslow[((size_t)plane_iw(a, p) * a.ny + iy) * a.nx + i];
+        v[e] = cmul(u, m);
+      }
+    }
+  }
+  F::run(v, lds + wave * C::LDSW, a.twx, lane);
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) {
+    int b, i;
+    edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
+    const int lr = rb + b;
+    if (lr < nrows) a.K[(size_t)lr * N + i] = cscale(v[e], a.scale);
+  }
+}
+
+// Sum of the Taylor terms in space:  y = -i (w dz / 2) sum_it f_it * F_x^-1(W_it).
+// BORN = true : K = scale * F_x( taper * (y + addin) )   (feeds the PSPI column pass)
+// BORN = false: out = (add ? out : 0) + y                (standalone Scatter forward)
+template <int N, bool BORN>
+__global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
+  using C = BRowCfg<N>;
+  using Fi = WaveFft<N, C::NB, false, true, PadLds<N>>;
+  using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
+  __shared__ float2 lds[kWaves * C::LDSW];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nrows = a.np * a.NY;
+  const int rb = (blockIdx.x * kWaves + wave) * C::NB;
+  if (rb >= nrows) return;
+  float2* wl = lds + wave * C::LDSW;
+  float2 acc[C::V], pw[C::V], inv[C::V];
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) {
+    int b, i;
+    edge_pos<N, C::NB, Fi::P::last(), false>(lane, e, b, i);
+    const int lr = rb + b;
+    acc[e] = czero();
+    pw[e] = czero();
+    inv[e] = czero();
+    if (lr < nrows) {
+      const int p = lr / a.NY;
+      scatter_factors(a, plane_iw(a, p), lr - p * a.NY, i, pw[e], inv[e]);
+    }
+  }
+  for (int it = 0; it < a.nterm; ++it) {
+    const float2* W = a.W + (size_t)it * a.wstride;
+    float2 x[C::V];
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      int b, i;
+      edge_pos<N, C::NB, Fi::P::first(), false>(lane, e, b, i);
+      const int lr = rb + b;
+      x[e] = lr < nrows ? W[(size_t)lr * N + i] : czero();
+    }
+    Fi::run(x, wl, a.twx, lane);
+    const float c = taylor_coef(it);
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      acc[e] = cadd(acc[e], cscale(cmul(x[e], pw[e]), c));   // c_it s~^(-1/2-it) * term
+      pw[e] = cmul(pw[e], inv[e]);
+    }
+  }
+  // acc *= -i * (w dz / 2) * scale     (propagator/src/scatter.cu:155-195)
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) {
+    int b, i;
+    edge_pos<N, C::NB, Fi::P::last(), false>(lane, e, b, i);
+    const int lr = rb + b;
+    if (lr < nrows) {
+      const int p = lr / a.NY;
+      const float h = 0.5f * a.w[plane_iw(a, p)] * a.dz * a.scale;
+      acc[e] = make_float2(h * acc[e].y, -h * acc[e].x);
+    }
+  }
+  if constexpr (!BORN) {
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      int b, i;
+      edge_pos<N, C::NB, Fi::P::last(), false>(lane, e, b, i);
+      const int lr = rb + b;
+      if (lr < nrows) {
+        const size_t idx = (size_t)lr * N + i;
+        float2 y = acc[e];
+        if (a.add) y = cadd(y, a.out[idx]);
+        a.out[idx] = y;
+      }
+    }
+  } else {
+    static_assert(Fi::P::last() == F::P::first(), "register hand-off needs mirrored plans");
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      int b, i;
+      edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
+      const int lr = rb + b;
+      if (lr < nrows) {
+        float2 y = cadd(acc[e], a.addin[(size_t)lr * N + i]);
+        acc[e] = cscale(y, a.tapx[i] * a.tapy[lr % a.NY]);
+      }
+    }
+    F::run(acc, wl, a.twx, lane);
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      int b, i;
+      edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
+      const int lr = rb + b;
+      if (lr < nrows) a.K[(size_t)lr * N + i] = cscale(acc[e], a.scale);
+    }
+  }
+}
+
+// W_it = scale * F_x( conj(f_it) * (+i w dz / 2) * in )   (Scatter adjoint, first half)
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
+  using C = BRowCfg<N>;
+  using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
+  __shared__ float2 lds[kWaves * C::LDSW];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nrows = a.np * a.NY;
+  const int rb = (blockIdx.x * kWaves + wave) * C::NB;
+  if (rb >= nrows) return;
+  float2* wl = lds + wave * C::LDSW;
+  float2 u[C::V], pw[C::V], inv[C::V];
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) {
+    int b, i;
+    edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
+    const int lr = rb + b;
+    u[e] = czero();
+    pw[e] = czero();
+    inv[e] = czero();
+    if (lr < nrows) {
+      const int p = lr / a.NY;
+      const float h = 0.5f * a.w[plane_iw(a, p)] * a.dz;   // scatter.cu:197-238
+      const float2 d = a.in[(size_t)lr * N + i];
+      u[e] = make_float2(-h * d.y, h * d.x);
+      scatter_factors(a, plane_iw(a, p), lr - p * a.NY, i, pw[e], inv[e]);
+    }
+  }
+  for (int it = 0; it < a.nterm; ++it) {
+    float2 x[C::V];
+    const float c = taylor_coef(it);
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      x[e] = cscale(cmulc(u[e], pw[e]), c);                // conj(c_it s~^(-1/2-it)) * u
+      pw[e] = cmul(pw[e], inv[e]);
+    }
+    F::run(x, wl, a.twx, lane);
+    float2* W = a.W + (size_t)it * a.wstride;
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      int b, i;
+      edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
+      const int lr = rb + b;
+      if (lr < nrows) W[(size_t)lr * N + i] = cscale(x[e], a.scale);
+    }
+  }
+}
+
+// ds[iw, y, x] (+)= conj(U) * scale * F_x^-1(K) on the model region.
+// Launched over planes of a single shot, so every model sample is owned by
+// exactly one lane: no atomics, deterministic (the reference's ic_adj uses
+// float atomicAdd, imaging_condition.cu:103-104).
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_rows_ic_adj(KArgs a) {
+  using C = BRowCfg<N>;
+  using Fi = WaveFft<N, C::NB, false, true, PadLds<N>>;
+  __shared__ float2 lds[kWaves * C::LDSW];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nrows = a.np * a.NY;
+  const int rb = (blockIdx.x * kWaves + wave) * C::NB;
+  if (rb >= nrows) return;
+  float2 x[C::V];
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) {
+    int b, i;
+    edge_pos<N, C::NB, Fi::P::first(), false>(lane, e, b, i);
+    const int lr = rb + b;
+    x[e] = lr < nrows ? a.K[(size_t)lr * N + i] : czero();
+  }
+  Fi::run(x, lds + wave * C::LDSW, a.twx, lane);
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) {
+    int b, i;
+    edge_pos<N, C::NB, Fi::P::last(), false>(lane, e, b, i);
+    const int lr = rb + b;
+    if (lr < nrows) {
+      const int p = lr / a.NY, iy = lr - p * a.NY;
+      if (iy < a.ny && i < a.nx) {
+        const size_t m = ((size_t)plane_iw(a, p) * a.ny + iy) * a.nx + i;
+        float2 y = cmulc(cscale(x[e], a.scale), a.bg[(size_t)lr * N + i]);
+        a.aux[m] = a.add ? cadd(a.aux[m], y) : y;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ point-wise Born helpers
+// img (+)= sum_{s, w} conj(U) * R over the model region: one thread per (y, x),
+// fixed summation order (the reduction of the migration image).
+__global__ __launch_bounds__(kThreads) void k_image(const float2* __restrict__ U, const float2* __restrict__ R,
+                                                    float2* img, int NX, int NY, int NW, int NS, int nx, int ny, int add) {
+  const int t = blockIdx.x * kThreads + threadIdx.x;
+  if (t >= nx * ny) return;
+  const int ix = t % nx, iy = t / nx;
+  float2 acc = czero();
+  for (int p = 0; p < NW * NS; ++p) {
+    const size_t idx = ((size_t)p * NY + iy) * NX + ix;
+    acc = cadd(acc, cmulc(R[idx], U[idx]));
+  }
+  img[t] = add ? cadd(img[t], acc) : acc;
+}
+
+// Standalone imaging condition (propagator/src/imaging_condition.cu):
+// forward D = (add ? D : 0) + U * img on the model region (padding: 0 or kept)
+__global__ __launch_bounds__(kThreads) void k_ic_fwd(const float2* __restrict__ img, float2* D, const float2* __restrict__ U,
+                                                     int NX, int NY, int NW, int NS, int nx, int ny, int add) {
+  const size_t n = (size_t)NX * NY * NW * NS;
+  for (size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x; idx < n; idx += (size_t)gridDim.x * kThreads) {
+    const int ix = (int)(idx % NX);
+    const size_t t = idx / NX;
+    const int iy = (int)(t % NY);
+    const int iw = (int)((t / NY) % NW);
+    float2 y = add ? D[idx] : czero();
+    if (ix < nx && iy < ny) y = cadd(y, cmul(U[idx], img[((size_t)iw * ny + iy) * nx + ix]));
+    D[idx] = y;
+  }
+}
+// adjoint img[w, y, x] (+)= sum_s conj(U) * D, one thread per image sample
+__global__ __launch_bounds__(kThreads) void k_ic_adj(float2* img, const float2* __restrict__ D, const float2* __restrict__ U,
+                                                     int NX, int NY, int NW, int NS, int nx, int ny, int add) {
+  const size_t n = (size_t)nx * ny * NW;
+  for (size_t t = (size_t)blockIdx.x * kThreads + threadIdx.x; t < n; t += (size_t)gridDim.x * kThreads) {
+    const int ix = (int)(t % nx);
+    const int iy = (int)((t / nx) % ny);
+    const int iw = (int)(t / ((size_t)nx * ny));
+    float2 acc = czero();
+    for (int s = 0; s < NS; ++s) {
+      const size_t idx = (((size_t)s * NW + iw) * NY + iy) * NX + ix;
+      acc = cadd(acc, cmulc(D[idx], U[idx]));
+    }
+    img[t] = add ? cadd(img[t], acc) : acc;
+  }
+}
+
+// ------------------------------------------------------------------ injection
+// Forward (traces -> wavefield, propagator/src/injection.cu:7-96) as a gather
+// over precomputed target nodes: every node of this depth level owns a
+// segment of (trace, weight) contributions, summed in a fixed order, so the
+// scatter is race-free and deterministic (the reference's kernel does a
+// non-atomic read-modify-write, injection.cu:89-92).
+__global__ __launch_bounds__(kThreads) void k_inj_fwd(const float2* __restrict__ tr, float2* wf, const long long* __restrict__ seg_base,
+                                                      const int* __restrict__ seg_off, const int* __restrict__ ent_trace,
+                                                      const float* __restrict__ ent_w, int s0, int nseg, int NW, long long plane) {
+  const long long n = (long long)nseg * NW;
+  for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < n; t += (long long)gridDim.x * kThreads) {
+    const int sg = s0 + (int)(t / NW), iw = (int)(t % NW);
+    float2 acc = czero();
+    for (int k = seg_off[sg]; k < seg_off[sg + 1]; ++k)
+      acc = cadd(acc, cscale(tr[(size_t)ent_trace[k] * NW + iw], ent_w[k]));
+    const long long idx = seg_base[sg] + (long long)iw * plane;
+    wf[idx] = cadd(wf[idx], acc);
+  }
+}
+// Adjoint (wavefield -> traces, injection.cu:98-187): each active trace gathers
+// its four corners.
+__global__ __launch_bounds__(kThreads) void k_inj_adj(float2* tr, const float2* __restrict__ wf, const int* __restrict__ act_trace,
+                                                      const long long* __restrict__ act_base, const float4* __restrict__ act_w,
+                                                      int a0, int nact, int NW, int NX, long long plane) {
+  const long long n = (long long)nact * NW;
+  for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < n; t += (long long)gridDim.x * kThreads) {
+    const int k = a0 + (int)(t / NW), iw = (int)(t % NW);
+    const long long b = act_base[k] + (long long)iw * plane;
+    const float4 w = act_w[k];
+    float2 v = cscale(wf[b], w.x);
+    v = cadd(v, cscale(wf[b + 1], w.y));
+    v = cadd(v, cscale(wf[b + NX], w.z));
+    v = cadd(v, cscale(wf[b + NX + 1], w.w));
+    const size_t o = (size_t)act_trace[k] * NW + iw;
+    tr[o] = cadd(tr[o], v);
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+template <class Fn>
+static hipError_t bdispatch(int n, Fn&& fn) {
+  switch (n) {
+#define CASE(S) case S: return fn(std::integral_constant<int, S>{});
+    CASE(8) CASE(12) CASE(16) CASE(24) CASE(32) CASE(48) CASE(64) CASE(96) CASE(128) CASE(192)
+    CASE(256) CASE(384) CASE(512) CASE(768) CASE(1024)
+#undef CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+static dim3 brow_grid(int N, int nrows) {
+  const int nb = row_nb(N);
+  const int waves = (nrows + nb - 1) / nb;
+  return dim3((waves + kWaves - 1) / kWaves);
+}
+static inline dim3 bflat(long long n) {
+  long long g = (n + kThreads - 1) / kThreads;
+  if (g > (1ll << 30)) g = 1ll << 30;
+  if (g < 1) g = 1;
+  return dim3((unsigned)g);
+}
+
+hipError_t launch_rows_ic_fwd(const KArgs& a, hipStream_t st) {
+  return bdispatch(a.NX, [&](auto nc) -> hipError_t {
+    constexpr int N = decltype(nc)::value;
+    hipLaunchKernelGGL((k_rows_ic_fwd<N>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    return hipGetLastError();
+  });
+}
+hipError_t launch_rows_scatter(bool born, const KArgs& a, hipStream_t st) {
+  return bdispatch(a.NX, [&](auto nc) -> hipError_t {
+    constexpr int N = decltype(nc)::value;
+    if (born) hipLaunchKernelGGL((k_rows_scatter<N, true>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    else hipLaunchKernelGGL((k_rows_scatter<N, false>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    return hipGetLastError();
+  });
+}
+hipError_t launch_rows_scatter_adj(const KArgs& a, hipStream_t st) {
+  return bdispatch(a.NX, [&](auto nc) -> hipError_t {
+    constexpr int N = decltype(nc)::value;
+    hipLaunchKernelGGL((k_rows_scatter_adj<N>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    return hipGetLastError();
+  });
+}
+hipError_t launch_rows_ic_adj(const KArgs& a, hipStream_t st) {
+  return bdispatch(a.NX, [&](auto nc) -> hipError_t {
+    constexpr int N = decltype(nc)::value;
+    hipLaunchKernelGGL((k_rows_ic_adj<N>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    return hipGetLastError();
+  });
+}
+hipError_t launch_image(const float2* U, const float2* R, float2* img, int NX, int NY, int NW, int NS, int nx,
+                        int ny, int add, hipStream_t st) {
+  hipLaunchKernelGGL(k_image, dim3((nx * ny + kThreads - 1) / kThreads), dim3(kThreads), 0, st, U, R, img, NX, NY, NW,
+                     NS, nx, ny, add);
+  return hipGetLastError();
+}
+hipError_t launch_ic_fwd(const float2* img, float2* D, const float2* U, int NX, int NY, int NW, int NS, int nx, int ny,
+                         int add, hipStream_t st) {
+  hipLaunchKernelGGL(k_ic_fwd, bflat((long long)NX * NY * NW * NS), dim3(kThreads), 0, st, img, D, U, NX, NY, NW, NS,
+                     nx, ny, add);
+  return hipGetLastError();
+}
+hipError_t launch_ic_adj(float2* img, const float2* D, const float2* U, int NX, int NY, int NW, int NS, int nx, int ny,
+                         int add, hipStream_t st) {
+  hipLaunchKernelGGL(k_ic_adj, bflat((long long)nx * ny * NW), dim3(kThreads), 0, st, img, D, U, NX, NY, NW, NS, nx,
+                     ny, add);
+  return hipGetLastError();
+}
+hipError_t launch_inj_fwd(const float2* tr, float2* wf, const long long* seg_base, const int* seg_off,
+                          const int* ent_trace, const float* ent_w, int s0, int nseg, int NW, long long plane,
+                          hipStream_t st) {
+  if (nseg <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_inj_fwd, bflat((long long)nseg * NW), dim3(kThreads), 0, st, tr, wf, seg_base, seg_off,
+                     ent_trace, ent_w, s0, nseg, NW, plane);
+  return hipGetLastError();
+}
+hipError_t launch_inj_adj(float2* tr, const float2* wf, const int* act_trace, const long long* act_base,
+                          const float4* act_w, int a0, int nact, int NW, int NX, long long plane, hipStream_t st) {
+  if (nact <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_inj_adj, bflat((long long)nact * NW), dim3(kThreads), 0, st, tr, wf, act_trace, act_base, act_w,
+                     a0, nact, NW, NX, plane);
+  return hipGetLastError();
+}
+
+}  // namespace fwix

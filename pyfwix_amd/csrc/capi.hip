@@ -141,23 +141,31 @@ PwArgs Tables::pwargs() const {
 int auto_chunk(const fwix_dims& d, int nterm, int requested) {
   const int planes = d.NW * d.NS;
   if (requested > 0) return requested < planes ? requested : planes;
-  // keep the chunk's intermediates (spectrum + nterm fan-out planes) around
-  // 128 MiB so that they stay in the 256 MB Infinity Cache between passes
+  // enough planes that a column pass launches >= 512 workgroups (2 per CU),
+  // at least what keeps the chunk's intermediates (spectrum + nterm fan-out
+  // planes) within 128 MiB, and at most 2 GiB of scratch
   const double per_plane = (double)(1 + nterm) * d.NX * d.NY * 8.0;
-  int c = (int)(128.0 * 1024 * 1024 / per_plane);
+  const int cw = fft_size_supported(d.NY) ? col_tile_width(d.NY) : 16;
+  const int blocks_per_plane = (d.NX + cw - 1) / cw;
+  int c = (512 + blocks_per_plane - 1) / blocks_per_plane;
+  const int c_cache = (int)(128.0 * 1024 * 1024 / per_plane);
+  if (c_cache > c) c = c_cache;
+  const int c_max = (int)(2048.0 * 1024 * 1024 / per_plane);
+  if (c > c_max) c = c_max;
   if (c < 1) c = 1;
   return c < planes ? c : planes;
 }
 
 // ------------------------------------------------------------------ one step engine
-void StepEngine::init(int method_, const fwix_dims& d, const fwix_params& p) {
+void StepEngine::init(int method_, const fwix_dims& d, const fwix_params& p, int wterms) {
   method = method_;
   params = p;
   tables.init(d, method == FWIX_PSPI ? p.taperx : 0, method == FWIX_PSPI ? p.tapery : 0);
-  chunk = auto_chunk(d, p.nref, p.planes_per_chunk);
+  if (wterms < p.nref) wterms = p.nref;
+  chunk = auto_chunk(d, wterms, p.planes_per_chunk);
   const size_t plane = (size_t)d.NX * d.NY;
   K.alloc(plane * chunk * sizeof(float2));
-  W.alloc(plane * chunk * p.nref * sizeof(float2));
+  W.alloc(plane * chunk * wterms * sizeof(float2));
 }
 
 void StepEngine::check_ready(int iz) const {

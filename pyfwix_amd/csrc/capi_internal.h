@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "fwix_hip.h"
 #include "fwix_launch.h"
@@ -86,10 +87,29 @@ struct StepEngine {
   const uint8_t* labels = nullptr;  // borrowed (nz, NW, NY, NX)
   DevBuf K, W;
   int chunk = 1;
-  void init(int method, const fwix_dims& d, const fwix_params& p);
+  // wterms: planes per chunk of the W scratch (>= nref; the Born path also
+  // stores its taylor+1 scattering terms there)
+  void init(int method, const fwix_dims& d, const fwix_params& p, int wterms = 0);
   void check_ready(int iz) const;
   KArgs base(int iz, int c0, int np) const;
   void apply(bool adjoint, int iz, int add, const float2* in, float2* out, hipStream_t st);
+};
+
+
+// Source / receiver injection plan (propagator/src/injection.cu): per depth
+// level, the target nodes with their (trace, weight) contributions (forward,
+// gathered per node) and the active traces with their four corners
+// (adjoint).  Built on the host from float32 coordinates with the kernel's
+// own float arithmetic.
+struct InjPlan {
+  int ntrace = 0, nlev = 0, NW = 0, NX = 0;
+  long long plane = 0;
+  std::vector<int> seg_lvl, act_lvl;  // (nlev + 1) offsets
+  DevBuf seg_base, seg_off, ent_trace, ent_w, act_trace, act_base, act_w;
+  void build(const fwix_dims& d, int nlev, int ntrace, const float* cx, const float* cy, const float* cz,
+             const int* ids);
+  void forward(int iz, const float2* traces, float2* wfld, hipStream_t st) const;  // wfld += Inj_iz(traces)
+  void adjoint(int iz, float2* traces, const float2* wfld, hipStream_t st) const;  // traces += Rec_iz^T wfld
 };
 
 }  // namespace fwix

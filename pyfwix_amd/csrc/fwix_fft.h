@@ -225,6 +225,10 @@ struct WaveFft {
   // on return it holds the spectrum at last-stage output positions
   // (edge_pos<..., P::last()>).  Unnormalized.
   __device__ __forceinline__ static void run(float2* v, float2* lds, const float2* __restrict__ tw, int lane) {
+    // Keep the twiddle loads next to their use: without this fence the
+    // compiler hoists every stage's (loop-invariant) twiddles out of the
+    // per-reference loops of the fused kernels and runs out of registers.
+    asm volatile("" ::: "memory");
     stages<0>(v, lds, tw, lane);
   }
 };

@@ -13,23 +13,23 @@ constexpr int row_nb(int N) {
   const int base = size_has3(N) ? 768 : 512;
   return N >= base ? 1 : base / N;
 }
-// Columns per 256-thread tile in the column passes (4 waves x CW/4 columns).
+// Columns per column-pass tile: 16 waves x CW/16 columns, chosen so each lane
+// holds V = CW*N/1024 values: 8 (2^a, N <= 512), 16 (1024) or 12 (3*2^a).
 constexpr int col_cw(int N) {
-  if (N < 128) return (size_has3(N) ? 3072 : 2048) / N;
-  if (N <= 512) return 16;
-  return 8;  // 768, 1024: 2 columns per wave keeps V <= 32 without spills
+  const int cw = (size_has3(N) ? 12288 : 8192) / N;
+  return cw < 16 ? 16 : cw;
 }
+constexpr int kColThreads = 1024;
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 
 // LDS layouts -------------------------------------------------------------
+// Stockham exchanges write with stride R inside a batch; one pad slot every
+// 32 complex samples spreads those writes over all 64 LDS banks.
+constexpr int padded(int N) { return N + N / 32; }
 template <int N>
-struct RowLds {  // one wave's NB rows, row-major
-  __device__ __forceinline__ static int addr(int b, int i) { return b * N + i; }
-};
-template <int CWP>
-struct ColLds {  // tile [N][CWP], base pointer already offset to the wave's first column
-  __device__ __forceinline__ static int addr(int b, int i) { return i * CWP + b; }
+struct PadLds {  // one wave's NB batches, batch-major, padded
+  __device__ __forceinline__ static int addr(int b, int i) { return b * padded(N) + i + (i >> 5); }
 };
 
 // One argument block for every step kernel (passed by value).
@@ -64,33 +64,71 @@ struct KArgs {
 
 __device__ __forceinline__ int plane_iw(const KArgs& a, int p_local) { return (a.p0 + p_local) % a.NW; }
 
+// sin and cos of x for |x| up to ~1e4: Cody-Waite reduction by pi/2 with a
+// three-part constant (1.5703125 + 4.837512969970703e-4 + 7.54979012640433e-8,
+// the first two short enough that n*P is exact), then Taylor polynomials on
+// [-pi/4, pi/4] (truncation below 3e-8).  Much smaller than the libm
+// sincosf, whose large-argument path costs registers in the unrolled
+// per-sample loops of the fused kernels.
+__device__ __forceinline__ void fast_sincos(float x, float* s, float* c) {
+  const float n = rintf(x * 0.636619772367581343f);
+  float r = fmaf(-n, 1.5703125f, x);
+  r = fmaf(-n, 4.837512969970703e-4f, r);
+  r = fmaf(-n, 7.54979012640433e-8f, r);
+  const float r2 = r * r;
+  float ps = fmaf(r2, 2.75573192e-6f, -1.98412698e-4f);   // 1/9!, -1/7!
+  ps = fmaf(r2, ps, 8.33333333e-3f);                      // 1/5!
+  ps = fmaf(r2, ps, -1.66666667e-1f);                     // -1/3!
+  const float sr = fmaf(r * r2, ps, r);
+  float pc = fmaf(r2, 2.48015873e-5f, -1.38888889e-3f);   // 1/8!, -1/6!
+  pc = fmaf(r2, pc, 4.16666667e-2f);                      // 1/4!
+  pc = fmaf(r2, pc, -0.5f);
+  const float cr = fmaf(r2, pc, 1.f);
+  const int q = (int)n & 3;
+  const float ss = (q & 1) ? cr : sr;
+  const float cc = (q & 1) ? sr : cr;
+  *s = (q & 2) ? -ss : ss;
+  *c = ((q + 1) & 2) ? -cc : cc;
+}
+
 // Phase-shift multiplier exp(-i kz dz) for one reference slowness.
 // Reference formula (propagator/src/phase_shift.cu:36-59): a = w2 Re s - k^2,
 // b = w2 (Im s - eps Re s), kr = sqrt((c+a)/2), ki = sqrt((c-a)/2),
-// c = sqrt(a^2+b^2).  Evaluated here as the numerically stable complex square
-// root (the smaller of kr, ki from kr*ki = |b|/2), identical in exact
-// arithmetic but free of the float32 cancellation in c-a / c+a.
+// c = sqrt(a^2+b^2).  Evaluated as the numerically stable complex square
+// root (the smaller of kr, ki from kr*ki = |b|/2): identical in exact
+// arithmetic, free of the float32 cancellation in c-a / c+a.
 // MODE 0: forward  exp(-ki dz) exp(-i kr dz)
 // MODE 1: adjoint  kr -> -kr if b > 0; exp(-ki dz) exp(+i kr dz)
 // MODE 2: inverse  kr -> -kr if b > 0; exp(+ki dz) exp(+i kr dz)
+// b depends only on (frequency, reference): callers test b == 0 once and
+// use ps_real (kz real or purely imaginary) for the whole plane.
+template <int MODE>
+__device__ __forceinline__ float2 ps_real(float a, float dz) {
+  const float t = __builtin_amdgcn_sqrtf(fabsf(a)) * dz;
+  float sn, cs;
+  fast_sincos(t, &sn, &cs);
+  const float e = __expf(MODE == 2 ? t : -t);
+  if (a >= 0.f) return MODE == 0 ? make_float2(cs, -sn) : make_float2(cs, sn);
+  return make_float2(e, 0.f);
+}
+template <int MODE>
+__device__ __forceinline__ float2 ps_general(float a, float b, float dz) {
+  const float c = __builtin_amdgcn_sqrtf(fmaf(a, a, b * b));
+  const float big = __builtin_amdgcn_sqrtf(0.5f * (c + fabsf(a)));
+  const float small = 0.5f * fabsf(b) * __builtin_amdgcn_rcpf(big);
+  float kr = a >= 0.f ? big : small;
+  const float ki = a >= 0.f ? small : big;
+  if (MODE != 0 && b > 0.f) kr = -kr;
+  const float att = __expf((MODE == 2 ? ki : -ki) * dz);
+  float sn, cs;
+  fast_sincos(kr * dz, &sn, &cs);
+  return MODE == 0 ? make_float2(att * cs, -att * sn) : make_float2(att * cs, att * sn);
+}
 template <int MODE>
 __device__ __forceinline__ float2 ps_mult(float w2, float sre, float sim, float k2, float dz, float eps) {
   const float a = fmaf(w2, sre, -k2);
   const float b = w2 * (sim - eps * sre);
-  const float c = sqrtf(fmaf(a, a, b * b));
-  float kr, ki;
-  if (a >= 0.f) {
-    kr = sqrtf(0.5f * (c + a));
-    ki = kr > 0.f ? 0.5f * fabsf(b) / kr : 0.f;
-  } else {
-    ki = sqrtf(0.5f * (c - a));
-    kr = ki > 0.f ? 0.5f * fabsf(b) / ki : 0.f;
-  }
-  if (MODE != 0 && b > 0.f) kr = -kr;
-  const float att = expf((MODE == 2 ? ki : -ki) * dz);
-  float sn, cs;
-  sincosf(kr * dz, &sn, &cs);
-  return MODE == 0 ? make_float2(att * cs, -att * sn) : make_float2(att * cs, att * sn);
+  return b == 0.f ? ps_real<MODE>(a, dz) : ps_general<MODE>(a, b, dz);
 }
 
 }  // namespace fwix

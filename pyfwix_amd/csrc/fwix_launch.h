@@ -22,6 +22,7 @@ struct PwArgs {
 };
 
 bool fft_size_supported(int n);
+int col_tile_width(int n);  // columns per column-pass workgroup for an n-point column FFT
 
 // fused step passes (step_kernels.hip)
 hipError_t launch_rows(int kind, const KArgs& a, hipStream_t st);      // 0 fwd, 1 inv, 2 fwd+pre-taper, 3 inv+post-taper
@@ -37,5 +38,22 @@ hipError_t launch_select(const float2* m, float2* d, const PwArgs& a, hipStream_
 hipError_t launch_taper(const float2* m, float2* d, const PwArgs& a, hipStream_t st);
 hipError_t launch_axpy(const float2* m, float2* d, size_t n, float alpha, int add, hipStream_t st);
 hipError_t launch_zero(float2* d, size_t n, hipStream_t st);
+
+// Born / imaging / injection kernels (born_kernels.hip)
+hipError_t launch_rows_ic_fwd(const KArgs& a, hipStream_t st);
+hipError_t launch_rows_scatter(bool born, const KArgs& a, hipStream_t st);
+hipError_t launch_rows_scatter_adj(const KArgs& a, hipStream_t st);
+hipError_t launch_rows_ic_adj(const KArgs& a, hipStream_t st);
+hipError_t launch_image(const float2* U, const float2* R, float2* img, int NX, int NY, int NW, int NS, int nx,
+                        int ny, int add, hipStream_t st);
+hipError_t launch_ic_fwd(const float2* img, float2* D, const float2* U, int NX, int NY, int NW, int NS, int nx, int ny,
+                         int add, hipStream_t st);
+hipError_t launch_ic_adj(float2* img, const float2* D, const float2* U, int NX, int NY, int NW, int NS, int nx, int ny,
+                         int add, hipStream_t st);
+hipError_t launch_inj_fwd(const float2* tr, float2* wf, const long long* seg_base, const int* seg_off,
+                          const int* ent_trace, const float* ent_w, int s0, int nseg, int NW, long long plane,
+                          hipStream_t st);
+hipError_t launch_inj_adj(float2* tr, const float2* wf, const int* act_trace, const long long* act_base,
+                          const float4* act_w, int a0, int nact, int NW, int NX, long long plane, hipStream_t st);
 
 }  // namespace fwix

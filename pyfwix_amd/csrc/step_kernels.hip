@@ -27,14 +27,24 @@ template <int N>
 struct RowCfg {
   static constexpr int NB = row_nb(N);
   static constexpr int V = NB * N / 64;
-  static constexpr int LDSW = NB * N;
+  static constexpr int LDSW = NB * padded(N);
 };
+// Column passes: 16 waves x NB columns.  The [N][CW+1] tile that stages the
+// coalesced loads / stores is reused, between block barriers, as eight
+// private padded exchange regions (one per wave) for the Stockham stages.
 template <int N>
 struct ColCfg {
+  static constexpr int THREADS = kColThreads;
+  static constexpr int WAVES = THREADS / 64;
   static constexpr int CW = col_cw(N);
-  static constexpr int NB = CW / kWaves;
+  static constexpr int NB = CW / WAVES;
   static constexpr int V = NB * N / 64;
   static constexpr int CWP = CW + 1;
+  static constexpr int NROW = N;
+  static constexpr int TILE = N * CWP;
+  static constexpr int PRIV = NB * padded(N);
+  static constexpr int SMEM = TILE > WAVES * PRIV ? TILE : WAVES * PRIV;
+  static_assert(NB >= 1 && V >= 4, "column tile too narrow");
 };
 
 // ------------------------------------------------------------------ row passes
@@ -42,7 +52,7 @@ struct ColCfg {
 template <int N, bool INV, bool PRE_TAPER, bool POST_TAPER>
 __global__ __launch_bounds__(kThreads) void k_rows(KArgs a) {
   using C = RowCfg<N>;
-  using F = WaveFft<N, C::NB, false, INV, RowLds<N>>;
+  using F = WaveFft<N, C::NB, false, INV, PadLds<N>>;
   __shared__ float2 lds[kWaves * C::LDSW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nrows = a.np * a.NY;
@@ -83,7 +93,7 @@ __global__ __launch_bounds__(kThreads) void k_rows(KArgs a) {
 template <int N>
 __global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
   using C = RowCfg<N>;
-  using F = WaveFft<N, C::NB, false, false, RowLds<N>>;
+  using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
   __shared__ float2 lds[kWaves * C::LDSW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nrows = a.np * a.NY;
@@ -126,7 +136,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
 template <int N>
 __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
   using C = RowCfg<N>;
-  using F = WaveFft<N, C::NB, false, true, RowLds<N>>;
+  using F = WaveFft<N, C::NB, false, true, PadLds<N>>;
   __shared__ float2 lds[kWaves * C::LDSW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nrows = a.np * a.NY;
@@ -147,16 +157,23 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
       lab[e] = a.labels[((size_t)plane_iw(a, p) * a.NY + iy) * N + i];
     }
   }
-  for (int r = 0; r < a.nref; ++r) {
+  float2 nxt[C::V];
+  auto fetch = [&](int r) {
     const float2* W = a.W + (size_t)r * a.wstride;
-    float2 x[C::V];
 #pragma unroll
     for (int e = 0; e < C::V; ++e) {
       int b, i;
       edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
       const int lr = rb + b;
-      x[e] = lr < nrows ? W[(size_t)lr * N + i] : czero();
+      nxt[e] = lr < nrows ? W[(size_t)lr * N + i] : czero();
     }
+  };
+  fetch(0);
+  for (int r = 0; r < a.nref; ++r) {
+    float2 x[C::V];
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) x[e] = nxt[e];
+    if (r + 1 < a.nref) fetch(r + 1);
     F::run(x, wl, a.twx, lane);
 #pragma unroll
     for (int e = 0; e < C::V; ++e)
@@ -178,162 +195,243 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
 }
 
 // ------------------------------------------------------------------ column passes
-template <int N, int CW, int CWP>
+// Pointwise spectral multiplier of term r at (kx, ky) of frequency iw.
+// MULT 0 / 1: forward / adjoint phase shift of reference slowness r
+// (propagator/src/phase_shift.cu); MULT 2: Born scattering wavenumber factor
+// (|k|/w)^(2r) = (k^2/w^2)^r (propagator/src/scatter.cu:23-59).
+// The term's uniform part (reference slowness, real/complex branch) is set up
+// once per (plane, term) so the unrolled per-sample loops carry no divergent
+// or if-converted paths.
+template <int MULT>
+struct TermMult {
+  float w2, wsre, b, iw2, dz;
+  int r;
+  bool real;
+  __device__ __forceinline__ TermMult(const KArgs& a, int r_, int iw, float w2_) : w2(w2_), r(r_), dz(a.dz) {
+    if constexpr (MULT == 2) {
+      iw2 = 1.f / w2;
+      real = true; wsre = 0.f; b = 0.f;
+    } else {
+      const float2 sr = a.sref[r * a.NW + iw];
+      wsre = w2 * sr.x;
+      b = w2 * (sr.y - a.eps * sr.x);
+      real = b == 0.f;
+      iw2 = 0.f;
+    }
+  }
+  template <bool REAL>
+  __device__ __forceinline__ float2 at(float kxv, float kyv) const {
+    const float k2 = fmaf(kxv, kxv, kyv * kyv);
+    if constexpr (MULT == 2) {
+      const float q = k2 * iw2;
+      float m = 1.f;
+      for (int i = 0; i < r; ++i) m *= q;
+      return make_float2(m, 0.f);
+    } else if constexpr (REAL) {
+      return ps_real<MULT>(wsre - k2, dz);
+    } else {
+      return ps_general<MULT>(wsre - k2, b, dz);
+    }
+  }
+};
+
+// Coalesced tile load / store: every thread first issues all of its global
+// loads into registers (PER = N*CW/THREADS of them), then writes LDS, so the
+// memory latency is paid once per tile rather than once per element.
+template <class C>
+struct TileRegs {
+  static constexpr int PER = C::NROW * C::CW / C::THREADS;
+  static_assert(PER * C::THREADS == C::NROW * C::CW, "tile must divide evenly over the threads");
+  float2 t[PER];
+  __device__ __forceinline__ void fetch(const float2* __restrict__ src, int NX, int c0) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = threadIdx.x + k * C::THREADS;
+      const int i = e / C::CW, c = e % C::CW, col = c0 + c;
+      t[k] = col < NX ? src[(size_t)i * NX + col] : czero();
+    }
+  }
+  __device__ __forceinline__ void commit(float2* tile) const {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = threadIdx.x + k * C::THREADS;
+      tile[(e / C::CW) * C::CWP + e % C::CW] = t[k];
+    }
+  }
+};
+template <class C>
 __device__ __forceinline__ void tile_load(float2* tile, const float2* __restrict__ src, int NX, int c0) {
-  for (int e = threadIdx.x; e < N * CW; e += kThreads) {
-    const int i = e / CW, c = e % CW, col = c0 + c;
-    tile[i * CWP + c] = col < NX ? src[(size_t)i * NX + col] : czero();
+  TileRegs<C> r;
+  r.fetch(src, NX, c0);
+  r.commit(tile);
+}
+template <class C>
+__device__ __forceinline__ void tile_store(const float2* tile, float2* __restrict__ dst, int NX, int c0, int add) {
+  constexpr int PER = C::NROW * C::CW / C::THREADS;
+  float2 t[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int e = threadIdx.x + k * C::THREADS;
+    t[k] = tile[(e / C::CW) * C::CWP + e % C::CW];
+  }
+  if (add) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = threadIdx.x + k * C::THREADS;
+      const int col = c0 + e % C::CW;
+      if (col < NX) t[k] = cadd(t[k], dst[(size_t)(e / C::CW) * NX + col]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int e = threadIdx.x + k * C::THREADS;
+    const int col = c0 + e % C::CW;
+    if (col < NX) dst[(size_t)(e / C::CW) * NX + col] = t[k];
   }
 }
-template <int N, int CW, int CWP>
-__device__ __forceinline__ void tile_store(const float2* tile, float2* __restrict__ dst, int NX, int c0, int add) {
-  for (int e = threadIdx.x; e < N * CW; e += kThreads) {
-    const int i = e / CW, c = e % CW, col = c0 + c;
-    if (col < NX) {
-      const size_t idx = (size_t)i * NX + col;
-      float2 y = tile[i * CWP + c];
-      if (add) y = cadd(y, dst[idx]);
-      dst[idx] = y;
-    }
+// this wave's columns at the first-stage positions of radix R, from the tile
+template <int N, class C, int R>
+__device__ __forceinline__ void tile_read(float2* v, const float2* tile, int wave, int lane) {
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) {
+    int b, i;
+    edge_pos<N, C::NB, R, false>(lane, e, b, i);
+    v[e] = tile[i * C::CWP + wave * C::NB + b];
+  }
+}
+template <int N, class C, int R>
+__device__ __forceinline__ void tile_write(float2* tile, const float2* v, int wave, int lane) {
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) {
+    int b, i;
+    edge_pos<N, C::NB, R, false>(lane, e, b, i);
+    tile[i * C::CWP + wave * C::NB + b] = v[e];
   }
 }
 
 // out = (add ? out : 0) + scale * F_y(in) on each plane (plain 2-D FFT column pass)
 template <int N, bool INV>
-__global__ __launch_bounds__(kThreads) void k_cols(KArgs a) {
+__global__ __launch_bounds__(ColCfg<N>::THREADS) void k_cols(KArgs a) {
   using C = ColCfg<N>;
-  using F = WaveFft<N, C::NB, true, INV, ColLds<C::CWP>>;
-  __shared__ float2 tile[N * C::CWP];
+  using F = WaveFft<N, C::NB, false, INV, PadLds<N>>;
+  __shared__ float2 smem[C::SMEM];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int p = blockIdx.y, c0 = blockIdx.x * C::CW;
   const size_t pb = (size_t)p * a.NX * N;
-  tile_load<N, C::CW, C::CWP>(tile, a.in + pb, a.NX, c0);
+  float2* priv = smem + wave * C::PRIV;
+  tile_load<C>(smem, a.in + pb, a.NX, c0);
   __syncthreads();
-  float2* wl = tile + wave * C::NB;
   float2 v[C::V];
-#pragma unroll
-  for (int e = 0; e < C::V; ++e) {
-    int b, i;
-    edge_pos<N, C::NB, F::P::first(), true>(lane, e, b, i);
-    v[e] = wl[i * C::CWP + b];
-  }
-  F::run(v, wl, a.twy, lane);
+  tile_read<N, C, F::P::first()>(v, smem, wave, lane);
+  __syncthreads();
+  F::run(v, priv, a.twy, lane);
   const float s = a.scale;  // column normalisation 1/sqrt(NY)
 #pragma unroll
-  for (int e = 0; e < C::V; ++e) {
-    int b, i;
-    edge_pos<N, C::NB, F::P::last(), true>(lane, e, b, i);
-    wl[i * C::CWP + b] = cscale(v[e], s);
-  }
+  for (int e = 0; e < C::V; ++e) v[e] = cscale(v[e], s);
   __syncthreads();
-  tile_store<N, C::CW, C::CWP>(tile, a.out + pb, a.NX, c0, a.add);
+  tile_write<N, C, F::P::last()>(smem, v, wave, lane);
+  __syncthreads();
+  tile_store<C>(smem, a.out + pb, a.NX, c0, a.add);
 }
 
-// cols_fanout: W_r = F_y^-1[Phi_r * F_y(K)] / NY for every reference r.
-// MODE 0: forward phase shift, 1: adjoint phase shift.
+// cols_fanout: W_r = F_y^-1[M_r * F_y(K)] / NY for every term r (M_r: term_mult<MODE>).
 template <int N, int MODE>
-__global__ __launch_bounds__(kThreads) void k_cols_fanout(KArgs a) {
+__global__ __launch_bounds__(ColCfg<N>::THREADS) void k_cols_fanout(KArgs a) {
   using C = ColCfg<N>;
-  using F = WaveFft<N, C::NB, true, false, ColLds<C::CWP>>;
-  using Fi = WaveFft<N, C::NB, true, true, ColLds<C::CWP>>;
+  using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
+  using Fi = WaveFft<N, C::NB, false, true, PadLds<N>>;
   static_assert(F::P::last() == Fi::P::first(), "register hand-off needs mirrored plans");
-  __shared__ float2 tile[N * C::CWP];
+  __shared__ float2 smem[C::SMEM];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int p = blockIdx.y, c0 = blockIdx.x * C::CW;
   const size_t pb = (size_t)p * a.NX * N;
-  tile_load<N, C::CW, C::CWP>(tile, a.K + pb, a.NX, c0);
+  float2* priv = smem + wave * C::PRIV;
+  tile_load<C>(smem, a.K + pb, a.NX, c0);
   __syncthreads();
-  float2* wl = tile + wave * C::NB;
   float2 spec[C::V];
-#pragma unroll
-  for (int e = 0; e < C::V; ++e) {
-    int b, i;
-    edge_pos<N, C::NB, F::P::first(), true>(lane, e, b, i);
-    spec[e] = wl[i * C::CWP + b];
-  }
-  F::run(spec, wl, a.twy, lane);
+  tile_read<N, C, F::P::first()>(spec, smem, wave, lane);
+  __syncthreads();
+  F::run(spec, priv, a.twy, lane);
   const int iw = plane_iw(a, p);
   const float w2 = a.w2[iw];
   const float s2 = a.scale;  // 1/NY: forward and inverse column normalisation together
   const int colw = c0 + wave * C::NB;
-  for (int r = 0; r < a.nref; ++r) {
-    const float2 sr = a.sref[r * a.NW + iw];
+  for (int r = 0; r < a.nterm; ++r) {
+    const TermMult<MODE> tm(a, r, iw, w2);
     float2 x[C::V];
+    auto mult = [&](auto real) {
 #pragma unroll
-    for (int e = 0; e < C::V; ++e) {
-      int b, i;
-      edge_pos<N, C::NB, F::P::last(), true>(lane, e, b, i);
-      const int col = colw + b;
-      const float kxv = a.kx[col < a.NX ? col : 0], kyv = a.ky[i];
-      const float2 ph = ps_mult<MODE>(w2, sr.x, sr.y, fmaf(kxv, kxv, kyv * kyv), a.dz, a.eps);
-      x[e] = cscale(cmul(spec[e], ph), s2);
-    }
-    Fi::run(x, wl, a.twy, lane);
-#pragma unroll
-    for (int e = 0; e < C::V; ++e) {
-      int b, i;
-      edge_pos<N, C::NB, Fi::P::last(), true>(lane, e, b, i);
-      wl[i * C::CWP + b] = x[e];
-    }
+      for (int e = 0; e < C::V; ++e) {
+        int b, i;
+        edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
+        const int col = colw + b;
+        const float2 ph = tm.template at<decltype(real)::value>(a.kx[col < a.NX ? col : 0], a.ky[i]);
+        x[e] = cscale(cmul(spec[e], ph), s2);
+      }
+    };
+    if (tm.real) mult(std::true_type{});
+    else mult(std::false_type{});
+    Fi::run(x, priv, a.twy, lane);
     __syncthreads();
-    tile_store<N, C::CW, C::CWP>(tile, a.W + (size_t)r * a.wstride + pb, a.NX, c0, 0);
+    tile_write<N, C, Fi::P::last()>(smem, x, wave, lane);
+    __syncthreads();
+    tile_store<C>(smem, a.W + (size_t)r * a.wstride + pb, a.NX, c0, 0);
     __syncthreads();
   }
 }
 
-// cols_fanin: K = F_y^-1[sum_r Phi_r * F_y(W_r)] / NY.
+// cols_fanin: K = F_y^-1[sum_r M_r * F_y(W_r)] / NY (M_r: term_mult<MODE>).
 template <int N, int MODE>
-__global__ __launch_bounds__(kThreads) void k_cols_fanin(KArgs a) {
+__global__ __launch_bounds__(ColCfg<N>::THREADS) void k_cols_fanin(KArgs a) {
   using C = ColCfg<N>;
-  using F = WaveFft<N, C::NB, true, false, ColLds<C::CWP>>;
-  using Fi = WaveFft<N, C::NB, true, true, ColLds<C::CWP>>;
+  using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
+  using Fi = WaveFft<N, C::NB, false, true, PadLds<N>>;
   static_assert(F::P::last() == Fi::P::first(), "register hand-off needs mirrored plans");
-  __shared__ float2 tile[N * C::CWP];
+  __shared__ float2 smem[C::SMEM];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int p = blockIdx.y, c0 = blockIdx.x * C::CW;
   const size_t pb = (size_t)p * a.NX * N;
-  float2* wl = tile + wave * C::NB;
+  float2* priv = smem + wave * C::PRIV;
   const int iw = plane_iw(a, p);
   const float w2 = a.w2[iw];
   const int colw = c0 + wave * C::NB;
   float2 acc[C::V];
 #pragma unroll
   for (int e = 0; e < C::V; ++e) acc[e] = czero();
-  for (int r = 0; r < a.nref; ++r) {
-    tile_load<N, C::CW, C::CWP>(tile, a.W + (size_t)r * a.wstride + pb, a.NX, c0);
+  TileRegs<C> next;
+  next.fetch(a.W + pb, a.NX, c0);
+  for (int r = 0; r < a.nterm; ++r) {
+    next.commit(smem);
     __syncthreads();
+    // the next term's loads are in flight during this term's transform
+    if (r + 1 < a.nterm) next.fetch(a.W + (size_t)(r + 1) * a.wstride + pb, a.NX, c0);
     float2 v[C::V];
+    tile_read<N, C, F::P::first()>(v, smem, wave, lane);
+    __syncthreads();
+    F::run(v, priv, a.twy, lane);
+    const TermMult<MODE> tm(a, r, iw, w2);
+    auto mult = [&](auto real) {
 #pragma unroll
-    for (int e = 0; e < C::V; ++e) {
-      int b, i;
-      edge_pos<N, C::NB, F::P::first(), true>(lane, e, b, i);
-      v[e] = wl[i * C::CWP + b];
-    }
-    F::run(v, wl, a.twy, lane);
-    const float2 sr = a.sref[r * a.NW + iw];
-#pragma unroll
-    for (int e = 0; e < C::V; ++e) {
-      int b, i;
-      edge_pos<N, C::NB, F::P::last(), true>(lane, e, b, i);
-      const int col = colw + b;
-      const float kxv = a.kx[col < a.NX ? col : 0], kyv = a.ky[i];
-      const float2 ph = ps_mult<MODE>(w2, sr.x, sr.y, fmaf(kxv, kxv, kyv * kyv), a.dz, a.eps);
-      acc[e] = cadd(acc[e], cmul(v[e], ph));
-    }
+      for (int e = 0; e < C::V; ++e) {
+        int b, i;
+        edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
+        const int col = colw + b;
+        const float2 ph = tm.template at<decltype(real)::value>(a.kx[col < a.NX ? col : 0], a.ky[i]);
+        acc[e] = cadd(acc[e], cmul(v[e], ph));
+      }
+    };
+    if (tm.real) mult(std::true_type{});
+    else mult(std::false_type{});
     __syncthreads();
   }
   const float s2 = a.scale;
 #pragma unroll
   for (int e = 0; e < C::V; ++e) acc[e] = cscale(acc[e], s2);
-  Fi::run(acc, wl, a.twy, lane);
-#pragma unroll
-  for (int e = 0; e < C::V; ++e) {
-    int b, i;
-    edge_pos<N, C::NB, Fi::P::last(), true>(lane, e, b, i);
-    wl[i * C::CWP + b] = acc[e];
-  }
+  Fi::run(acc, priv, a.twy, lane);
   __syncthreads();
-  tile_store<N, C::CW, C::CWP>(tile, a.K + pb, a.NX, c0, 0);
+  tile_write<N, C, Fi::P::last()>(smem, acc, wave, lane);
+  __syncthreads();
+  tile_store<C>(smem, a.K + pb, a.NX, c0, 0);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -348,6 +446,8 @@ bool fft_size_supported(int n) {
     default: return false;
   }
 }
+
+int col_tile_width(int n) { return col_cw(n); }
 
 template <class Fn>
 static hipError_t dispatch_n(int n, Fn&& fn) {
@@ -406,8 +506,8 @@ hipError_t launch_cols(bool inv, const KArgs& a, hipStream_t st) {
   return dispatch_n(a.NY, [&](auto nc) -> hipError_t {
     constexpr int N = decltype(nc)::value;
     const dim3 g = col_grid(N, a.NX, a.np);
-    if (inv) hipLaunchKernelGGL((k_cols<N, true>), g, dim3(kThreads), 0, st, a);
-    else hipLaunchKernelGGL((k_cols<N, false>), g, dim3(kThreads), 0, st, a);
+    if (inv) hipLaunchKernelGGL((k_cols<N, true>), g, dim3(kColThreads), 0, st, a);
+    else hipLaunchKernelGGL((k_cols<N, false>), g, dim3(kColThreads), 0, st, a);
     return hipGetLastError();
   });
 }
@@ -416,8 +516,9 @@ hipError_t launch_cols_fanout(int mode, const KArgs& a, hipStream_t st) {
   return dispatch_n(a.NY, [&](auto nc) -> hipError_t {
     constexpr int N = decltype(nc)::value;
     const dim3 g = col_grid(N, a.NX, a.np);
-    if (mode == 0) hipLaunchKernelGGL((k_cols_fanout<N, 0>), g, dim3(kThreads), 0, st, a);
-    else hipLaunchKernelGGL((k_cols_fanout<N, 1>), g, dim3(kThreads), 0, st, a);
+    if (mode == 0) hipLaunchKernelGGL((k_cols_fanout<N, 0>), g, dim3(kColThreads), 0, st, a);
+    else if (mode == 1) hipLaunchKernelGGL((k_cols_fanout<N, 1>), g, dim3(kColThreads), 0, st, a);
+    else hipLaunchKernelGGL((k_cols_fanout<N, 2>), g, dim3(kColThreads), 0, st, a);
     return hipGetLastError();
   });
 }
@@ -426,8 +527,9 @@ hipError_t launch_cols_fanin(int mode, const KArgs& a, hipStream_t st) {
   return dispatch_n(a.NY, [&](auto nc) -> hipError_t {
     constexpr int N = decltype(nc)::value;
     const dim3 g = col_grid(N, a.NX, a.np);
-    if (mode == 0) hipLaunchKernelGGL((k_cols_fanin<N, 0>), g, dim3(kThreads), 0, st, a);
-    else hipLaunchKernelGGL((k_cols_fanin<N, 1>), g, dim3(kThreads), 0, st, a);
+    if (mode == 0) hipLaunchKernelGGL((k_cols_fanin<N, 0>), g, dim3(kColThreads), 0, st, a);
+    else if (mode == 1) hipLaunchKernelGGL((k_cols_fanin<N, 1>), g, dim3(kColThreads), 0, st, a);
+    else hipLaunchKernelGGL((k_cols_fanin<N, 2>), g, dim3(kColThreads), 0, st, a);
     return hipGetLastError();
   });
 }

@@ -16,7 +16,8 @@ import torch
 from . import _lib
 from ._lib import FWIX_NSPS, FWIX_PSPI, Handle, call, create, default_params, make_dims
 
-__all__ = ["FFT2d", "PhaseShift", "Selector", "Taper", "OneStep", "ptr", "stream",
+__all__ = ["FFT2d", "PhaseShift", "Selector", "Taper", "OneStep", "Injection", "Scatter",
+           "ImagingCondition", "OneWay", "Born", "ptr", "stream",
            "make_dims", "default_params", "FWIX_PSPI", "FWIX_NSPS"]
 
 
@@ -150,3 +151,222 @@ class OneStep(_Op):
 
     def set_depth(self, iz: int) -> None:
         call("fwix_onestep_set_depth", self._h.h, int(iz))
+
+
+def _f32(a, n=None, name="array") -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float32).ravel())
+    if n is not None and a.size != n:
+        raise ValueError(f"{name}: expected {n} values, got {a.size}")
+    return a
+
+
+def _i32(a, n=None, name="array") -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.int32).ravel())
+    if n is not None and a.size != n:
+        raise ValueError(f"{name}: expected {n} values, got {a.size}")
+    return a
+
+
+def _vp(a: np.ndarray) -> C.c_void_p:
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Injection:
+    """Trace injection / recording at one depth step (``propagator/src/Injection.h:10-55``).
+
+    ``forward(add, traces, wfld)``: wfld (+)= Inj_iz(traces); ``adjoint(add, traces,
+    wfld)``: traces (+)= Rec_iz^T wfld.  traces (ntrace, NW), wfld (NS, NW, NY, NX).
+    """
+
+    def __init__(self, dims, cx, cy, cz, ids):
+        self.dims = dims
+        self.ntrace = len(np.ravel(cx))
+        self._h = Handle(create("fwix_injection_create", C.byref(dims), self.ntrace))
+        self.set_coords(cx, cy, cz, ids)
+
+    def set_coords(self, cx, cy, cz, ids) -> None:
+        n = self.ntrace
+        self._coords = (_f32(cx, n, "cx"), _f32(cy, n, "cy"), _f32(cz, n, "cz"), _i32(ids, n, "ids"))
+        call("fwix_injection_set_coords", self._h.h, *map(_vp, self._coords))
+
+    def set_depth(self, iz: int) -> None:
+        call("fwix_injection_set_depth", self._h.h, int(iz))
+
+    def forward(self, add, traces, wfld):
+        _check(traces, (self.ntrace, self.dims.NW), name="traces")
+        _check(wfld, _wshape(self.dims), name="wavefield")
+        call("fwix_injection_forward", self._h.h, int(bool(add)), ptr(traces), ptr(wfld), stream())
+        return wfld
+
+    def adjoint(self, add, traces, wfld):
+        _check(traces, (self.ntrace, self.dims.NW), name="traces")
+        _check(wfld, _wshape(self.dims), name="wavefield")
+        call("fwix_injection_adjoint", self._h.h, int(bool(add)), ptr(traces), ptr(wfld), stream())
+        return traces
+
+
+class Scatter(_Op):
+    """Born scattering operator of one depth (``propagator/src/Scatter.h:12-104``).
+
+    ``set_slow`` borrows the unpadded background slowness (nz, NW, ny, nx).
+    """
+    _kind = "scatter"
+
+    def __init__(self, dims, params=None, **kw):
+        super().__init__(dims)
+        self.params = params if params is not None else default_params(**kw)
+        self._h = Handle(create("fwix_scatter_create", C.byref(dims), C.byref(self.params)))
+        self._slow = None
+
+    def set_slow(self, slow: torch.Tensor) -> None:
+        d = self.dims
+        _check(slow, (d.nz, d.NW, d.ny, d.nx), name="slowness")
+        self._slow = slow
+        call("fwix_scatter_set_slow", self._h.h, ptr(slow))
+
+    def set_depth(self, iz: int) -> None:
+        call("fwix_scatter_set_depth", self._h.h, int(iz))
+
+
+class ImagingCondition:
+    """Imaging condition of one depth (``propagator/src/ImagingCondition.h:11-43``).
+
+    model: extended image (NW, ny, nx); data: wavefield (NS, NW, NY, NX).
+    """
+
+    def __init__(self, dims):
+        self.dims = dims
+        self._h = Handle(create("fwix_ic_create", C.byref(dims)))
+        self._bg = None
+
+    def set_background(self, bg: torch.Tensor) -> None:
+        _check(bg, _wshape(self.dims), name="background")
+        self._bg = bg
+        call("fwix_ic_set_background", self._h.h, ptr(bg))
+
+    def forward(self, add, img, wfld):
+        d = self.dims
+        _check(img, (d.NW, d.ny, d.nx), name="image")
+        _check(wfld, _wshape(d), name="wavefield")
+        call("fwix_ic_forward", self._h.h, int(bool(add)), ptr(img), ptr(wfld), stream())
+        return wfld
+
+    def adjoint(self, add, img, wfld):
+        d = self.dims
+        _check(img, (d.NW, d.ny, d.nx), name="image")
+        _check(wfld, _wshape(d), name="wavefield")
+        call("fwix_ic_adjoint", self._h.h, int(bool(add)), ptr(img), ptr(wfld), stream())
+        return img
+
+
+class OneWay:
+    """Downward (direction 0) / Upward (direction 1) recursion (``propagator/src/OneWay.h:14-245``)."""
+
+    def __init__(self, direction: int, method: int, dims, params=None, **kw):
+        self.dims = dims
+        self.params = params if params is not None else default_params(**kw)
+        self._h = Handle(create("fwix_oneway_create", int(direction), int(method), C.byref(dims),
+                                C.byref(self.params)))
+        self._refs = None
+
+    def set_refs(self, sref: torch.Tensor, labels: torch.Tensor) -> None:
+        d, p = self.dims, self.params
+        _check(sref, (d.nz, p.nref, d.NW), name="sref")
+        _check(labels, (d.nz, d.NW, d.NY, d.NX), torch.uint8, name="labels")
+        self._refs = (sref, labels)
+        call("fwix_oneway_set_refs", self._h.h, ptr(sref), ptr(labels))
+
+    def forward(self, add, model, data, checkpoints: torch.Tensor | None = None):
+        s = _wshape(self.dims)
+        _check(model, s, name="model")
+        _check(data, s, name="data")
+        if checkpoints is not None:
+            _check(checkpoints, (self.dims.nz, *s), name="checkpoints")
+        call("fwix_oneway_forward", self._h.h, int(bool(add)), ptr(model), ptr(data), ptr(checkpoints),
+             stream())
+        return data
+
+    def adjoint(self, add, model, data):
+        s = _wshape(self.dims)
+        _check(model, s, name="model")
+        _check(data, s, name="data")
+        call("fwix_oneway_adjoint", self._h.h, int(bool(add)), ptr(model), ptr(data), stream())
+        return model
+
+
+class Born:
+    """Background modeling + linearized Born operator (down-scattering channel) and migration.
+
+    ``propagator/src/Propagator.cpp:155-180`` and ``propagator/src/ExtendedBorn.cpp:152-208``.
+    Model ds (nz, NW, ny, nx); data (nrec, NW); image (nz, ny, nx).
+    ``ckpt_every`` > 1 keeps every ckpt_every-th background wavefield and
+    recomputes the others in the adjoint.
+    """
+
+    def __init__(self, dims, params=None, ckpt_every: int = 1, **kw):
+        self.dims = dims
+        self.params = params if params is not None else default_params(**kw)
+        self._h = Handle(create("fwix_born_create", C.byref(dims), C.byref(self.params), int(ckpt_every)))
+        self._keep = {}
+        self.nsrc = self.nrec = 0
+
+    def set_refs(self, sref, labels):
+        d, p = self.dims, self.params
+        _check(sref, (d.nz, p.nref, d.NW), name="sref")
+        _check(labels, (d.nz, d.NW, d.NY, d.NX), torch.uint8, name="labels")
+        self._keep["refs"] = (sref, labels)
+        call("fwix_born_set_refs", self._h.h, ptr(sref), ptr(labels))
+
+    def set_slow(self, slow):
+        d = self.dims
+        _check(slow, (d.nz, d.NW, d.ny, d.nx), name="slowness")
+        self._keep["slow"] = slow
+        call("fwix_born_set_slow", self._h.h, ptr(slow))
+
+    def set_sources(self, x, y, z, ids):
+        n = len(np.ravel(x))
+        a = (_f32(x, n, "x"), _f32(y, n, "y"), _f32(z, n, "z"), _i32(ids, n, "ids"))
+        call("fwix_born_set_sources", self._h.h, n, *map(_vp, a))
+        self.nsrc = n
+
+    def set_receivers(self, x, y, z, ids):
+        n = len(np.ravel(x))
+        a = (_f32(x, n, "x"), _f32(y, n, "y"), _f32(z, n, "z"), _i32(ids, n, "ids"))
+        call("fwix_born_set_receivers", self._h.h, n, *map(_vp, a))
+        self.nrec = n
+
+    def set_wavelet(self, wavelet):
+        _check(wavelet, (self.nsrc, self.dims.NW), name="wavelet")
+        self._keep["wavelet"] = wavelet
+        call("fwix_born_set_wavelet", self._h.h, ptr(wavelet))
+
+    def model(self, data: torch.Tensor | None = None, add: bool = False):
+        if data is not None:
+            _check(data, (self.nrec, self.dims.NW), name="data")
+        call("fwix_born_model", self._h.h, int(bool(add)), ptr(data), stream())
+        return data
+
+    def forward(self, add, dslow, data):
+        d = self.dims
+        _check(dslow, (d.nz, d.NW, d.ny, d.nx), name="model")
+        _check(data, (self.nrec, d.NW), name="data")
+        call("fwix_born_forward", self._h.h, int(bool(add)), ptr(dslow), ptr(data), stream())
+        return data
+
+    def adjoint(self, add, dslow, data):
+        d = self.dims
+        _check(dslow, (d.nz, d.NW, d.ny, d.nx), name="model")
+        _check(data, (self.nrec, d.NW), name="data")
+        call("fwix_born_adjoint", self._h.h, int(bool(add)), ptr(dslow), ptr(data), stream())
+        return dslow
+
+    def migrate(self, add, image, data):
+        d = self.dims
+        _check(image, (d.nz, d.ny, d.nx), name="image")
+        _check(data, (self.nrec, d.NW), name="data")
+        call("fwix_born_migrate", self._h.h, int(bool(add)), ptr(image), ptr(data), stream())
+        return image
+
+    @property
+    def steps(self) -> int:
+        return int(_lib.lib.fwix_born_steps(self._h.h))

@@ -1,0 +1,242 @@
+"""GPU parity of injection, scattering, imaging condition, depth recursions and
+the Born operator against the FP64 oracle.
+
+Pins besides the oracle: dot-product tests at 1e-5 for every adjoint pair
+(propagator/tests/born_unit_test.cpp:19-307, prop_unit_test.cpp:199-340),
+checkpoint/recompute == store-all, and run-to-run bitwise determinism.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from fwix_testutil import TOL_DOT, TOL_WFLD, crand, dims_of, dot_err, grid, rel, smooth_slowness
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(x, dev):
+    import torch
+    if x.dtype == np.uint8:
+        return torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+    return torch.from_numpy(np.ascontiguousarray(x.astype(np.complex64))).to(dev)
+
+
+def _z(shape, dev):
+    import torch
+    return torch.zeros(shape, dtype=torch.complex64, device=dev)
+
+
+def _geometry(g, rng, nrec=40, nsrc=None):
+    nsrc = nsrc or g.NS
+    sx = rng.uniform(g.ox + 2 * g.dx, g.ox + (g.nx - 3) * g.dx, nsrc)
+    sy = rng.uniform(g.oy + 2 * g.dy, g.oy + (g.ny - 3) * g.dy, nsrc)
+    sz = np.full(nsrc, g.oz + 0.3 * g.dz)
+    sid = np.arange(nsrc) % g.NS
+    rx = rng.uniform(g.ox, g.ox + (g.nx - 1) * g.dx, nrec)
+    ry = rng.uniform(g.oy, g.oy + (g.ny - 1) * g.dy, nrec)
+    rz = rng.uniform(g.oz, g.oz + 2.5 * g.dz, nrec)
+    # a few receivers on shared nodes, one outside the grid
+    rx[:4] = g.ox + 3 * g.dx
+    ry[:4] = g.oy + 5 * g.dy
+    rx[-1] = g.ox - 5 * g.dx
+    rid = np.arange(nrec) % g.NS
+    f32 = np.float32
+    return (sx.astype(f32), sy.astype(f32), sz.astype(f32), sid), (rx.astype(f32), ry.astype(f32),
+                                                                  rz.astype(f32), rid)
+
+
+def test_injection_matches_oracle(dev):
+    from pyfwix_amd.ops import Injection
+    rng = np.random.default_rng(31)
+    g = grid(48, 32, 5, 2, nx=40, ny=24, nz=4)
+    _, (rx, ry, rz, rid) = _geometry(g, rng)
+    op = Injection(dims_of(g), rx, ry, rz, rid)
+    tr = crand(rng, (len(rx), g.NW))
+    wf = crand(rng, (g.NS, g.NW, g.NY, g.NX))
+    shape = (g.NS, g.NW, g.NY, g.NX)
+    for iz in range(4):
+        op.set_depth(iz)
+        out = _t(wf, dev)
+        op.forward(True, _t(tr, dev), out)
+        ref = O.injection_forward(tr, rx, ry, rz, rid, g, iz, wfld=wf, add=True, shape=shape)
+        assert rel(out.cpu().numpy(), ref) < 1e-6
+        t2 = _z((len(rx), g.NW), dev)
+        op.adjoint(False, t2, _t(wf, dev))
+        ref = O.injection_adjoint(wf, rx, ry, rz, rid, g, iz)
+        assert rel(t2.cpu().numpy(), ref) < 1e-6
+        # dot test
+        Ax = _z(shape, dev)
+        op.forward(False, _t(tr, dev), Ax)
+        assert dot_err(tr, Ax.cpu().numpy(), wf, t2.cpu().numpy()) < TOL_DOT
+
+
+@pytest.mark.parametrize("taylor", [0, 1, 2])
+def test_scatter_matches_oracle(dev, taylor):
+    from pyfwix_amd.ops import Scatter
+    rng = np.random.default_rng(7 + taylor)
+    g = grid(64, 48, 4, 2, nx=48, ny=40, nz=3)
+    slow = smooth_slowness(rng, g, imag=0.01)
+    op = Scatter(dims_of(g), taylor=taylor, eps_scat=0.04)
+    op.set_slow(_t(slow, dev))
+    op.set_depth(2)
+    shape = (g.NS, g.NW, g.NY, g.NX)
+    x, y = crand(rng, shape), crand(rng, shape)
+    out = _z(shape, dev)
+    op.forward(False, _t(x, dev), out)
+    ref = O.scatter_forward(x, slow[2].astype(np.complex64).astype(np.complex128), g, g.dz, 0.04, taylor)
+    assert rel(out.cpu().numpy(), ref) < TOL_WFLD
+    m = _t(x, dev)
+    op.adjoint(True, m, _t(y, dev))
+    ref = x + O.scatter_adjoint(y, slow[2].astype(np.complex64).astype(np.complex128), g, g.dz, 0.04, taylor)
+    assert rel(m.cpu().numpy(), ref) < TOL_WFLD
+    Ax, Aty = _z(shape, dev), _z(shape, dev)
+    op.forward(False, _t(x, dev), Ax)
+    op.adjoint(False, Aty, _t(y, dev))
+    assert dot_err(x, Ax.cpu().numpy(), y, Aty.cpu().numpy()) < TOL_DOT
+
+
+def test_imaging_condition_matches_oracle(dev):
+    from pyfwix_amd.ops import ImagingCondition
+    rng = np.random.default_rng(17)
+    g = grid(32, 24, 3, 3, nx=20, ny=16)
+    shape = (g.NS, g.NW, g.NY, g.NX)
+    U = crand(rng, shape)
+    img = crand(rng, (g.NW, g.ny, g.nx))
+    D = crand(rng, shape)
+    op = ImagingCondition(dims_of(g))
+    Ud = _t(U, dev)
+    op.set_background(Ud)
+    out = _t(D, dev)
+    op.forward(True, _t(img, dev), out)
+    assert rel(out.cpu().numpy(), O.ic_forward(img, U, d=D, add=True)) < 1e-6
+    out0 = _t(D, dev)
+    op.forward(False, _t(img, dev), out0)
+    assert rel(out0.cpu().numpy(), O.ic_forward(img, U)) < 1e-6
+    im = _z((g.NW, g.ny, g.nx), dev)
+    op.adjoint(False, im, _t(D, dev))
+    assert rel(im.cpu().numpy(), O.ic_adjoint(D, U, g.nx, g.ny)) < 1e-6
+    assert dot_err(img, out0.cpu().numpy(), D, im.cpu().numpy()) < TOL_DOT
+
+
+def _refs(g, rng, nref, imag=0.0):
+    from pyfwix_amd.refsampler import select_refs
+    slow = smooth_slowness(rng, g, imag=imag)
+    sref, labels = select_refs(slow, nref, g.NX - g.nx, g.NY - g.ny)
+    return slow, sref, labels
+
+
+@pytest.mark.parametrize("direction", [0, 1])
+@pytest.mark.parametrize("method", ["pspi", "nsps"])
+def test_oneway_matches_oracle(dev, direction, method):
+    from pyfwix_amd.ops import FWIX_NSPS, FWIX_PSPI, OneWay
+    rng = np.random.default_rng(41 + direction)
+    g = grid(48, 64, 3, 2, nx=36, ny=48, nz=5)
+    slow, sref, labels = _refs(g, rng, 3)
+    tap = 6 if method == "pspi" else 0
+    op = OneWay(direction, FWIX_PSPI if method == "pspi" else FWIX_NSPS, dims_of(g), nref=3,
+                taperx=tap, tapery=tap)
+    op.set_refs(_t(sref, dev), _t(labels, dev))
+    shape = (g.NS, g.NW, g.NY, g.NX)
+    x, y = crand(rng, shape), crand(rng, shape)
+    s64, lab = sref.astype(np.complex128), labels.astype(np.int64)
+    out = _z(shape, dev)
+    ck = _z((g.nz, *shape), dev) if direction == 0 else None
+    op.forward(False, _t(x, dev), out, ck)
+    f = O.downward_forward if direction == 0 else O.upward_forward
+    b = O.downward_adjoint if direction == 0 else O.upward_adjoint
+    cks = [] if direction == 0 else None
+    if direction == 0:
+        ref = f(x, s64, lab, g, g.dz, 0.0, tap, tap, method, checkpoints=cks)
+    else:
+        ref = f(x, s64, lab, g, g.dz, 0.0, tap, tap, method)
+    assert rel(out.cpu().numpy(), ref) < TOL_WFLD
+    if direction == 0:
+        assert rel(ck.cpu().numpy(), np.stack(cks)) < TOL_WFLD
+    Aty = _z(shape, dev)
+    op.adjoint(False, Aty, _t(y, dev))
+    assert rel(Aty.cpu().numpy(), b(y, s64, lab, g, g.dz, 0.0, tap, tap, method)) < TOL_WFLD
+    assert dot_err(x, out.cpu().numpy(), y, Aty.cpu().numpy()) < TOL_DOT
+
+
+def _born_setup(dev, ckpt=1, seed=5, NS=2, tap=4, nz=6):
+    from pyfwix_amd.ops import Born
+    rng = np.random.default_rng(seed)
+    g = grid(48, 48, 4, NS, nx=32, ny=36, nz=nz, ow=6.0, dw=3.0)
+    slow, sref, labels = _refs(g, rng, 3)
+    src, rec = _geometry(g, rng, nrec=48)
+    born = Born(dims_of(g), nref=3, taperx=tap, tapery=tap, taylor=1, eps_scat=0.04, ckpt_every=ckpt)
+    born.set_refs(_t(sref, dev), _t(labels, dev))
+    born.set_slow(_t(slow, dev))
+    born.set_sources(*src)
+    born.set_receivers(*rec)
+    wav = crand(rng, (len(src[0]), g.NW))
+    born.set_wavelet(_t(wav, dev))
+    return g, born, slow, sref, labels, src, rec, wav, rng
+
+
+def test_born_background_and_forward_match_oracle(dev):
+    g, born, slow, sref, labels, src, rec, wav, rng = _born_setup(dev)
+    s64, lab, sl = sref.astype(np.complex128), labels.astype(np.int64), slow.astype(np.complex64).astype(np.complex128)
+    d0 = _z((len(rec[0]), g.NW), dev)
+    born.model(d0)
+    ref_d0, bg = O.model_background(wav, src, rec, s64, lab, g, g.dz, 0.0, 4, 4)
+    assert rel(d0.cpu().numpy(), ref_d0) < TOL_WFLD
+    ds = crand(rng, (g.nz, g.NW, g.ny, g.nx)) * 1e-7
+    d = _z((len(rec[0]), g.NW), dev)
+    born.forward(False, _t(ds, dev), d)
+    ref = O.born_forward(ds, bg, sl, rec, s64, lab, g, g.dz, 0.0, 4, 4, 0.04, 1)
+    assert rel(d.cpu().numpy(), ref) < TOL_WFLD
+    dres = crand(rng, (len(rec[0]), g.NW))
+    m = _z((g.nz, g.NW, g.ny, g.nx), dev)
+    born.adjoint(False, m, _t(dres, dev))
+    ref_m = O.born_adjoint(dres, bg, sl, rec, s64, lab, g, g.dz, 0.0, 4, 4, 0.04, 1)
+    assert rel(m.cpu().numpy(), ref_m) < TOL_WFLD
+    assert dot_err(ds, d.cpu().numpy(), dres, m.cpu().numpy()) < TOL_DOT
+    img = _z((g.nz, g.ny, g.nx), dev)
+    born.migrate(False, img, _t(dres, dev))
+    assert rel(img.cpu().numpy(), O.migrate(dres, bg, rec, s64, lab, g, g.dz, 0.0, 4, 4)) < TOL_WFLD
+
+
+def test_born_dot_no_taper_single_shot(dev):
+    g, born, *_rest = _born_setup(dev, NS=1, tap=0, seed=9)
+    rng = _rest[-1]
+    nrec = born.nrec
+    born.model()
+    ds = crand(rng, (g.nz, g.NW, g.ny, g.nx))
+    dd = crand(rng, (nrec, g.NW))
+    d = _z((nrec, g.NW), dev)
+    m = _z((g.nz, g.NW, g.ny, g.nx), dev)
+    born.forward(False, _t(ds, dev), d)
+    born.adjoint(False, m, _t(dd, dev))
+    assert dot_err(ds, d.cpu().numpy(), dd, m.cpu().numpy()) < TOL_DOT
+    # add = true
+    born.forward(True, _t(ds, dev), d)
+    born.adjoint(True, m, _t(dd, dev))
+    assert dot_err(ds, d.cpu().numpy(), dd, m.cpu().numpy()) < TOL_DOT
+
+
+def test_born_checkpointing_is_exact_and_deterministic(dev):
+    import torch
+    outs = []
+    for ck in (1, 3, 4):
+        g, born, *_rest = _born_setup(dev, ckpt=ck, seed=13, nz=7)
+        rng = np.random.default_rng(99)
+        born.model()
+        dd = crand(rng, (born.nrec, g.NW))
+        ds = crand(rng, (g.nz, g.NW, g.ny, g.nx))
+        m = _z((g.nz, g.NW, g.ny, g.nx), dev)
+        born.adjoint(False, m, _t(dd, dev))
+        d = _z((born.nrec, g.NW), dev)
+        born.forward(False, _t(ds, dev), d)
+        outs.append((m, d))
+    for m, d in outs[1:]:
+        assert torch.equal(m, outs[0][0])
+        assert torch.equal(d, outs[0][1])
+    # run to run
+    g, born, *_rest = _born_setup(dev, ckpt=1, seed=13, nz=7)
+    rng = np.random.default_rng(99)
+    born.model()
+    dd = crand(rng, (born.nrec, g.NW))
+    m = _z((g.nz, g.NW, g.ny, g.nx), dev)
+    born.adjoint(False, m, _t(dd, dev))
+    assert torch.equal(m, outs[0][0])
