@@ -62,7 +62,11 @@ typedef struct fwix_params {
   int planes_per_chunk;     /* planes per fused launch; 0 = automatic */
 } fwix_params;
 
-enum { FWIX_PSPI = 0, FWIX_NSPS = 1 };
+/* step methods; FWIX_PSPI_PLAIN runs the PSPI step at the reference's
+ * operator granularity (taper, 2-D FFT, per reference: phase shift, inverse
+ * 2-D FFT, select; PSPI.cpp:66-103) with one kernel per operator and one
+ * shot per launch -- the comparator of the fused path. */
+enum { FWIX_PSPI = 0, FWIX_NSPS = 1, FWIX_PSPI_PLAIN = 2 };
 
 /* ---------------------------------------------------------------- library */
 int fwix_version(void);
@@ -119,7 +123,7 @@ int fwix_taper_adjoint(fwix_op op, int add, fwix_c64* model, const fwix_c64* dat
  * PSPI.cpp:66-103).  The PSPI adjoint always includes the taper (the
  * reference's non-in-place adjoint omits it, PSPI.cpp:26-44) and the NSPS
  * adjoint is the true adjoint (NSPS.cpp:16 passes swapped arguments). */
-int fwix_onestep_create(int method, const fwix_dims* d, const fwix_params* p, fwix_op* out);
+int fwix_onestep_create(int method, const fwix_dims* d, const fwix_params* p, fwix_op* out);  /* any FWIX_* method */
 int fwix_onestep_set_refs(fwix_op op, const fwix_c64* sref_dev, const uint8_t* labels_dev);
 int fwix_onestep_set_depth(fwix_op op, int iz);
 int fwix_onestep_forward(fwix_op op, int add, const fwix_c64* model, fwix_c64* data, fwix_stream s);
@@ -199,6 +203,18 @@ int fwix_born_adjoint(fwix_op op, int add, fwix_c64* dslow, const fwix_c64* data
 int fwix_born_migrate(fwix_op op, int add, fwix_c64* image, const fwix_c64* data, fwix_stream s);
 /* depth steps (PSPI applications of one wavefield) executed so far, -1 on a bad handle */
 long long fwix_born_steps(fwix_op op);
+
+/* ---------------------------------------------------------------- live pass timing
+ * No counterpart in the reference (which times with std::chrono and pasted
+ * Nsight numbers).  When enabled, every pass of the hot path records a pair
+ * of HIP events on its launch stream; fwix_timing_collect synchronizes them
+ * and accumulates per pass: milliseconds, launches and compulsory HBM bytes
+ * (what the pass must read and write).  fwix_timing_report writes
+ * {"pass": {"ms": .., "launches": .., "bytes": ..}, ...} as JSON. */
+int fwix_timing_enable(int on);
+int fwix_timing_collect(void);
+int fwix_timing_reset(void);
+int fwix_timing_report(char* buf, int len);
 
 #ifdef __cplusplus
 }

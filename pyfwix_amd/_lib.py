@@ -21,6 +21,7 @@ lib = C.CDLL(str(_LIB_PATH))
 
 FWIX_PSPI = 0
 FWIX_NSPS = 1
+FWIX_PSPI_PLAIN = 2
 
 
 class fwix_dims(C.Structure):
@@ -96,6 +97,10 @@ _SIGS = {
     "fwix_born_forward": [_P, _I, _P, _P, _P],
     "fwix_born_adjoint": [_P, _I, _P, _P, _P],
     "fwix_born_migrate": [_P, _I, _P, _P, _P],
+    "fwix_timing_enable": [_I],
+    "fwix_timing_collect": [],
+    "fwix_timing_reset": [],
+    "fwix_timing_report": [C.c_char_p, _I],
 }
 
 EXPORTS = ["fwix_version", "fwix_last_error", "fwix_params_default", "fwix_fft_size_supported",
@@ -161,3 +166,25 @@ def make_dims(NX, NY, NW, NS, nx=None, ny=None, ox=0.0, dx=1.0, oy=0.0, dy=1.0,
 
 def fft_size_supported(n: int) -> bool:
     return bool(lib.fwix_fft_size_supported(int(n)))
+
+
+def timing_enable(reserve_pairs: int = 1 << 16) -> None:
+    """Start recording per-pass HIP events (``reserve_pairs`` event pairs pre-created)."""
+    call("fwix_timing_enable", int(reserve_pairs))
+
+
+def timing_disable() -> None:
+    call("fwix_timing_enable", 0)
+
+
+def timing_collect() -> dict:
+    """Synchronize recorded events; return {pass: {"ms", "launches", "bytes"}} accumulated so far."""
+    import json
+    call("fwix_timing_collect")
+    buf = C.create_string_buffer(1 << 16)
+    call("fwix_timing_report", buf, len(buf))
+    return json.loads(buf.value.decode())
+
+
+def timing_reset() -> None:
+    call("fwix_timing_reset")

@@ -276,7 +276,10 @@ struct BornOp : fwix_op_s {
     for (int z = 0; z < d.nz; ++z) {
       src.forward(z, wavelet, u, st);
       if (data) rec.adjoint(z, data, u, st);
-      if (z % ckpt == 0) hip_check(launch_axpy(u, (float2*)store.p + (size_t)(z / ckpt) * n, n, 1.f, 0, st), "store");
+      if (z % ckpt == 0) {
+        PassTimer t("checkpoint_store", 16.0 * n, st);
+        hip_check(launch_axpy(u, (float2*)store.p + (size_t)(z / ckpt) * n, n, 1.f, 0, st), "store");
+      }
       if (z + 1 < d.nz) {
         eng.apply(false, z, 0, u, u, st);
         ++steps;
@@ -313,21 +316,23 @@ struct BornOp : fwix_op_s {
           KArgs a = born_args(z, p0, np);
           a.dslow = dslow + (size_t)z * d.NW * d.ny * d.nx;
           a.bg = bg + (size_t)p0 * plane;
+          const double pb = 8.0 * np * plane, mb = 8.0 * np * d.nx * d.ny;
+          const int nt = par.taylor + 1, nr = par.nref;
           a.scale = sx;
-          hip_check(launch_rows_ic_fwd(a, st), "born ic");
+          { PassTimer t("born_rows_ic", 2 * pb + mb, st); hip_check(launch_rows_ic_fwd(a, st), "born ic"); }
           a.wstride = (long long)np * plane;
           a.scale = sy2;
-          hip_check(launch_cols_fanout(2, a, st), "born scatter columns");
+          { PassTimer t("born_cols_kfac", (1 + nt) * pb, st); hip_check(launch_cols_fanout(2, a, st), "born scatter columns"); }
           a.addin = du + (size_t)p0 * plane;
           a.scale = sx;
-          hip_check(launch_rows_scatter(true, a, st), "born scatter rows");
-          a.nterm = par.nref;
+          { PassTimer t("born_rows_scatter", (nt + 2) * pb + mb, st); hip_check(launch_rows_scatter(true, a, st), "born scatter rows"); }
+          a.nterm = nr;
           a.scale = sy2;
-          hip_check(launch_cols_fanout(0, a, st), "born pspi columns");
+          { PassTimer t("cols_fanout", (1 + nr) * pb, st); hip_check(launch_cols_fanout(0, a, st), "born pspi columns"); }
           a.out = du + (size_t)p0 * plane;
           a.add = 0;
           a.scale = sx;
-          hip_check(launch_rows_select(a, st), "born pspi rows");
+          { PassTimer t("rows_select", (1 + nr) * pb + pb / 8, st); hip_check(launch_rows_select(a, st), "born pspi rows"); }
         }
       ++steps;
     }
@@ -361,22 +366,24 @@ struct BornOp : fwix_op_s {
               float2* lc = l + (size_t)p0 * plane;
               KArgs a = born_args(z, p0, np);
               a.wstride = (long long)np * plane;
+              const double pb = 8.0 * np * plane, mb = 8.0 * np * d.nx * d.ny;
+              const int nt = par.taylor + 1, nr = par.nref;
               // mu = P_z^H lam (PSPI adjoint, in place)
               a.in = lc; a.scale = sx;
-              hip_check(launch_rows_fanout(a, st), "born adjoint pspi rows");
-              a.nterm = par.nref; a.scale = sy2;
-              hip_check(launch_cols_fanin(1, a, st), "born adjoint pspi columns");
+              { PassTimer t("rows_fanout", (1 + nr) * pb + pb / 8, st); hip_check(launch_rows_fanout(a, st), "born adjoint pspi rows"); }
+              a.nterm = nr; a.scale = sy2;
+              { PassTimer t("cols_fanin", (1 + nr) * pb, st); hip_check(launch_cols_fanin(1, a, st), "born adjoint pspi columns"); }
               a.in = a.K; a.out = lc; a.add = 0; a.scale = sx;
-              hip_check(launch_rows(par.taperx > 0 || par.tapery > 0 ? 3 : 1, a, st), "born adjoint pspi inverse");
+              { PassTimer t("rows_inv", 2 * pb, st); hip_check(launch_rows(par.taperx > 0 || par.tapery > 0 ? 3 : 1, a, st), "born adjoint pspi inverse"); }
               // ds_z += conj(U_z) S_z^H mu
-              a.in = lc; a.nterm = par.taylor + 1; a.scale = sx;
-              hip_check(launch_rows_scatter_adj(a, st), "born adjoint scatter rows");
+              a.in = lc; a.nterm = nt; a.scale = sx;
+              { PassTimer t("born_rows_scatter_adj", (1 + nt) * pb + mb, st); hip_check(launch_rows_scatter_adj(a, st), "born adjoint scatter rows"); }
               a.scale = sy2;
-              hip_check(launch_cols_fanin(2, a, st), "born adjoint scatter columns");
+              { PassTimer t("born_cols_kfac_adj", (1 + nt) * pb, st); hip_check(launch_cols_fanin(2, a, st), "born adjoint scatter columns"); }
               a.bg = bg + (size_t)p0 * plane;
               a.aux = dslow + (size_t)z * mz;
               a.add = 1; a.scale = sx;
-              hip_check(launch_rows_ic_adj(a, st), "born adjoint imaging");
+              { PassTimer t("born_rows_ic_adj", 2 * pb + 2 * mb, st); hip_check(launch_rows_ic_adj(a, st), "born adjoint imaging"); }
             }
         }
         ++steps;
@@ -527,9 +534,7 @@ int fwix_oneway_create(int direction, int method, const fwix_dims* d, const fwix
 int fwix_oneway_set_refs(fwix_op op, const fwix_c64* sref, const uint8_t* labels) {
   return guard([&] {
     auto* p = as<OneWayOp>(op, KIND_ONEWAY);
-    if (!sref || !labels) throw Error("null reference slownesses or labels");
-    p->eng.sref = (const float2*)sref;
-    p->eng.labels = labels;
+    p->eng.set_refs((const float2*)sref, labels);
   });
 }
 int fwix_oneway_forward(fwix_op op, int add, const fwix_c64* m, fwix_c64* d, fwix_c64* ckpt, fwix_stream s) {
@@ -573,9 +578,7 @@ int fwix_born_create(const fwix_dims* d, const fwix_params* par, int ckpt_every,
 int fwix_born_set_refs(fwix_op op, const fwix_c64* sref, const uint8_t* labels) {
   return guard([&] {
     auto* p = as<BornOp>(op, KIND_BORN);
-    if (!sref || !labels) throw Error("null reference slownesses or labels");
-    p->eng.sref = (const float2*)sref;
-    p->eng.labels = labels;
+    p->eng.set_refs((const float2*)sref, labels);
     p->have_bg = false;
   });
 }

@@ -5,7 +5,10 @@
 #include "capi_internal.h"
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -19,6 +22,50 @@ void set_error(const std::string& s) { g_err = s; }
 
 void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) throw Error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------------ live pass timing
+namespace {
+struct EvRec {
+  const char* name;
+  double bytes;
+  hipEvent_t a, b;
+};
+struct PassStat {
+  double ms = 0, bytes = 0;
+  long long n = 0;
+};
+std::mutex g_tmu;
+bool g_timing = false;
+std::vector<EvRec> g_pending;
+std::vector<hipEvent_t> g_pool;
+std::map<std::string, PassStat> g_stat;
+
+hipEvent_t take_event() {
+  if (g_pool.empty()) {
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+  }
+  hipEvent_t e = g_pool.back();
+  g_pool.pop_back();
+  return e;
+}
+}  // namespace
+
+PassTimer::PassTimer(const char* name, double bytes, hipStream_t s) {
+  if (!g_timing) return;
+  std::lock_guard<std::mutex> g(g_tmu);
+  EvRec r{name, bytes, take_event(), take_event()};
+  hip_check(hipEventRecord(r.a, s), "hipEventRecord");
+  slot = (int)g_pending.size();
+  st = s;
+  g_pending.push_back(r);
+}
+PassTimer::~PassTimer() {
+  if (slot < 0) return;
+  std::lock_guard<std::mutex> g(g_tmu);
+  (void)hipEventRecord(g_pending[slot].b, st);
 }
 
 void DevBuf::alloc(size_t bytes) {
@@ -158,14 +205,38 @@ int auto_chunk(const fwix_dims& d, int nterm, int requested) {
 
 // ------------------------------------------------------------------ one step engine
 void StepEngine::init(int method_, const fwix_dims& d, const fwix_params& p, int wterms) {
-  method = method_;
+  plain = method_ == FWIX_PSPI_PLAIN;
+  method = plain ? FWIX_PSPI : method_;
   params = p;
   tables.init(d, method == FWIX_PSPI ? p.taperx : 0, method == FWIX_PSPI ? p.tapery : 0);
+  const size_t plane = (size_t)d.NX * d.NY;
+  if (plain) {  // two whole-shot scratch wavefields
+    chunk = d.NW;
+    K.alloc(plane * d.NW * sizeof(float2));
+    T.alloc(plane * d.NW * sizeof(float2));
+    return;
+  }
   if (wterms < p.nref) wterms = p.nref;
   chunk = auto_chunk(d, wterms, p.planes_per_chunk);
-  const size_t plane = (size_t)d.NX * d.NY;
   K.alloc(plane * chunk * sizeof(float2));
   W.alloc(plane * chunk * wterms * sizeof(float2));
+}
+
+void StepEngine::set_refs(const float2* sref_, const uint8_t* labels_) {
+  if (!sref_ || !labels_) throw Error("null reference slownesses or labels");
+  sref = sref_;
+  labels = labels_;
+  masks = false;
+  if (params.nref > 32) return;  // presence skipping covers references 0..31
+  const fwix_dims& d = tables.dims;
+  const long long nplanes = (long long)d.nz * d.NW;
+  rowmask.alloc((size_t)nplanes * d.NY * sizeof(uint32_t));
+  planemask.alloc((size_t)nplanes * sizeof(uint32_t));
+  hip_check(hipDeviceSynchronize(), "synchronize before presence masks");
+  hip_check(launch_presence(labels, (uint32_t*)rowmask.p, (uint32_t*)planemask.p, d.NX, d.NY, nplanes, nullptr),
+            "presence masks");
+  hip_check(hipDeviceSynchronize(), "presence masks");
+  masks = true;
 }
 
 void StepEngine::check_ready(int iz) const {
@@ -180,6 +251,10 @@ KArgs StepEngine::base(int iz, int c0, int np) const {
   a.W = (float2*)W.p;
   a.labels = labels + (size_t)iz * d.NW * d.NY * d.NX;
   a.sref = sref + (size_t)iz * params.nref * d.NW;
+  if (masks) {
+    a.rowmask = (const uint32_t*)rowmask.p + (size_t)iz * d.NW * d.NY;
+    a.planemask = (const uint32_t*)planemask.p + (size_t)iz * d.NW;
+  }
   a.nref = params.nref;
   a.nterm = params.nref;
   a.p0 = c0;
@@ -193,6 +268,7 @@ KArgs StepEngine::base(int iz, int c0, int np) const {
 // out (+)= P_z in     (forward) or    out (+)= P_z^H in    (adjoint)
 void StepEngine::apply(bool adjoint, int iz, int add, const float2* in, float2* out, hipStream_t st) {
   check_ready(iz);
+  if (plain) return apply_plain(adjoint, iz, add, in, out, st);
   const fwix_dims& d = tables.dims;
   const size_t plane = (size_t)d.NX * d.NY;
   const int planes = d.NW * d.NS;
@@ -200,26 +276,85 @@ void StepEngine::apply(bool adjoint, int iz, int add, const float2* in, float2* 
   const float sy2 = (float)(1.0 / (double)d.NY);
   const bool pspi = method == FWIX_PSPI;
   const bool taper = pspi && (params.taperx > 0 || params.tapery > 0);
+  const int nr = params.nref;
   for (int c0 = 0; c0 < planes; c0 += chunk) {
     const int np = (planes - c0) < chunk ? (planes - c0) : chunk;
+    const double pb = 8.0 * np * plane;  // bytes of one chunk-sized complex64 array
     KArgs a = base(iz, c0, np);
     const float2* cin = in + (size_t)c0 * plane;
     float2* cout = out + (size_t)c0 * plane;
     // "shared spectrum" shape: PSPI forward and NSPS adjoint
     if ((pspi && !adjoint) || (!pspi && adjoint)) {
       a.in = cin; a.out = a.K; a.add = 0; a.scale = sx;
-      hip_check(launch_rows(taper ? 2 : 0, a, st), "rows pass");
+      { PassTimer t("rows_fwd", 2 * pb, st); hip_check(launch_rows(taper ? 2 : 0, a, st), "rows pass"); }
       a.scale = sy2;
-      hip_check(launch_cols_fanout(adjoint ? 1 : 0, a, st), "column fan-out pass");
+      { PassTimer t("cols_fanout", (1 + nr) * pb, st); hip_check(launch_cols_fanout(adjoint ? 1 : 0, a, st), "column fan-out pass"); }
       a.out = cout; a.add = add; a.scale = sx;
-      hip_check(launch_rows_select(a, st), "rows select pass");
+      { PassTimer t("rows_select", (1 + nr + (add ? 1 : 0)) * pb + pb / 8, st); hip_check(launch_rows_select(a, st), "rows select pass"); }
     } else {  // "masked inputs" shape: PSPI adjoint and NSPS forward
       a.in = cin; a.scale = sx;
-      hip_check(launch_rows_fanout(a, st), "rows fan-out pass");
+      { PassTimer t("rows_fanout", (1 + nr) * pb + pb / 8, st); hip_check(launch_rows_fanout(a, st), "rows fan-out pass"); }
       a.scale = sy2;
-      hip_check(launch_cols_fanin(adjoint ? 1 : 0, a, st), "column fan-in pass");
+      { PassTimer t("cols_fanin", (1 + nr) * pb, st); hip_check(launch_cols_fanin(adjoint ? 1 : 0, a, st), "column fan-in pass"); }
       a.in = a.K; a.out = cout; a.add = add; a.scale = sx;
-      hip_check(launch_rows(taper ? 3 : 1, a, st), "rows inverse pass");
+      { PassTimer t("rows_inv", (2 + (add ? 1 : 0)) * pb, st); hip_check(launch_rows(taper ? 3 : 1, a, st), "rows inverse pass"); }
+    }
+  }
+}
+
+// The PSPI step at the reference's operator granularity, one shot per launch
+// (PSPI.cpp:66-103): taper, 2-D FFT (rows + columns), then per reference a
+// phase shift, an inverse 2-D FFT and a select-accumulate; the adjoint mirrors
+// it.  Same arithmetic as the fused path, one HBM round trip per operator.
+void StepEngine::apply_plain(bool adjoint, int iz, int add, const float2* in, float2* out, hipStream_t st) {
+  const fwix_dims& d = tables.dims;
+  const size_t plane = (size_t)d.NX * d.NY, shot = plane * d.NW;
+  const double sb = 8.0 * shot;
+  const float sx = (float)(1.0 / std::sqrt((double)d.NX)), sy = (float)(1.0 / std::sqrt((double)d.NY));
+  float2* Kb = (float2*)K.p;
+  float2* Tb = (float2*)T.p;
+  PwArgs pa = tables.pwargs();
+  pa.NS = 1;
+  pa.dz = d.dz;
+  pa.eps = params.eps;
+  pa.labels = labels + (size_t)iz * d.NW * d.NY * d.NX;
+  auto fft = [&](bool inv, float2* buf) {  // in-place unitary 2-D FFT of one shot
+    KArgs f = tables.kargs();
+    f.NS = 1; f.p0 = 0; f.np = d.NW;
+    f.in = buf; f.out = buf; f.add = 0; f.scale = sx;
+    { PassTimer t("plain_fft_rows", 2 * sb, st); hip_check(launch_rows(inv ? 1 : 0, f, st), "plain fft rows"); }
+    f.scale = sy;
+    { PassTimer t("plain_fft_cols", 2 * sb, st); hip_check(launch_cols(inv, f, st), "plain fft columns"); }
+  };
+  for (int is = 0; is < d.NS; ++is) {
+    const float2* sin_ = in + (size_t)is * shot;
+    float2* sout = out + (size_t)is * shot;
+    if (!adjoint) {
+      pa.add = 0;
+      { PassTimer t("plain_taper", 2 * sb, st); hip_check(launch_taper(sin_, Kb, pa, st), "plain taper"); }
+      fft(false, Kb);
+      if (!add) hip_check(launch_zero(sout, shot, st), "plain zero");
+      for (int r = 0; r < params.nref; ++r) {
+        pa.sref = sref + ((size_t)iz * params.nref + r) * d.NW;
+        pa.add = 0;
+        { PassTimer t("plain_ps", 2 * sb, st); hip_check(launch_phase_shift(0, Kb, Tb, pa, st), "plain phase shift"); }
+        fft(true, Tb);
+        pa.value = r; pa.add = 1;
+        { PassTimer t("plain_select", 3 * sb + sb / 8, st); hip_check(launch_select(Tb, sout, pa, st), "plain select"); }
+      }
+    } else {
+      hip_check(launch_zero(Kb, shot, st), "plain zero");
+      for (int r = 0; r < params.nref; ++r) {
+        pa.value = r; pa.add = 0;
+        { PassTimer t("plain_select", 2 * sb + sb / 8, st); hip_check(launch_select(sin_, Tb, pa, st), "plain select"); }
+        fft(false, Tb);
+        pa.sref = sref + ((size_t)iz * params.nref + r) * d.NW;
+        pa.add = 1;
+        { PassTimer t("plain_ps", 3 * sb, st); hip_check(launch_phase_shift(1, Tb, Kb, pa, st), "plain phase shift"); }
+      }
+      fft(true, Kb);
+      pa.add = add;
+      { PassTimer t("plain_taper", (2 + (add ? 1 : 0)) * sb, st); hip_check(launch_taper(Kb, sout, pa, st), "plain taper"); }
     }
   }
 }
@@ -438,7 +573,8 @@ int fwix_onestep_create(int method, const fwix_dims* d, const fwix_params* p, fw
   return guard([&] {
     validate_dims(d, true);
     validate_params(p);
-    if (method != FWIX_PSPI && method != FWIX_NSPS) throw Error("method must be FWIX_PSPI or FWIX_NSPS");
+    if (method != FWIX_PSPI && method != FWIX_NSPS && method != FWIX_PSPI_PLAIN)
+      throw Error("method must be FWIX_PSPI, FWIX_NSPS or FWIX_PSPI_PLAIN");
     if (d->nz < 1) throw Error("dims: nz must be >= 1");
     if (!out) throw Error("null output handle");
     auto* op = new OneStepOp();
@@ -449,9 +585,7 @@ int fwix_onestep_create(int method, const fwix_dims* d, const fwix_params* p, fw
 int fwix_onestep_set_refs(fwix_op op, const fwix_c64* sref, const uint8_t* labels) {
   return guard([&] {
     auto* p = as<OneStepOp>(op, KIND_ONESTEP);
-    if (!sref || !labels) throw Error("null reference slownesses or labels");
-    p->eng.sref = (const float2*)sref;
-    p->eng.labels = labels;
+    p->eng.set_refs((const float2*)sref, labels);
   });
 }
 int fwix_onestep_set_depth(fwix_op op, int iz) {
@@ -471,6 +605,60 @@ int fwix_onestep_adjoint(fwix_op op, int add, fwix_c64* m, const fwix_c64* d, fw
   return guard([&] {
     auto* p = as<OneStepOp>(op, KIND_ONESTEP);
     p->eng.apply(true, p->iz, add, (const float2*)d, (float2*)m, S(s));
+  });
+}
+
+// ---------------------------------------------------------------- timing
+int fwix_timing_enable(int on) {
+  return guard([&] {
+    std::lock_guard<std::mutex> g(g_tmu);
+    if (on > 0)  // pre-create event pairs so the timed region does not allocate them
+      while ((int)g_pool.size() < 2 * on) {
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        g_pool.push_back(e);
+      }
+    g_timing = on > 0;
+  });
+}
+int fwix_timing_collect(void) {
+  return guard([&] {
+    std::lock_guard<std::mutex> g(g_tmu);
+    for (auto& r : g_pending) {
+      hip_check(hipEventSynchronize(r.b), "hipEventSynchronize");
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
+      PassStat& ps = g_stat[r.name];
+      ps.ms += ms;
+      ps.bytes += r.bytes;
+      ps.n += 1;
+      g_pool.push_back(r.a);
+      g_pool.push_back(r.b);
+    }
+    g_pending.clear();
+  });
+}
+int fwix_timing_reset(void) {
+  return guard([&] {
+    std::lock_guard<std::mutex> g(g_tmu);
+    g_stat.clear();
+  });
+}
+int fwix_timing_report(char* buf, int len) {
+  return guard([&] {
+    std::lock_guard<std::mutex> g(g_tmu);
+    std::string out = "{";
+    bool first = true;
+    for (auto& kv : g_stat) {
+      char tmp[256];
+      std::snprintf(tmp, sizeof(tmp), "%s\"%s\": {\"ms\": %.6f, \"launches\": %lld, \"bytes\": %.0f}",
+                    first ? "" : ", ", kv.first.c_str(), kv.second.ms, kv.second.n, kv.second.bytes);
+      out += tmp;
+      first = false;
+    }
+    out += "}";
+    if (!buf || len < (int)out.size() + 1) throw Error("timing report buffer too small");
+    std::memcpy(buf, out.c_str(), out.size() + 1);
   });
 }
 

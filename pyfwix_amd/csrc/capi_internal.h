@@ -74,6 +74,16 @@ struct Tables {
 };
 
 void validate_dims(const fwix_dims* d, bool need_fft);
+
+// Live per-pass timing (fwix_timing_*): HIP events recorded on the launch
+// stream around every pass of the hot path, with the pass's compulsory HBM
+// bytes.  A no-op unless enabled.
+struct PassTimer {
+  PassTimer(const char* name, double bytes, hipStream_t st);
+  ~PassTimer();
+  int slot = -1;
+  hipStream_t st = nullptr;
+};
 void validate_params(const fwix_params* p);
 int auto_chunk(const fwix_dims& d, int nterm, int requested);
 
@@ -81,18 +91,25 @@ int auto_chunk(const fwix_dims& d, int nterm, int requested);
 // run as chunks of planes through the three fused passes.
 struct StepEngine {
   int method = FWIX_PSPI;
+  bool plain = false;  // reference-granularity per-operator kernels (comparator path)
+  DevBuf T;            // plain path: second whole-shot scratch
   fwix_params params{};
   Tables tables;
   const float2* sref = nullptr;     // borrowed (nz, nref, NW)
   const uint8_t* labels = nullptr;  // borrowed (nz, NW, NY, NX)
   DevBuf K, W;
+  DevBuf rowmask, planemask;  // reference presence per (depth, w, y) row and per (depth, w) plane
+  bool masks = false;
   int chunk = 1;
+  // borrow refs / labels for all depths and derive the presence masks
+  void set_refs(const float2* sref, const uint8_t* labels);
   // wterms: planes per chunk of the W scratch (>= nref; the Born path also
   // stores its taylor+1 scattering terms there)
   void init(int method, const fwix_dims& d, const fwix_params& p, int wterms = 0);
   void check_ready(int iz) const;
   KArgs base(int iz, int c0, int np) const;
   void apply(bool adjoint, int iz, int add, const float2* in, float2* out, hipStream_t st);
+  void apply_plain(bool adjoint, int iz, int add, const float2* in, float2* out, hipStream_t st);
 };
 
 

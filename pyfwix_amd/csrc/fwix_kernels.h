@@ -39,6 +39,8 @@ struct KArgs {
   float2* K;               // chunk spectrum scratch (np planes)
   float2* W;               // per-reference / per-Taylor-term scratch (nterm * np planes)
   const uint8_t* labels;   // (NW, NY, NX) reference labels of the current depth
+  const uint32_t* rowmask; // (NW, NY) bit r set: reference r labels a node of that row (null: all)
+  const uint32_t* planemask; // (NW) OR of rowmask over the plane's rows (null: all)
   const float2* sref;      // (nref, NW) reference slownesses of the current depth
   const float* kx;         // (NX) wavenumbers
   const float* ky;         // (NY)
@@ -63,6 +65,12 @@ struct KArgs {
 };
 
 __device__ __forceinline__ int plane_iw(const KArgs& a, int p_local) { return (a.p0 + p_local) % a.NW; }
+
+// lowest set bit of m above position r (32 when none); r = -1 gives the lowest
+__device__ __forceinline__ int next_bit(uint32_t m, int r) {
+  const uint32_t rest = r >= 31 ? 0u : (m & ~((2u << r) - 1u));
+  return rest ? __builtin_ctz(rest) : 32;
+}
 
 // sin and cos of x for |x| up to ~1e4: Cody-Waite reduction by pi/2 with a
 // three-part constant (1.5703125 + 4.837512969970703e-4 + 7.54979012640433e-8,

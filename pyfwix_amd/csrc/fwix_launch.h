@@ -31,6 +31,8 @@ hipError_t launch_rows_select(const KArgs& a, hipStream_t st);
 hipError_t launch_cols(bool inv, const KArgs& a, hipStream_t st);
 hipError_t launch_cols_fanout(int mode, const KArgs& a, hipStream_t st);
 hipError_t launch_cols_fanin(int mode, const KArgs& a, hipStream_t st);
+hipError_t launch_presence(const uint8_t* labels, uint32_t* rowmask, uint32_t* planemask, int NX, int NY,
+                           long long nplanes, hipStream_t st);
 
 // point-wise kernels (ops_kernels.hip)
 hipError_t launch_phase_shift(int mode, const float2* m, float2* d, const PwArgs& a, hipStream_t st);

@@ -116,7 +116,20 @@ __global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
     }
   }
   const float s = a.scale;
+  uint32_t wm = 0xffffffffu;
+  if (a.rowmask) {
+    wm = 0;
+    for (int b = 0; b < C::NB; ++b) {
+      const int lr = rb + b;
+      if (lr < nrows) {
+        const int p = lr / a.NY;
+        wm |= a.rowmask[plane_iw(a, p) * a.NY + (lr - p * a.NY)];
+      }
+    }
+    wm = __builtin_amdgcn_readfirstlane(wm);
+  }
   for (int r = 0; r < a.nref; ++r) {
+    if (r < 32 && !((wm >> r) & 1u)) continue;  // rows without reference r: the column pass reads zeros
     float2 x[C::V];
 #pragma unroll
     for (int e = 0; e < C::V; ++e) x[e] = (lab[e] == r) ? u[e] : czero();
@@ -157,6 +170,20 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
       lab[e] = a.labels[((size_t)plane_iw(a, p) * a.NY + iy) * N + i];
     }
   }
+  // references present in this wave's rows (wave-uniform)
+  uint32_t wm = 0xffffffffu;
+  if (a.rowmask) {
+    wm = 0;
+    for (int b = 0; b < C::NB; ++b) {
+      const int lr = rb + b;
+      if (lr < nrows) {
+        const int p = lr / a.NY;
+        wm |= a.rowmask[plane_iw(a, p) * a.NY + (lr - p * a.NY)];
+      }
+    }
+    wm = __builtin_amdgcn_readfirstlane(wm);
+  }
+  const uint32_t live = a.nref >= 32 ? wm : (wm & ((1u << a.nref) - 1u));
   float2 nxt[C::V];
   auto fetch = [&](int r) {
     const float2* W = a.W + (size_t)r * a.wstride;
@@ -168,16 +195,21 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
       nxt[e] = lr < nrows ? W[(size_t)lr * N + i] : czero();
     }
   };
-  fetch(0);
-  for (int r = 0; r < a.nref; ++r) {
+  // absent references contribute nothing to these rows: skip their loads and transforms
+  int r = a.rowmask ? next_bit(live, -1) : 0;
+  const int rend = a.rowmask ? 32 : a.nref;
+  if (r < rend) fetch(r);
+  while (r < rend) {
     float2 x[C::V];
 #pragma unroll
     for (int e = 0; e < C::V; ++e) x[e] = nxt[e];
-    if (r + 1 < a.nref) fetch(r + 1);
+    const int rn = a.rowmask ? next_bit(live, r) : r + 1;
+    if (rn < rend) fetch(rn);
     F::run(x, wl, a.twx, lane);
 #pragma unroll
     for (int e = 0; e < C::V; ++e)
       if (lab[e] == r) acc[e] = cadd(acc[e], x[e]);
+    r = rn;
   }
   const float s = a.scale;
 #pragma unroll
@@ -251,6 +283,16 @@ struct TileRegs {
       t[k] = col < NX ? src[(size_t)i * NX + col] : czero();
     }
   }
+  // rows whose mask bit is clear are known to be zero and are not loaded
+  __device__ __forceinline__ void fetch_masked(const float2* __restrict__ src, int NX, int c0,
+                                               const uint32_t* __restrict__ rm, uint32_t bit) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = threadIdx.x + k * C::THREADS;
+      const int i = e / C::CW, c = e % C::CW, col = c0 + c;
+      t[k] = (col < NX && (rm[i] & bit)) ? src[(size_t)i * NX + col] : czero();
+    }
+  }
   __device__ __forceinline__ void commit(float2* tile) const {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
@@ -264,6 +306,18 @@ __device__ __forceinline__ void tile_load(float2* tile, const float2* __restrict
   TileRegs<C> r;
   r.fetch(src, NX, c0);
   r.commit(tile);
+}
+// store only the rows whose mask bit is set (rows that a later pass reads)
+template <class C>
+__device__ __forceinline__ void tile_store_masked(const float2* tile, float2* __restrict__ dst, int NX, int c0,
+                                                  const uint32_t* __restrict__ rm, uint32_t bit) {
+  constexpr int PER = C::NROW * C::CW / C::THREADS;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int e = threadIdx.x + k * C::THREADS;
+    const int i = e / C::CW, col = c0 + e % C::CW;
+    if (col < NX && (rm[i] & bit)) dst[(size_t)i * NX + col] = tile[i * C::CWP + e % C::CW];
+  }
 }
 template <class C>
 __device__ __forceinline__ void tile_store(const float2* tile, float2* __restrict__ dst, int NX, int c0, int add) {
@@ -356,7 +410,10 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_cols_fanout(KArgs a) {
   const float w2 = a.w2[iw];
   const float s2 = a.scale;  // 1/NY: forward and inverse column normalisation together
   const int colw = c0 + wave * C::NB;
+  const uint32_t* rm = (MODE != 2 && a.rowmask) ? a.rowmask + iw * a.NY : nullptr;
+  const uint32_t pm = (MODE != 2 && a.planemask) ? a.planemask[iw] : 0xffffffffu;
   for (int r = 0; r < a.nterm; ++r) {
+    if (r < 32 && !((pm >> r) & 1u)) continue;  // reference absent from the plane: W_r is never read
     const TermMult<MODE> tm(a, r, iw, w2);
     float2 x[C::V];
     auto mult = [&](auto real) {
@@ -375,7 +432,8 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_cols_fanout(KArgs a) {
     __syncthreads();
     tile_write<N, C, Fi::P::last()>(smem, x, wave, lane);
     __syncthreads();
-    tile_store<C>(smem, a.W + (size_t)r * a.wstride + pb, a.NX, c0, 0);
+    if (rm) tile_store_masked<C>(smem, a.W + (size_t)r * a.wstride + pb, a.NX, c0, rm, r < 32 ? 1u << r : 0xffffffffu);
+    else tile_store<C>(smem, a.W + (size_t)r * a.wstride + pb, a.NX, c0, 0);
     __syncthreads();
   }
 }
@@ -398,13 +456,24 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_cols_fanin(KArgs a) {
   float2 acc[C::V];
 #pragma unroll
   for (int e = 0; e < C::V; ++e) acc[e] = czero();
+  const uint32_t* rm = (MODE != 2 && a.rowmask) ? a.rowmask + iw * a.NY : nullptr;
+  const uint32_t pm = (MODE != 2 && a.planemask) ? a.planemask[iw] : 0xffffffffu;
+  const uint32_t live = a.nterm >= 32 ? pm : (pm & ((1u << a.nterm) - 1u));
+  const bool masked = rm != nullptr;
+  const int rend = masked ? 32 : a.nterm;
+  auto fetch = [&](TileRegs<C>& tr, int r) {
+    if (masked) tr.fetch_masked(a.W + (size_t)r * a.wstride + pb, a.NX, c0, rm, 1u << r);
+    else tr.fetch(a.W + (size_t)r * a.wstride + pb, a.NX, c0);
+  };
   TileRegs<C> next;
-  next.fetch(a.W + pb, a.NX, c0);
-  for (int r = 0; r < a.nterm; ++r) {
+  int r = masked ? next_bit(live, -1) : 0;
+  if (r < rend) fetch(next, r);
+  while (r < rend) {
     next.commit(smem);
     __syncthreads();
-    // the next term's loads are in flight during this term's transform
-    if (r + 1 < a.nterm) next.fetch(a.W + (size_t)(r + 1) * a.wstride + pb, a.NX, c0);
+    // the next present term's loads are in flight during this term's transform
+    const int rn = masked ? next_bit(live, r) : r + 1;
+    if (rn < rend) fetch(next, rn);
     float2 v[C::V];
     tile_read<N, C, F::P::first()>(v, smem, wave, lane);
     __syncthreads();
@@ -423,6 +492,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_cols_fanin(KArgs a) {
     if (tm.real) mult(std::true_type{});
     else mult(std::false_type{});
     __syncthreads();
+    r = rn;
   }
   const float s2 = a.scale;
 #pragma unroll
@@ -448,6 +518,39 @@ bool fft_size_supported(int n) {
 }
 
 int col_tile_width(int n) { return col_cw(n); }
+
+// Presence masks of reference labels: bit r of rowmask[row] is set when
+// label r occurs in that (w, y) row; planemask is the OR over a plane's rows.
+__global__ __launch_bounds__(kThreads) void k_presence(const uint8_t* __restrict__ labels, uint32_t* rowmask,
+                                                       int NX, long long nrows) {
+  const long long row = (long long)blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= nrows) return;
+  uint32_t m = 0;
+  for (int x = lane; x < NX; x += 64) {
+    const int l = labels[row * NX + x];
+    if (l < 32) m |= 1u << l;
+  }
+  for (int off = 32; off > 0; off >>= 1) m |= (uint32_t)__shfl_xor((int)m, off);
+  if (lane == 0) rowmask[row] = m;
+}
+__global__ __launch_bounds__(kThreads) void k_planemask(const uint32_t* __restrict__ rowmask, uint32_t* planemask,
+                                                        int NY, long long nplanes) {
+  const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (p >= nplanes) return;
+  uint32_t m = 0;
+  for (int y = 0; y < NY; ++y) m |= rowmask[p * NY + y];
+  planemask[p] = m;
+}
+hipError_t launch_presence(const uint8_t* labels, uint32_t* rowmask, uint32_t* planemask, int NX, int NY,
+                           long long nplanes, hipStream_t st) {
+  const long long nrows = nplanes * NY;
+  hipLaunchKernelGGL(k_presence, dim3((unsigned)((nrows + kWaves - 1) / kWaves)), dim3(kThreads), 0, st, labels,
+                     rowmask, NX, nrows);
+  hipLaunchKernelGGL(k_planemask, dim3((unsigned)((nplanes + kThreads - 1) / kThreads)), dim3(kThreads), 0, st,
+                     rowmask, planemask, NY, nplanes);
+  return hipGetLastError();
+}
 
 template <class Fn>
 static hipError_t dispatch_n(int n, Fn&& fn) {

@@ -14,11 +14,11 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import FWIX_NSPS, FWIX_PSPI, Handle, call, create, default_params, make_dims
+from ._lib import FWIX_NSPS, FWIX_PSPI, FWIX_PSPI_PLAIN, Handle, call, create, default_params, make_dims
 
 __all__ = ["FFT2d", "PhaseShift", "Selector", "Taper", "OneStep", "Injection", "Scatter",
            "ImagingCondition", "OneWay", "Born", "ptr", "stream",
-           "make_dims", "default_params", "FWIX_PSPI", "FWIX_NSPS"]
+           "make_dims", "default_params", "FWIX_PSPI", "FWIX_NSPS", "FWIX_PSPI_PLAIN"]
 
 
 def ptr(t: torch.Tensor | None) -> C.c_void_p:

@@ -152,3 +152,69 @@ def test_pspi_equal_refs_is_gazdag(dev):
         out = torch.zeros((g.NS, g.NW, g.NY, g.NX), dtype=torch.complex64, device=dev)
         op.forward(False, _t(x, dev), out)
         assert rel(out.cpu().numpy(), ref) < 2e-6
+
+
+@pytest.mark.parametrize("NX,NY", [(64, 48), (512, 96), (24, 32)])
+@pytest.mark.parametrize("method", ["pspi", "nsps"])
+def test_step_sparse_references(dev, NX, NY, method):
+    """References absent from whole planes or rows (the presence masks skip them)."""
+    import torch
+    from pyfwix_amd.ops import FWIX_NSPS, FWIX_PSPI, OneStep
+    g = grid(NX, NY, 3, 2, nz=2)
+    rng = np.random.default_rng(77)
+    nref = 5
+    sref = (1.0 / rng.uniform(1500, 3000, (2, nref, g.NW)) ** 2).astype(np.complex64)
+    labels = np.zeros((2, g.NW, g.NY, g.NX), np.uint8)
+    labels[1, 0] = 2                                        # plane with a single reference
+    labels[1, 1, : NY // 2] = 4                             # rows with 4, rows with 0
+    labels[1, 1, NY // 2:, : NX // 3] = 1                   # rows mixing 0 and 1
+    labels[1, 2] = rng.choice([0, 3], (g.NY, g.NX))         # refs 1, 2, 4 absent
+    op = OneStep(FWIX_PSPI if method == "pspi" else FWIX_NSPS, dims_of(g), nref=nref, taperx=3, tapery=2)
+    op.set_refs(_t(sref, dev), _t(labels, dev, np.uint8))
+    op.set_depth(1)
+    shape = (g.NS, g.NW, g.NY, g.NX)
+    x, y = crand(rng, shape), crand(rng, shape)
+    s64, lab = sref[1].astype(np.complex128), labels[1].astype(np.int64)
+    out = torch.zeros(shape, dtype=torch.complex64, device=dev)
+    op.forward(False, _t(x, dev), out)
+    ref = (O.pspi_forward(x, s64, lab, g, g.dz, 0.0, 3, 2) if method == "pspi"
+           else O.nsps_forward(x, s64, lab, g, g.dz, 0.0))
+    assert rel(out.cpu().numpy(), ref) < TOL_WFLD
+    m = torch.zeros_like(out)
+    op.adjoint(False, m, _t(y, dev))
+    ref = (O.pspi_adjoint(y, s64, lab, g, g.dz, 0.0, 3, 2) if method == "pspi"
+           else O.nsps_adjoint(y, s64, lab, g, g.dz, 0.0))
+    assert rel(m.cpu().numpy(), ref) < TOL_WFLD
+    assert dot_err(x, out.cpu().numpy(), y, m.cpu().numpy()) < TOL_DOT
+
+
+@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[3]], ids=lambda c: "x".join(map(str, c)))
+def test_plain_path_matches_fused_and_oracle(dev, case):
+    """The per-operator comparator path computes the same PSPI step."""
+    import torch
+    from pyfwix_amd.ops import FWIX_PSPI, FWIX_PSPI_PLAIN, OneStep
+    from pyfwix_amd.refsampler import select_refs
+    NX, NY, NW, NS, nref, tap = case
+    g = grid(NX, NY, NW, NS, nx=NX - NX // 4, ny=NY - NY // 4, nz=2)
+    rng = np.random.default_rng(5)
+    slow = smooth_slowness(rng, g)
+    sref, labels = select_refs(slow, nref, NX - g.nx, NY - g.ny)
+    shape = (g.NS, g.NW, g.NY, g.NX)
+    x = crand(rng, shape)
+    outs = {}
+    for name, method in (("fused", FWIX_PSPI), ("plain", FWIX_PSPI_PLAIN)):
+        op = OneStep(method, dims_of(g), nref=nref, taperx=tap, tapery=tap)
+        op.set_refs(_t(sref, dev), _t(labels, dev, np.uint8))
+        op.set_depth(1)
+        f = torch.zeros(shape, dtype=torch.complex64, device=dev)
+        op.forward(False, _t(x, dev), f)
+        a = _t(x, dev)
+        op.adjoint(True, a, _t(x, dev))
+        outs[name] = (f.cpu().numpy(), a.cpu().numpy())
+    s64, lab = sref[1].astype(np.complex128), labels[1].astype(np.int64)
+    ref_f = O.pspi_forward(x, s64, lab, g, g.dz, 0.0, tap, tap)
+    ref_a = x + O.pspi_adjoint(x, s64, lab, g, g.dz, 0.0, tap, tap)
+    for name in outs:
+        assert rel(outs[name][0], ref_f) < TOL_WFLD
+        assert rel(outs[name][1], ref_a) < TOL_WFLD
+    assert rel(outs["plain"][0], outs["fused"][0]) < 2e-6

@@ -1,0 +1,258 @@
+"""CPU checks that pin the FP64 oracle itself (no GPU).
+
+Every check is an independent statement of the mathematics or a known answer
+from the reference's own tests, so that a dropped term, a wrong sign or index
+or a transposed axis in the oracle fails here:
+
+* FFT: brute-force DFT matrix, DC and plane-wave spikes
+  (operator/tests/operator_unit_test.cpp:50-99), unitarity.
+* wavenumbers: numpy.fft.fftfreq, with the reference's +Nyquist convention
+  (propagator/src/PhaseShift.h:49-55).
+* phase shift: exp(-i dz sqrt(w^2 s - k^2)) with numpy's principal complex
+  square root; adjoint = conjugate and inverse = reciprocal when
+  Im s <= eps Re s (prop_unit_test.cpp:56-67).
+* PSPI / NSPS: the textbook Gazdag phase shift when all references agree,
+  a plane wave gains exactly its multiplier, masks partition unity,
+  dot-product tests.
+* taper: closed-form values at the edges.
+* scatter: the Taylor series of dkz/ds = w^2 / (2 kz), checked against the
+  exact derivative of kz = sqrt(w^2 s - k^2) for a plane wave, and against a
+  central finite difference of the phase-shift multiplier.
+* injection: weights of a trace sum to one over its 8 corners; a trace on a
+  node hits only that node; adjoint pair.
+* recursions and Born: dot-product tests and a two-step unrolled example.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from fwix_testutil import crand, dot_err, grid, rel, smooth_slowness
+
+TOL = 1e-11
+
+
+def dft_matrix(n):
+    j = np.arange(n)
+    return np.exp(-2j * np.pi * np.outer(j, j) / n) / math.sqrt(n)
+
+
+def test_fft2_is_the_unitary_dft():
+    rng = np.random.default_rng(0)
+    x = crand(rng, (2, 3, 6, 4))
+    Fy, Fx = dft_matrix(6), dft_matrix(4)
+    ref = np.einsum("ab,stbc,dc->stad", Fy, x, Fx)
+    assert rel(O.fft2(x), ref) < 1e-13
+    assert rel(O.ifft2(O.fft2(x)), x) < 1e-13
+    assert abs(np.linalg.norm(O.fft2(x)) - np.linalg.norm(x)) < 1e-10
+
+
+def test_fft2_known_answers():
+    n1, n2 = 12, 8
+    out = O.fft2(np.ones((1, 1, n2, n1)))
+    assert abs(out[0, 0, 0, 0] - math.sqrt(n1 * n2)) < 1e-12
+    out[0, 0, 0, 0] = 0
+    assert np.abs(out).max() < 1e-12
+    k1, k2 = 3, 5
+    i1, i2 = np.arange(n1)[None, :], np.arange(n2)[:, None]
+    pw = np.exp(2j * np.pi * (k1 * i1 / n1 + k2 * i2 / n2))[None, None]
+    out = O.fft2(pw)
+    assert abs(out[0, 0, k2, k1] - math.sqrt(n1 * n2)) < 1e-12
+    out[0, 0, k2, k1] = 0
+    assert np.abs(out).max() < 1e-12
+
+
+@pytest.mark.parametrize("n", [8, 9, 12])
+def test_kgrid_convention(n):
+    k = O.kgrid(n, 2.5)
+    ref = 2 * np.pi * np.fft.fftfreq(n, 2.5)
+    if n % 2 == 0:  # reference keeps +Nyquist (ik <= n/2 positive)
+        ref[n // 2] = -ref[n // 2]
+    assert np.allclose(k, ref, rtol=0, atol=1e-15)
+
+
+def test_phase_shift_closed_form():
+    rng = np.random.default_rng(1)
+    nw = 5
+    kx, ky, w2 = O.kgrid(16, 10.0), O.kgrid(12, 10.0), O.w2grid(nw, 5.0, 4.0)
+    k2 = kx[None, None, :] ** 2 + ky[None, :, None] ** 2
+    for eps, imag in ((0.0, 0.0), (0.04, 0.0), (0.0, -0.3), (0.05, 0.01)):
+        s = (1.0 / rng.uniform(1500, 2500, nw) ** 2) * (1 + 1j * imag)
+        b = w2 * (s.imag - eps * s.real)
+        assert np.all(b <= 0)
+        a = w2[:, None, None] * s.real[:, None, None] - k2
+        bb = np.broadcast_to(b[:, None, None], a.shape)
+        kz = np.conj(np.sqrt(a - 1j * bb))  # kr - i ki with ki >= 0: the decaying (evanescent) branch
+        fwd = O.phase_shift_mult(s, kx, ky, w2, 7.0, eps, "fwd")
+        # 1e-12: the reference's (c +- a)/2 form cancels in FP64 where |b| << |a|
+        assert np.allclose(fwd, np.exp(-1j * 7.0 * kz), rtol=0, atol=1e-12)
+        adj = O.phase_shift_mult(s, kx, ky, w2, 7.0, eps, "adj")
+        inv = O.phase_shift_mult(s, kx, ky, w2, 7.0, eps, "inv")
+        assert np.allclose(adj, np.conj(fwd), rtol=0, atol=1e-13)
+        assert np.allclose(inv * fwd, 1.0, rtol=0, atol=1e-9)
+    # propagating modes keep unit modulus, evanescent modes decay as exp(-sqrt(k^2 - w^2 s) dz)
+    s = np.full(nw, 1.0 / 2000.0 ** 2)
+    fwd = O.phase_shift_mult(s, kx, ky, w2, 7.0, 0.0, "fwd")
+    a = w2[:, None, None] * s[0] - k2
+    assert np.allclose(np.abs(fwd[a > 0]), 1.0, atol=1e-14)
+    assert np.allclose(fwd[a < 0], np.exp(-np.sqrt(-a[a < 0]) * 7.0), atol=1e-14)
+
+
+def _refs(g, rng, nref):
+    from pyfwix_amd.refsampler import select_refs
+    slow = smooth_slowness(rng, g)
+    sref, labels = select_refs(slow, nref, g.NX - g.nx, g.NY - g.ny)
+    return slow, sref.astype(np.complex128), labels.astype(np.int64)
+
+
+def test_pspi_reduces_to_gazdag_and_plane_wave():
+    g = grid(16, 12, 3, 2, nz=1)
+    rng = np.random.default_rng(2)
+    s0 = 1.0 / 1800.0 ** 2
+    sref = np.full((3, g.NW), s0, complex)
+    labels = rng.integers(0, 3, (g.NW, g.NY, g.NX))
+    x = crand(rng, (g.NS, g.NW, g.NY, g.NX))
+    kx, ky, w2 = O.kgrid(g.NX, g.dx), O.kgrid(g.NY, g.dy), O.w2grid(g.NW, g.ow, g.dw)
+    phi = O.phase_shift_mult(sref[0], kx, ky, w2, g.dz, 0.0)
+    gaz = O.ifft2(phi[None] * O.fft2(x))
+    assert rel(O.pspi_forward(x, sref, labels, g, g.dz), gaz) < TOL
+    assert rel(O.nsps_forward(x, sref, labels, g, g.dz), gaz) < TOL
+    # a plane wave exp(i (kx x + ky y)) gains exactly Phi(kx, ky): catches swapped axes
+    jx, jy = 3, 2
+    X = np.arange(g.NX)[None, :]
+    Y = np.arange(g.NY)[:, None]
+    pw = np.exp(2j * np.pi * (jx * X / g.NX + jy * Y / g.NY))
+    x = np.broadcast_to(pw, (g.NS, g.NW, g.NY, g.NX)).astype(complex)
+    out = O.pspi_forward(x, sref, labels, g, g.dz)
+    assert rel(out, phi[None, :, jy, jx][..., None, None] * x) < TOL
+
+
+def test_masks_partition_unity_and_taper_edges():
+    rng = np.random.default_rng(3)
+    x = crand(rng, (2, 3, 10, 14))
+    lab = rng.integers(0, 4, (3, 10, 14))
+    assert rel(sum(O.select(x, lab, r) for r in range(4)), x) < 1e-15
+    w = O.taper_weights(14, 10, 4, 3)
+    wx, wy = w[5, :] / w[5, 7], w[:, 7] / w[5, 7]
+    assert wx[0] == 0 and wx[4] == 1 and wx[9] == 1 and wy[0] == 0 and wy[3] == 1
+    assert abs(wx[1] - 0.5 * (1 - math.cos(math.pi / 4))) < 1e-15
+    assert abs(wx[13] - 0.5 * (1 + math.cos(math.pi * 3 / 4))) < 1e-15
+    assert abs(wx[10] - 1.0) < 1e-15
+
+
+@pytest.mark.parametrize("method", ["pspi", "nsps"])
+def test_step_dot_products(method):
+    rng = np.random.default_rng(4)
+    g = grid(16, 12, 3, 2, nx=12, ny=9, nz=1)
+    _, sref, labels = _refs(g, rng, 3)
+    x, y = crand(rng, (g.NS, g.NW, g.NY, g.NX)), crand(rng, (g.NS, g.NW, g.NY, g.NX))
+    if method == "pspi":
+        Ax = O.pspi_forward(x, sref[0], labels[0], g, g.dz, 0.04, 3, 2)
+        Aty = O.pspi_adjoint(y, sref[0], labels[0], g, g.dz, 0.04, 3, 2)
+    else:
+        Ax = O.nsps_forward(x, sref[0], labels[0], g, g.dz, 0.04)
+        Aty = O.nsps_adjoint(y, sref[0], labels[0], g, g.dz, 0.04)
+    assert dot_err(x, Ax, y, Aty) < 1e-12
+    if method == "pspi":  # the reference's non-in-place adjoint drops the taper: not an adjoint
+        bad = O.pspi_adjoint(y, sref[0], labels[0], g, g.dz, 0.04, 3, 2, taper_adjoint=False)
+        assert dot_err(x, Ax, y, bad) > 1e-6
+
+
+@pytest.mark.parametrize("taylor", [0, 1, 4])
+def test_scatter_is_the_taylor_series_of_dkz_ds(taylor):
+    # laterally constant slowness, one plane wave at small |k|
+    g = grid(32, 16, 2, 1, nz=1, ow=20.0, dw=5.0)
+    s = 1.0 / 1700.0 ** 2
+    slow = np.full((g.NW, g.ny, g.nx), s, complex)
+    jx, jy = 1, 1
+    X, Y = np.arange(g.NX)[None, :], np.arange(g.NY)[:, None]
+    x = np.broadcast_to(np.exp(2j * np.pi * (jx * X / g.NX + jy * Y / g.NY)), (1, g.NW, g.NY, g.NX)).astype(complex)
+    out = O.scatter_forward(x, slow, g, g.dz, eps_s=0.0, ntaylor=taylor)
+    w = O.wgrid(g.NW, g.ow, g.dw)
+    k2 = O.kgrid(g.NX, g.dx)[jx] ** 2 + O.kgrid(g.NY, g.dy)[jy] ** 2
+    q = k2 / (w ** 2 * s)
+    coef = [math.comb(2 * n, n) / 4 ** n for n in range(5)]  # (1-q)^-1/2 binomial series
+    series = sum(coef[n] * q ** n for n in range(taylor + 1))
+    mult = -1j * g.dz * 0.5 * w / math.sqrt(s) * series
+    assert rel(out, mult[None, :, None, None] * x) < 1e-10  # q = 0.35: a wrong coefficient errs by >1e-4
+    exact = -1j * g.dz * w ** 2 / (2 * np.sqrt(w ** 2 * s - k2))  # -i dz d(kz)/ds
+    assert np.all(np.abs(mult / exact - 1) <= 1.2 * q ** (taylor + 1) + 1e-12)
+    # and d/ds of the phase-shift multiplier exp(-i kz dz) by central differences
+    h = s * 1e-5
+    kx, ky = O.kgrid(g.NX, g.dx), O.kgrid(g.NY, g.dy)
+    w2 = O.w2grid(g.NW, g.ow, g.dw)
+    pp = O.phase_shift_mult(np.full(g.NW, s + h), kx, ky, w2, g.dz, 0.0)[:, jy, jx]
+    pm = O.phase_shift_mult(np.full(g.NW, s - h), kx, ky, w2, g.dz, 0.0)[:, jy, jx]
+    p0 = O.phase_shift_mult(np.full(g.NW, s), kx, ky, w2, g.dz, 0.0)[:, jy, jx]
+    fd = (pp - pm) / (2 * h) / p0
+    assert np.allclose(fd, exact, rtol=1e-7)
+
+
+def test_scatter_and_ic_dot_products():
+    rng = np.random.default_rng(5)
+    g = grid(16, 12, 3, 2, nx=12, ny=10, nz=1)
+    slow = smooth_slowness(rng, g, imag=0.02)[0]
+    x, y = crand(rng, (g.NS, g.NW, g.NY, g.NX)), crand(rng, (g.NS, g.NW, g.NY, g.NX))
+    assert dot_err(x, O.scatter_forward(x, slow, g, g.dz, 0.04, 2), y,
+                   O.scatter_adjoint(y, slow, g, g.dz, 0.04, 2)) < 1e-12
+    U = crand(rng, x.shape)
+    img = crand(rng, (g.NW, g.ny, g.nx))
+    assert dot_err(img, O.ic_forward(img, U), y, O.ic_adjoint(y, U, g.nx, g.ny)) < 1e-12
+
+
+def test_injection_weights_and_adjoint():
+    g = grid(12, 10, 2, 2, nz=4, dx=5.0, dy=4.0, dz=3.0)
+    rng = np.random.default_rng(6)
+    n = 30
+    cx = rng.uniform(0, 10 * 5.0, n).astype(np.float32)
+    cy = rng.uniform(0, 8 * 4.0, n).astype(np.float32)
+    cz = rng.uniform(0, 2.5 * 3.0, n).astype(np.float32)
+    ids = rng.integers(0, 2, n)
+    cx[0], cy[0], cz[0] = 15.0, 8.0, 3.0          # exactly on node (ix 3, iy 2, iz 1)
+    tot = np.zeros(n)
+    for iz in range(4):
+        for t, s, iy, ix, w in O.injection_weights(cx, cy, cz, ids, g, iz):
+            tot[t] += sum(w)
+    assert np.allclose(tot, 1.0, atol=1e-6)
+    tr = np.zeros((n, g.NW), complex)
+    tr[0] = 1.0
+    wf = O.injection_forward(tr, cx, cy, cz, ids, g, 1, shape=(g.NS, g.NW, g.NY, g.NX))
+    nz_idx = np.argwhere(np.abs(wf) > 0)
+    assert {tuple(i[2:]) for i in nz_idx} == {(2, 3)}
+    assert np.allclose(wf[ids[0], :, 2, 3], 1.0)
+    x = crand(rng, (n, g.NW))
+    y = crand(rng, (g.NS, g.NW, g.NY, g.NX))
+    for iz in range(4):
+        Ax = O.injection_forward(x, cx, cy, cz, ids, g, iz, shape=y.shape)
+        Aty = O.injection_adjoint(y, cx, cy, cz, ids, g, iz)
+        assert dot_err(x, Ax, y, Aty) < 1e-12
+
+
+def test_recursions_and_born_dot_products():
+    rng = np.random.default_rng(7)
+    g = grid(16, 12, 2, 1, nx=12, ny=9, nz=3, ow=6.0, dw=3.0)
+    slow, sref, labels = _refs(g, rng, 2)
+    x, y = crand(rng, (g.NS, g.NW, g.NY, g.NX)), crand(rng, (g.NS, g.NW, g.NY, g.NX))
+    # unrolled two-step example fixes the order of the depth loop
+    two = O.downward_forward(x, sref[:2], labels[:2], g, g.dz, 0.0, 2, 2)
+    manual = O.pspi_forward(O.pspi_forward(x, sref[0], labels[0], g, g.dz, 0.0, 2, 2), sref[1], labels[1],
+                            g, g.dz, 0.0, 2, 2)
+    assert rel(two, manual) < 1e-14
+    for f, b in ((O.downward_forward, O.downward_adjoint), (O.upward_forward, O.upward_adjoint)):
+        assert dot_err(x, f(x, sref, labels, g, g.dz, 0.0, 2, 2), y, b(y, sref, labels, g, g.dz, 0.0, 2, 2)) < 1e-12
+    src = (np.array([55.0], np.float32), np.array([45.0], np.float32), np.array([2.0], np.float32), np.array([0]))
+    rx, ry = np.meshgrid(np.arange(1, 11) * 10.0, np.arange(1, 8) * 10.0)
+    rec = (rx.ravel().astype(np.float32), ry.ravel().astype(np.float32), np.full(rx.size, 4.0, np.float32),
+           np.zeros(rx.size, int))
+    wav = crand(rng, (1, g.NW))
+    _, bg = O.model_background(wav, src, rec, sref, labels, g, g.dz, 0.0, 2, 2)
+    ds = crand(rng, (g.nz, g.NW, g.ny, g.nx))
+    dd = crand(rng, (rx.size, g.NW))
+    d = O.born_forward(ds, bg, slow, rec, sref, labels, g, g.dz, 0.0, 2, 2, 0.04, 1)
+    m = O.born_adjoint(dd, bg, slow, rec, sref, labels, g, g.dz, 0.0, 2, 2, 0.04, 1)
+    assert dot_err(ds, d, dd, m) < 1e-12
+    img = O.migrate(dd, bg, rec, sref, labels, g, g.dz, 0.0, 2, 2)
+    # the migration image is the Born adjoint's imaging core: with S = identity
+    # it equals sum_w of the IC adjoint, a real identity we can check on one depth
+    assert img.shape == (g.nz, g.ny, g.nx) and np.all(np.isfinite(img))
