@@ -174,20 +174,25 @@ struct ScatterOp : fwix_op_s {
       a.slow = slow + (size_t)iz * d.NW * d.ny * d.nx;
       const float2* cin = in + (size_t)c0 * plane;
       float2* cout = out + (size_t)c0 * plane;
+      const int T = eng.params.taylor;
       if (!adjoint) {  // Scatter.cpp:90-102
-        a.in = cin; a.out = a.K; a.add = 0; a.scale = sx;
-        hip_check(launch_rows(0, a, st), "scatter rows");
-        a.scale = sy2;
-        hip_check(launch_cols_fanout(2, a, st), "scatter columns");
-        a.out = cout; a.add = add; a.scale = sx;
+        if (T >= 1) {
+          a.in = cin; a.out = a.K; a.add = 0; a.scale = sx;
+          hip_check(launch_rows(0, a, st), "scatter rows");
+          a.nterm = T; a.scale = sy2;
+          hip_check(launch_cols_fanout(2, a, st), "scatter columns");
+        }
+        a.in = cin; a.out = cout; a.add = add; a.nterm = T + 1; a.scale = sx;
         hip_check(launch_rows_scatter(false, a, st), "scatter combine");
       } else {  // Scatter.cpp:104-116
-        a.in = cin; a.scale = sx;
-        hip_check(launch_rows_scatter_adj(a, st), "scatter adjoint rows");
-        a.scale = sy2;
-        hip_check(launch_cols_fanin(2, a, st), "scatter adjoint columns");
-        a.in = a.K; a.out = cout; a.add = add; a.scale = sx;
-        hip_check(launch_rows(1, a, st), "scatter adjoint inverse");
+        a.in = cin; a.nterm = T + 1; a.scale = sx;
+        if (T >= 1) {
+          hip_check(launch_rows_scatter_adj(false, a, st), "scatter adjoint rows");
+          a.nterm = T; a.scale = sy2;
+          hip_check(launch_cols_fanin(2, a, st), "scatter adjoint columns");
+        }
+        a.in = cin; a.out = cout; a.add = add; a.nterm = T + 1; a.scale = sx;
+        hip_check(launch_rows_adj_out(false, a, st), "scatter adjoint combine");
       }
     }
   }
@@ -314,25 +319,28 @@ struct BornOp : fwix_op_s {
         for (int w0 = 0; w0 < d.NW; w0 += cw) {
           const int p0 = is * d.NW + w0, np = std::min(cw, d.NW - w0);
           KArgs a = born_args(z, p0, np);
+          const double pb = 8.0 * np * plane, mb = 8.0 * np * d.nx * d.ny;
+          const double wb = 8.0 * d.NX * eng.present_rows(z, p0, np);
+          const int T = par.taylor;
           a.dslow = dslow + (size_t)z * d.NW * d.ny * d.nx;
           a.bg = bg + (size_t)p0 * plane;
-          const double pb = 8.0 * np * plane, mb = 8.0 * np * d.nx * d.ny;
-          const int nt = par.taylor + 1, nr = par.nref;
-          a.scale = sx;
-          { PassTimer t("born_rows_ic", 2 * pb + mb, st); hip_check(launch_rows_ic_fwd(a, st), "born ic"); }
           a.wstride = (long long)np * plane;
-          a.scale = sy2;
-          { PassTimer t("born_cols_kfac", (1 + nt) * pb, st); hip_check(launch_cols_fanout(2, a, st), "born scatter columns"); }
+          if (T >= 1) {
+            a.scale = sx;
+            { PassTimer t("born_rows_ic", 2 * pb + mb, st); hip_check(launch_rows_ic_fwd(a, st), "born ic"); }
+            a.nterm = T; a.scale = sy2;
+            { PassTimer t("born_cols_kfac", (1 + T) * pb, st); hip_check(launch_cols_fanout(2, a, st), "born scatter columns"); }
+          }
           a.addin = du + (size_t)p0 * plane;
-          a.scale = sx;
-          { PassTimer t("born_rows_scatter", (nt + 2) * pb + mb, st); hip_check(launch_rows_scatter(true, a, st), "born scatter rows"); }
-          a.nterm = nr;
+          a.nterm = T + 1; a.scale = sx;
+          { PassTimer t("born_rows_scatter", (T + 3) * pb + 2 * mb, st); hip_check(launch_rows_scatter(true, a, st), "born scatter rows"); }
+          a.nterm = par.nref;
           a.scale = sy2;
-          { PassTimer t("cols_fanout", (1 + nr) * pb, st); hip_check(launch_cols_fanout(0, a, st), "born pspi columns"); }
+          { PassTimer t("cols_fanout", pb + wb, st); hip_check(launch_cols_fanout(0, a, st), "born pspi columns"); }
           a.out = du + (size_t)p0 * plane;
           a.add = 0;
           a.scale = sx;
-          { PassTimer t("rows_select", (1 + nr) * pb + pb / 8, st); hip_check(launch_rows_select(a, st), "born pspi rows"); }
+          { PassTimer t("rows_select", wb + pb + pb / 8, st); hip_check(launch_rows_select(a, st), "born pspi rows"); }
         }
       ++steps;
     }
@@ -367,23 +375,24 @@ struct BornOp : fwix_op_s {
               KArgs a = born_args(z, p0, np);
               a.wstride = (long long)np * plane;
               const double pb = 8.0 * np * plane, mb = 8.0 * np * d.nx * d.ny;
-              const int nt = par.taylor + 1, nr = par.nref;
-              // mu = P_z^H lam (PSPI adjoint, in place)
+              const double wb = 8.0 * d.NX * eng.present_rows(z, p0, np);
+              const int T = par.taylor;
+              // mu = P_z^H lam (PSPI adjoint, written in place) fused with S_z^H's first row pass
               a.in = lc; a.scale = sx;
-              { PassTimer t("rows_fanout", (1 + nr) * pb + pb / 8, st); hip_check(launch_rows_fanout(a, st), "born adjoint pspi rows"); }
-              a.nterm = nr; a.scale = sy2;
-              { PassTimer t("cols_fanin", (1 + nr) * pb, st); hip_check(launch_cols_fanin(1, a, st), "born adjoint pspi columns"); }
-              a.in = a.K; a.out = lc; a.add = 0; a.scale = sx;
-              { PassTimer t("rows_inv", 2 * pb, st); hip_check(launch_rows(par.taperx > 0 || par.tapery > 0 ? 3 : 1, a, st), "born adjoint pspi inverse"); }
+              { PassTimer t("rows_fanout", pb + pb / 8 + wb, st); hip_check(launch_rows_fanout(a, st), "born adjoint pspi rows"); }
+              a.nterm = par.nref; a.scale = sy2;
+              { PassTimer t("cols_fanin", wb + pb, st); hip_check(launch_cols_fanin(1, a, st), "born adjoint pspi columns"); }
+              a.out = lc; a.nterm = T + 1; a.scale = sx;
+              { PassTimer t("born_rows_inv_scatter_adj", (2 + T) * pb + mb, st); hip_check(launch_rows_scatter_adj(true, a, st), "born adjoint inverse + scatter rows"); }
+              if (T >= 1) {
+                a.nterm = T; a.scale = sy2;
+                { PassTimer t("born_cols_kfac_adj", (1 + T) * pb, st); hip_check(launch_cols_fanin(2, a, st), "born adjoint scatter columns"); }
+              }
               // ds_z += conj(U_z) S_z^H mu
-              a.in = lc; a.nterm = nt; a.scale = sx;
-              { PassTimer t("born_rows_scatter_adj", (1 + nt) * pb + mb, st); hip_check(launch_rows_scatter_adj(a, st), "born adjoint scatter rows"); }
-              a.scale = sy2;
-              { PassTimer t("born_cols_kfac_adj", (1 + nt) * pb, st); hip_check(launch_cols_fanin(2, a, st), "born adjoint scatter columns"); }
-              a.bg = bg + (size_t)p0 * plane;
+              a.in = lc; a.bg = bg + (size_t)p0 * plane;
               a.aux = dslow + (size_t)z * mz;
-              a.add = 1; a.scale = sx;
-              { PassTimer t("born_rows_ic_adj", 2 * pb + 2 * mb, st); hip_check(launch_rows_ic_adj(a, st), "born adjoint imaging"); }
+              a.add = 1; a.nterm = T + 1; a.scale = sx;
+              { PassTimer t("born_rows_adj_ic", 3 * pb + 3 * mb, st); hip_check(launch_rows_adj_out(true, a, st), "born adjoint imaging"); }
             }
         }
         ++steps;

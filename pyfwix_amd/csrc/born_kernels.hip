@@ -4,15 +4,19 @@
 // ForwardScattering, Scattering.cpp:43-52):
 //     dU_{z+1} = P_z [ dU_z + S_z(U_z * ds_z) ]
 //   rows_ic_fwd       v = U_z * ds_z (zero in the padding), F_x        -> K
-//   cols_fanout<2>    F_y, (k^2/w^2)^it, F_y^-1 for it = 0..T           -> W_it
-//   rows_scatter      F_x^-1 W_it, * c_it s~^(-1/2-it), sum, * -i w dz/2,
-//                     + dU_z, taper, F_x                                  -> K
+//   cols_fanout<2>    F_y, (k^2/w^2)^it, F_y^-1 for it = 1..T           -> W_it
+//   rows_scatter      order 0 from U_z * ds_z, orders >= 1 from F_x^-1 W_it,
+//                     * c_it s~^(-1/2-it), sum, * -i w dz/2, + dU_z, taper, F_x -> K
 //   cols_fanout<0>, rows_select: the PSPI step                           -> dU_{z+1}
 // Born adjoint step (ExtendedBorn.cpp:180-208, Scattering.cpp:54-61):
-//   mu = P_z^H lam (three PSPI passes), then
-//   rows_scatter_adj  * +i w dz/2, conj(c_it s~^(-1/2-it)), F_x          -> W_it
-//   cols_fanin<2>     F_y, (k^2/w^2)^it, sum, F_y^-1                     -> K
-//   rows_ic_adj       F_x^-1, ds_z += conj(U_z) * (.) on the model region
+//   rows_fanout, cols_fanin<1>     the first two PSPI adjoint passes     -> K
+//   rows_scatter_adj<fused>        mu = taper F_x^-1 K (written to lam),
+//                                  * +i w dz/2, conj(c_it s~^(-1/2-it)), F_x  -> W_it (it >= 1)
+//   cols_fanin<2>                  F_y, (k^2/w^2)^it, sum, F_y^-1        -> K
+//   rows_adj_out<ic>               F_x^-1 K + order-0 term,
+//                                  ds_z += conj(U_z) * (.) on the model region
+// Taylor order 0 is the identity in wavenumber and never goes through a
+// transform; only orders >= 1 use the column passes.
 #include "fwix_kernels.h"
 #include "fwix_launch.h"
 
@@ -35,7 +39,7 @@ struct BRowCfg {
 __device__ __forceinline__ void scatter_factors(const KArgs& a, int iw, int iy, int ix, float2& rsq, float2& inv) {
   const int yy = iy < a.ny ? iy : a.ny - 1;
   const int xx = ix < a.nx ? ix : a.nx - 1;
-  const float2 s = a.slow[((size_t)iw * a.ny + yy) * a.nx + xx];
+  const float2 s = a.slow[(unsigned)((iw * a.ny + yy) * a.nx + xx)];
   const float sr = s.x, si = s.y - a.eps_s * s.x;
   // principal sqrt of (sr, si)
   const float r = sqrtf(fmaf(sr, sr, si * si));
@@ -77,8 +81,8 @@ __global__ __launch_bounds__(kThreads) void k_rows_ic_fwd(KArgs a) {
     if (lr < nrows) {
       const int p = lr / a.NY, iy = lr - p * a.NY;
       if (iy < a.ny && i < a.nx) {
-        const float2 u = a.bg[(size_t)lr * N + i];
-        const float2 m = a.dslow[((size_t)plane_iw(a, p) * a.ny + iy) * a.nx + i];
+        const float2 u = a.bg[(unsigned)(lr * N + i)];
+        const float2 m = a.dslow[(unsigned)((plane_iw(a, p) * a.ny + iy) * a.nx + i)];
         v[e] = cmul(u, m);
       }
     }
@@ -89,11 +93,14 @@ __global__ __launch_bounds__(kThreads) void k_rows_ic_fwd(KArgs a) {
     int b, i;
     edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
     const int lr = rb + b;
-    if (lr < nrows) a.K[(size_t)lr * N + i] = cscale(v[e], a.scale);
+    if (lr < nrows) a.K[(unsigned)(lr * N + i)] = cscale(v[e], a.scale);
   }
 }
 
-// Sum of the Taylor terms in space:  y = -i (w dz / 2) sum_it f_it * F_x^-1(W_it).
+// Sum of the Taylor terms in space:
+//   y = -i (w dz / 2) [ c_0 s~^(-1/2) v + sum_{it>=1} c_it s~^(-1/2-it) F_x^-1(W_{it-1}) ]
+// where order 0 needs no transform: v is the scattering source itself
+// (BORN: U_z * ds_z on the model region, else the input rows).
 // BORN = true : K = scale * F_x( taper * (y + addin) )   (feeds the PSPI column pass)
 // BORN = false: out = (add ? out : 0) + y                (standalone Scatter forward)
 template <int N, bool BORN>
@@ -117,29 +124,39 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
     pw[e] = czero();
     inv[e] = czero();
     if (lr < nrows) {
-      const int p = lr / a.NY;
-      scatter_factors(a, plane_iw(a, p), lr - p * a.NY, i, pw[e], inv[e]);
+      const int p = lr / a.NY, iy = lr - p * a.NY;
+      scatter_factors(a, plane_iw(a, p), iy, i, pw[e], inv[e]);
+      float2 v;
+      if constexpr (BORN) {
+        v = czero();
+        if (iy < a.ny && i < a.nx)
+          v = cmul(a.bg[(unsigned)(lr * N + i)], a.dslow[(unsigned)((plane_iw(a, p) * a.ny + iy) * a.nx + i)]);
+      } else {
+        v = a.in[(unsigned)(lr * N + i)];
+      }
+      acc[e] = cmul(v, pw[e]);                 // c_0 = 1
+      pw[e] = cmul(pw[e], inv[e]);
     }
   }
-  for (int it = 0; it < a.nterm; ++it) {
-    const float2* W = a.W + (size_t)it * a.wstride;
+  for (int it = 1; it < a.nterm; ++it) {
+    const float2* W = a.W + (size_t)(it - 1) * a.wstride;
     float2 x[C::V];
 #pragma unroll
     for (int e = 0; e < C::V; ++e) {
       int b, i;
       edge_pos<N, C::NB, Fi::P::first(), false>(lane, e, b, i);
       const int lr = rb + b;
-      x[e] = lr < nrows ? W[(size_t)lr * N + i] : czero();
+      x[e] = lr < nrows ? W[(unsigned)(lr * N + i)] : czero();
     }
     Fi::run(x, wl, a.twx, lane);
-    const float c = taylor_coef(it);
+    const float c = taylor_coef(it) * a.scale;  // the inverse row transform's 1/sqrt(NX)
 #pragma unroll
     for (int e = 0; e < C::V; ++e) {
       acc[e] = cadd(acc[e], cscale(cmul(x[e], pw[e]), c));   // c_it s~^(-1/2-it) * term
       pw[e] = cmul(pw[e], inv[e]);
     }
   }
-  // acc *= -i * (w dz / 2) * scale     (propagator/src/scatter.cu:155-195)
+  // acc *= -i * (w dz / 2)     (propagator/src/scatter.cu:155-195)
 #pragma unroll
   for (int e = 0; e < C::V; ++e) {
     int b, i;
@@ -147,7 +164,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
     const int lr = rb + b;
     if (lr < nrows) {
       const int p = lr / a.NY;
-      const float h = 0.5f * a.w[plane_iw(a, p)] * a.dz * a.scale;
+      const float h = 0.5f * a.w[plane_iw(a, p)] * a.dz;
       acc[e] = make_float2(h * acc[e].y, -h * acc[e].x);
     }
   }
@@ -158,7 +175,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
       edge_pos<N, C::NB, Fi::P::last(), false>(lane, e, b, i);
       const int lr = rb + b;
       if (lr < nrows) {
-        const size_t idx = (size_t)lr * N + i;
+        const unsigned idx = lr * N + i;
         float2 y = acc[e];
         if (a.add) y = cadd(y, a.out[idx]);
         a.out[idx] = y;
@@ -172,7 +189,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
       edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
       const int lr = rb + b;
       if (lr < nrows) {
-        float2 y = cadd(acc[e], a.addin[(size_t)lr * N + i]);
+        float2 y = cadd(acc[e], a.addin[(unsigned)(lr * N + i)]);
         acc[e] = cscale(y, a.tapx[i] * a.tapy[lr % a.NY]);
       }
     }
@@ -182,40 +199,75 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
       int b, i;
       edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
       const int lr = rb + b;
-      if (lr < nrows) a.K[(size_t)lr * N + i] = cscale(acc[e], a.scale);
+      if (lr < nrows) a.K[(unsigned)(lr * N + i)] = cscale(acc[e], a.scale);
     }
   }
 }
 
-// W_it = scale * F_x( conj(f_it) * (+i w dz / 2) * in )   (Scatter adjoint, first half)
-template <int N>
+// Scatter adjoint, first half: u = +i (w dz / 2) * mu and, for Taylor orders
+// it >= 1, W_{it-1} = scale * F_x( conj(c_it s~^(-1/2-it)) * u ).
+// FUSED = true : mu = taper * scale * F_x^-1(K) is the end of the PSPI adjoint
+//                step; it is written to out (the adjoint wavefield) as well.
+// FUSED = false: mu = in (standalone Scatter adjoint).
+template <int N, bool FUSED>
 __global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
   using C = BRowCfg<N>;
   using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
+  using Fi = WaveFft<N, C::NB, false, true, PadLds<N>>;
+  static_assert(Fi::P::last() == F::P::first(), "register hand-off needs mirrored plans");
   __shared__ float2 lds[kWaves * C::LDSW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nrows = a.np * a.NY;
   const int rb = (blockIdx.x * kWaves + wave) * C::NB;
   if (rb >= nrows) return;
   float2* wl = lds + wave * C::LDSW;
-  float2 u[C::V], pw[C::V], inv[C::V];
+  float2 u[C::V];
+  if constexpr (FUSED) {
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      int b, i;
+      edge_pos<N, C::NB, Fi::P::first(), false>(lane, e, b, i);
+      const int lr = rb + b;
+      u[e] = lr < nrows ? a.K[(unsigned)(lr * N + i)] : czero();
+    }
+    Fi::run(u, wl, a.twx, lane);
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      int b, i;
+      edge_pos<N, C::NB, Fi::P::last(), false>(lane, e, b, i);
+      const int lr = rb + b;
+      if (lr < nrows) {
+        u[e] = cscale(u[e], a.scale * a.tapx[i] * a.tapy[lr % a.NY]);
+        a.out[(unsigned)(lr * N + i)] = u[e];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      int b, i;
+      edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
+      const int lr = rb + b;
+      u[e] = lr < nrows ? a.in[(unsigned)(lr * N + i)] : czero();
+    }
+  }
+  if (a.nterm < 2) return;
+  float2 pw[C::V], inv[C::V];
 #pragma unroll
   for (int e = 0; e < C::V; ++e) {
     int b, i;
     edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
     const int lr = rb + b;
-    u[e] = czero();
     pw[e] = czero();
     inv[e] = czero();
     if (lr < nrows) {
       const int p = lr / a.NY;
       const float h = 0.5f * a.w[plane_iw(a, p)] * a.dz;   // scatter.cu:197-238
-      const float2 d = a.in[(size_t)lr * N + i];
-      u[e] = make_float2(-h * d.y, h * d.x);
+      u[e] = make_float2(-h * u[e].y, h * u[e].x);
       scatter_factors(a, plane_iw(a, p), lr - p * a.NY, i, pw[e], inv[e]);
+      pw[e] = cmul(pw[e], inv[e]);                          // start at order 1
     }
   }
-  for (int it = 0; it < a.nterm; ++it) {
+  for (int it = 1; it < a.nterm; ++it) {
     float2 x[C::V];
     const float c = taylor_coef(it);
 #pragma unroll
@@ -224,23 +276,26 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
       pw[e] = cmul(pw[e], inv[e]);
     }
     F::run(x, wl, a.twx, lane);
-    float2* W = a.W + (size_t)it * a.wstride;
+    float2* W = a.W + (size_t)(it - 1) * a.wstride;
 #pragma unroll
     for (int e = 0; e < C::V; ++e) {
       int b, i;
       edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
       const int lr = rb + b;
-      if (lr < nrows) W[(size_t)lr * N + i] = cscale(x[e], a.scale);
+      if (lr < nrows) W[(unsigned)(lr * N + i)] = cscale(x[e], a.scale);
     }
   }
 }
 
-// ds[iw, y, x] (+)= conj(U) * scale * F_x^-1(K) on the model region.
-// Launched over planes of a single shot, so every model sample is owned by
-// exactly one lane: no atomics, deterministic (the reference's ic_adj uses
-// float atomicAdd, imaging_condition.cu:103-104).
-template <int N>
-__global__ __launch_bounds__(kThreads) void k_rows_ic_adj(KArgs a) {
+// Scatter adjoint, second half:
+//   x = conj(c_0 s~^(-1/2)) * (+i w dz / 2) * in  +  scale * F_x^-1(K)   (K only for taylor >= 1)
+// IC = true : ds[iw, y, x] (+)= conj(U) * x on the model region (into aux).
+//             Launched over planes of a single shot, so every model sample
+//             is owned by exactly one lane: no atomics, deterministic (the
+//             reference's ic_adj uses float atomicAdd, imaging_condition.cu:103-104).
+// IC = false: out = (add ? out : 0) + x  (standalone Scatter adjoint).
+template <int N, bool IC>
+__global__ __launch_bounds__(kThreads) void k_rows_adj_out(KArgs a) {
   using C = BRowCfg<N>;
   using Fi = WaveFft<N, C::NB, false, true, PadLds<N>>;
   __shared__ float2 lds[kWaves * C::LDSW];
@@ -249,14 +304,19 @@ __global__ __launch_bounds__(kThreads) void k_rows_ic_adj(KArgs a) {
   const int rb = (blockIdx.x * kWaves + wave) * C::NB;
   if (rb >= nrows) return;
   float2 x[C::V];
+  if (a.nterm > 1) {
 #pragma unroll
-  for (int e = 0; e < C::V; ++e) {
-    int b, i;
-    edge_pos<N, C::NB, Fi::P::first(), false>(lane, e, b, i);
-    const int lr = rb + b;
-    x[e] = lr < nrows ? a.K[(size_t)lr * N + i] : czero();
+    for (int e = 0; e < C::V; ++e) {
+      int b, i;
+      edge_pos<N, C::NB, Fi::P::first(), false>(lane, e, b, i);
+      const int lr = rb + b;
+      x[e] = lr < nrows ? a.K[(unsigned)(lr * N + i)] : czero();
+    }
+    Fi::run(x, lds + wave * C::LDSW, a.twx, lane);
+  } else {
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) x[e] = czero();
   }
-  Fi::run(x, lds + wave * C::LDSW, a.twx, lane);
 #pragma unroll
   for (int e = 0; e < C::V; ++e) {
     int b, i;
@@ -264,10 +324,22 @@ __global__ __launch_bounds__(kThreads) void k_rows_ic_adj(KArgs a) {
     const int lr = rb + b;
     if (lr < nrows) {
       const int p = lr / a.NY, iy = lr - p * a.NY;
-      if (iy < a.ny && i < a.nx) {
-        const size_t m = ((size_t)plane_iw(a, p) * a.ny + iy) * a.nx + i;
-        float2 y = cmulc(cscale(x[e], a.scale), a.bg[(size_t)lr * N + i]);
-        a.aux[m] = a.add ? cadd(a.aux[m], y) : y;
+      const int iw = plane_iw(a, p);
+      const unsigned idx = lr * N + i;
+      float2 rsq, inv;
+      scatter_factors(a, iw, iy, i, rsq, inv);
+      const float h = 0.5f * a.w[iw] * a.dz;
+      const float2 d = a.in[idx];
+      const float2 u = make_float2(-h * d.y, h * d.x);
+      const float2 y = cadd(cscale(x[e], a.scale), cmulc(u, rsq));
+      if constexpr (IC) {
+        if (iy < a.ny && i < a.nx) {
+          const unsigned m = (iw * a.ny + iy) * a.nx + i;
+          const float2 g = cmulc(y, a.bg[idx]);
+          a.aux[m] = a.add ? cadd(a.aux[m], g) : g;
+        }
+      } else {
+        a.out[idx] = a.add ? cadd(a.out[idx], y) : y;
       }
     }
   }
@@ -397,17 +469,19 @@ hipError_t launch_rows_scatter(bool born, const KArgs& a, hipStream_t st) {
     return hipGetLastError();
   });
 }
-hipError_t launch_rows_scatter_adj(const KArgs& a, hipStream_t st) {
+hipError_t launch_rows_scatter_adj(bool fused, const KArgs& a, hipStream_t st) {
   return bdispatch(a.NX, [&](auto nc) -> hipError_t {
     constexpr int N = decltype(nc)::value;
-    hipLaunchKernelGGL((k_rows_scatter_adj<N>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    if (fused) hipLaunchKernelGGL((k_rows_scatter_adj<N, true>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    else hipLaunchKernelGGL((k_rows_scatter_adj<N, false>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
     return hipGetLastError();
   });
 }
-hipError_t launch_rows_ic_adj(const KArgs& a, hipStream_t st) {
+hipError_t launch_rows_adj_out(bool ic, const KArgs& a, hipStream_t st) {
   return bdispatch(a.NX, [&](auto nc) -> hipError_t {
     constexpr int N = decltype(nc)::value;
-    hipLaunchKernelGGL((k_rows_ic_adj<N>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    if (ic) hipLaunchKernelGGL((k_rows_adj_out<N, true>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    else hipLaunchKernelGGL((k_rows_adj_out<N, false>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
     return hipGetLastError();
   });
 }

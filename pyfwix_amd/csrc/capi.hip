@@ -188,13 +188,13 @@ PwArgs Tables::pwargs() const {
 int auto_chunk(const fwix_dims& d, int nterm, int requested) {
   const int planes = d.NW * d.NS;
   if (requested > 0) return requested < planes ? requested : planes;
-  // enough planes that a column pass launches >= 512 workgroups (2 per CU),
+  // enough planes that a column pass launches >= 1024 workgroups (4 per CU),
   // at least what keeps the chunk's intermediates (spectrum + nterm fan-out
   // planes) within 128 MiB, and at most 2 GiB of scratch
   const double per_plane = (double)(1 + nterm) * d.NX * d.NY * 8.0;
   const int cw = fft_size_supported(d.NY) ? col_tile_width(d.NY) : 16;
   const int blocks_per_plane = (d.NX + cw - 1) / cw;
-  int c = (512 + blocks_per_plane - 1) / blocks_per_plane;
+  int c = (1024 + blocks_per_plane - 1) / blocks_per_plane;
   const int c_cache = (int)(128.0 * 1024 * 1024 / per_plane);
   if (c_cache > c) c = c_cache;
   const int c_max = (int)(2048.0 * 1024 * 1024 / per_plane);
@@ -235,8 +235,21 @@ void StepEngine::set_refs(const float2* sref_, const uint8_t* labels_) {
   hip_check(hipDeviceSynchronize(), "synchronize before presence masks");
   hip_check(launch_presence(labels, (uint32_t*)rowmask.p, (uint32_t*)planemask.p, d.NX, d.NY, nplanes, nullptr),
             "presence masks");
-  hip_check(hipDeviceSynchronize(), "presence masks");
+  DevBuf cnt;
+  cnt.alloc((size_t)nplanes * sizeof(int));
+  hip_check(launch_present_count((const uint32_t*)rowmask.p, (int*)cnt.p, d.NY, nplanes, params.nref, nullptr),
+            "presence counts");
+  present.assign((size_t)nplanes, 0);
+  hip_check(hipMemcpy(present.data(), cnt.p, (size_t)nplanes * sizeof(int), hipMemcpyDeviceToHost), "counts D2H");
   masks = true;
+}
+
+double StepEngine::present_rows(int iz, int p0, int np) const {
+  const fwix_dims& d = tables.dims;
+  if (!masks) return (double)params.nref * np * d.NY;
+  double c = 0;
+  for (int p = 0; p < np; ++p) c += present[(size_t)iz * d.NW + (p0 + p) % d.NW];
+  return c;
 }
 
 void StepEngine::check_ready(int iz) const {
@@ -279,7 +292,8 @@ void StepEngine::apply(bool adjoint, int iz, int add, const float2* in, float2* 
   const int nr = params.nref;
   for (int c0 = 0; c0 < planes; c0 += chunk) {
     const int np = (planes - c0) < chunk ? (planes - c0) : chunk;
-    const double pb = 8.0 * np * plane;  // bytes of one chunk-sized complex64 array
+    const double pb = 8.0 * np * plane;           // bytes of one chunk-sized complex64 array
+    const double wb = 8.0 * d.NX * present_rows(iz, c0, np);  // per-reference rows moved
     KArgs a = base(iz, c0, np);
     const float2* cin = in + (size_t)c0 * plane;
     float2* cout = out + (size_t)c0 * plane;
@@ -288,14 +302,14 @@ void StepEngine::apply(bool adjoint, int iz, int add, const float2* in, float2* 
       a.in = cin; a.out = a.K; a.add = 0; a.scale = sx;
       { PassTimer t("rows_fwd", 2 * pb, st); hip_check(launch_rows(taper ? 2 : 0, a, st), "rows pass"); }
       a.scale = sy2;
-      { PassTimer t("cols_fanout", (1 + nr) * pb, st); hip_check(launch_cols_fanout(adjoint ? 1 : 0, a, st), "column fan-out pass"); }
+      { PassTimer t("cols_fanout", pb + wb, st); hip_check(launch_cols_fanout(adjoint ? 1 : 0, a, st), "column fan-out pass"); }
       a.out = cout; a.add = add; a.scale = sx;
-      { PassTimer t("rows_select", (1 + nr + (add ? 1 : 0)) * pb + pb / 8, st); hip_check(launch_rows_select(a, st), "rows select pass"); }
+      { PassTimer t("rows_select", wb + (1 + (add ? 1 : 0)) * pb + pb / 8, st); hip_check(launch_rows_select(a, st), "rows select pass"); }
     } else {  // "masked inputs" shape: PSPI adjoint and NSPS forward
       a.in = cin; a.scale = sx;
-      { PassTimer t("rows_fanout", (1 + nr) * pb + pb / 8, st); hip_check(launch_rows_fanout(a, st), "rows fan-out pass"); }
+      { PassTimer t("rows_fanout", pb + pb / 8 + wb, st); hip_check(launch_rows_fanout(a, st), "rows fan-out pass"); }
       a.scale = sy2;
-      { PassTimer t("cols_fanin", (1 + nr) * pb, st); hip_check(launch_cols_fanin(adjoint ? 1 : 0, a, st), "column fan-in pass"); }
+      { PassTimer t("cols_fanin", wb + pb, st); hip_check(launch_cols_fanin(adjoint ? 1 : 0, a, st), "column fan-in pass"); }
       a.in = a.K; a.out = cout; a.add = add; a.scale = sx;
       { PassTimer t("rows_inv", (2 + (add ? 1 : 0)) * pb, st); hip_check(launch_rows(taper ? 3 : 1, a, st), "rows inverse pass"); }
     }

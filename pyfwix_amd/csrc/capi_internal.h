@@ -100,6 +100,9 @@ struct StepEngine {
   DevBuf K, W;
   DevBuf rowmask, planemask;  // reference presence per (depth, w, y) row and per (depth, w) plane
   bool masks = false;
+  std::vector<int> present;   // (nz, NW): present (row, reference) pairs per plane
+  // rows of the per-reference scratch a chunk actually reads or writes
+  double present_rows(int iz, int p0, int np) const;
   int chunk = 1;
   // borrow refs / labels for all depths and derive the presence masks
   void set_refs(const float2* sref, const uint8_t* labels);

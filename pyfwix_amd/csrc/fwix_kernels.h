@@ -13,13 +13,14 @@ constexpr int row_nb(int N) {
   const int base = size_has3(N) ? 768 : 512;
   return N >= base ? 1 : base / N;
 }
-// Columns per column-pass tile: 16 waves x CW/16 columns, chosen so each lane
-// holds V = CW*N/1024 values: 8 (2^a, N <= 512), 16 (1024) or 12 (3*2^a).
+// Column passes: 512-thread workgroups, 8 waves x CW/8 columns, with each
+// lane holding V = CW*N/512 values: 8 (2^a, N <= 512), 16 (1024) or 12 (3*2^a).
+// Two workgroups fit a CU, so one tile's loads overlap another's transforms.
 constexpr int col_cw(int N) {
-  const int cw = (size_has3(N) ? 12288 : 8192) / N;
-  return cw < 16 ? 16 : cw;
+  const int cw = (size_has3(N) ? 6144 : 4096) / N;
+  return cw < 8 ? 8 : cw;
 }
-constexpr int kColThreads = 1024;
+constexpr int kColThreads = 512;
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 

@@ -33,6 +33,8 @@ hipError_t launch_cols_fanout(int mode, const KArgs& a, hipStream_t st);
 hipError_t launch_cols_fanin(int mode, const KArgs& a, hipStream_t st);
 hipError_t launch_presence(const uint8_t* labels, uint32_t* rowmask, uint32_t* planemask, int NX, int NY,
                            long long nplanes, hipStream_t st);
+hipError_t launch_present_count(const uint32_t* rowmask, int* cnt, int NY, long long nplanes, int nref,
+                                hipStream_t st);
 
 // point-wise kernels (ops_kernels.hip)
 hipError_t launch_phase_shift(int mode, const float2* m, float2* d, const PwArgs& a, hipStream_t st);
@@ -44,8 +46,8 @@ hipError_t launch_zero(float2* d, size_t n, hipStream_t st);
 // Born / imaging / injection kernels (born_kernels.hip)
 hipError_t launch_rows_ic_fwd(const KArgs& a, hipStream_t st);
 hipError_t launch_rows_scatter(bool born, const KArgs& a, hipStream_t st);
-hipError_t launch_rows_scatter_adj(const KArgs& a, hipStream_t st);
-hipError_t launch_rows_ic_adj(const KArgs& a, hipStream_t st);
+hipError_t launch_rows_scatter_adj(bool fused, const KArgs& a, hipStream_t st);
+hipError_t launch_rows_adj_out(bool ic, const KArgs& a, hipStream_t st);
 hipError_t launch_image(const float2* U, const float2* R, float2* img, int NX, int NY, int NW, int NS, int nx,
                         int ny, int add, hipStream_t st);
 hipError_t launch_ic_fwd(const float2* img, float2* D, const float2* U, int NX, int NY, int NW, int NS, int nx, int ny,

@@ -44,6 +44,10 @@ struct ColCfg {
   static constexpr int TILE = N * CWP;
   static constexpr int PRIV = NB * padded(N);
   static constexpr int SMEM = TILE > WAVES * PRIV ? TILE : WAVES * PRIV;
+  // waves per SIMD the register budget is sized for: 4 (two workgroups per
+  // CU, 128 VGPRs) where the column transform and its terms fit in 128
+  // registers, else 2 (256 VGPRs, no spills)
+  static constexpr int WPE = (!size_has3(N) && N <= 256) ? 4 : 2;
   static_assert(NB >= 1 && V >= 4, "column tile too narrow");
 };
 
@@ -67,7 +71,7 @@ __global__ __launch_bounds__(kThreads) void k_rows(KArgs a) {
     const int lr = rb + b;
     float2 x = czero();
     if (lr < nrows) {
-      x = a.in[(size_t)lr * N + i];
+      x = a.in[(unsigned)(lr * N + i)];
       if (PRE_TAPER) x = cscale(x, a.tapx[i] * a.tapy[lr % a.NY]);
     }
     v[e] = x;
@@ -80,7 +84,7 @@ __global__ __launch_bounds__(kThreads) void k_rows(KArgs a) {
     edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
     const int lr = rb + b;
     if (lr < nrows) {
-      const size_t idx = (size_t)lr * N + i;
+      const unsigned idx = lr * N + i;
       float2 y = cscale(v[e], s);
       if (POST_TAPER) y = cscale(y, a.tapx[i] * a.tapy[lr % a.NY]);
       if (a.add) y = cadd(y, a.out[idx]);
@@ -110,9 +114,9 @@ __global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
     u[e] = czero();
     lab[e] = 255;
     if (lr < nrows) {
-      u[e] = a.in[(size_t)lr * N + i];
+      u[e] = a.in[(unsigned)(lr * N + i)];
       const int p = lr / a.NY, iy = lr - p * a.NY;
-      lab[e] = a.labels[((size_t)plane_iw(a, p) * a.NY + iy) * N + i];
+      lab[e] = a.labels[(unsigned)((plane_iw(a, p) * a.NY + iy) * N + i)];
     }
   }
   const float s = a.scale;
@@ -140,7 +144,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
       int b, i;
       edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
       const int lr = rb + b;
-      if (lr < nrows) W[(size_t)lr * N + i] = cscale(x[e], s);
+      if (lr < nrows) W[(unsigned)(lr * N + i)] = cscale(x[e], s);
     }
   }
 }
@@ -167,7 +171,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
     lab[e] = 255;
     if (lr < nrows) {
       const int p = lr / a.NY, iy = lr - p * a.NY;
-      lab[e] = a.labels[((size_t)plane_iw(a, p) * a.NY + iy) * N + i];
+      lab[e] = a.labels[(unsigned)((plane_iw(a, p) * a.NY + iy) * N + i)];
     }
   }
   // references present in this wave's rows (wave-uniform)
@@ -192,7 +196,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
       int b, i;
       edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
       const int lr = rb + b;
-      nxt[e] = lr < nrows ? W[(size_t)lr * N + i] : czero();
+      nxt[e] = lr < nrows ? W[(unsigned)(lr * N + i)] : czero();
     }
   };
   // absent references contribute nothing to these rows: skip their loads and transforms
@@ -218,7 +222,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
     edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
     const int lr = rb + b;
     if (lr < nrows) {
-      const size_t idx = (size_t)lr * N + i;
+      const unsigned idx = lr * N + i;
       float2 y = cscale(acc[e], s);
       if (a.add) y = cadd(y, a.out[idx]);
       a.out[idx] = y;
@@ -230,7 +234,8 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
 // Pointwise spectral multiplier of term r at (kx, ky) of frequency iw.
 // MULT 0 / 1: forward / adjoint phase shift of reference slowness r
 // (propagator/src/phase_shift.cu); MULT 2: Born scattering wavenumber factor
-// (|k|/w)^(2r) = (k^2/w^2)^r (propagator/src/scatter.cu:23-59).
+// (|k|/w)^(2it) = (k^2/w^2)^it of Taylor order it = r + 1 (scatter.cu:23-59);
+// order 0 is the identity and is applied in space by the row kernels.
 // The term's uniform part (reference slowness, real/complex branch) is set up
 // once per (plane, term) so the unrolled per-sample loops carry no divergent
 // or if-converted paths.
@@ -252,11 +257,10 @@ struct TermMult {
     }
   }
   template <bool REAL>
-  __device__ __forceinline__ float2 at(float kxv, float kyv) const {
-    const float k2 = fmaf(kxv, kxv, kyv * kyv);
-    if constexpr (MULT == 2) {
+  __device__ __forceinline__ float2 at(float k2) const {
+    if constexpr (MULT == 2) {  // term r carries Taylor order r + 1 (order 0 needs no transform)
       const float q = k2 * iw2;
-      float m = 1.f;
+      float m = q;
       for (int i = 0; i < r; ++i) m *= q;
       return make_float2(m, 0.f);
     } else if constexpr (REAL) {
@@ -267,31 +271,80 @@ struct TermMult {
   }
 };
 
+// k^2 = kx^2 + ky^2 at this wave's values after a column transform
+// (positions of the last stage of radix R); loop-invariant over the terms.
+template <int N, class C, int R>
+__device__ __forceinline__ void column_k2(float* k2v, const KArgs& a, int colw, int lane) {
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) {
+    int b, i;
+    edge_pos<N, C::NB, R, false>(lane, e, b, i);
+    const int col = colw + b;
+    const float kxv = a.kx[col < a.NX ? col : 0], kyv = a.ky[i];
+    k2v[e] = fmaf(kxv, kxv, kyv * kyv);
+  }
+}
+
 // Coalesced tile load / store: every thread first issues all of its global
 // loads into registers (PER = N*CW/THREADS of them), then writes LDS, so the
 // memory latency is paid once per tile rather than once per element.
+// One plane of a wavefield (or of a term buffer) as a buffer resource: the
+// base address sits in scalar registers and each element costs one 32-bit
+// byte offset.  Offsets at or beyond the plane's size read as zero and drop
+// stores (hardware range check), which replaces the per-element bounds and
+// presence branches: such elements get the offset kOff.
+constexpr unsigned kOff = 0x80000000u;
+struct PlaneBuf {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ PlaneBuf(const float2* base, unsigned bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)bytes, 0x00020000)) {}
+  __device__ __forceinline__ float2 load(unsigned off) const {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+  }
+  __device__ __forceinline__ void store(float2 v, unsigned off) const {
+    using U2 = __attribute__((ext_vector_type(2))) unsigned;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, (int)off, 0, 0);
+  }
+};
+// Byte offsets of this thread's tile elements: element k sits in row
+// threadIdx.x / CW + k * (THREADS / CW), column c0 + threadIdx.x % CW.
+template <class C>
+struct TileOffs {
+  static constexpr int PER = C::NROW * C::CW / C::THREADS;
+  static_assert(PER * C::THREADS == C::NROW * C::CW && C::THREADS % C::CW == 0, "tile must divide evenly");
+  unsigned o0, step;
+  __device__ __forceinline__ TileOffs(int NX, int c0) {
+    const int col = c0 + (int)threadIdx.x % C::CW;
+    o0 = col < NX ? (unsigned)(((int)threadIdx.x / C::CW) * NX + col) * 8u : kOff;
+    step = (unsigned)(C::THREADS / C::CW * NX) * 8u;
+  }
+  __device__ __forceinline__ unsigned at(int k) const { return o0 + (unsigned)k * step; }
+};
+__device__ __forceinline__ unsigned plane_bytes(const KArgs& a) { return (unsigned)(a.NX * a.NY) * 8u; }
+
+// presence masks of the tile rows, staged in LDS (all set when masks are off)
+template <class C>
+__device__ __forceinline__ void tile_row_masks(uint32_t* rml, const uint32_t* __restrict__ rm) {
+  for (int i = threadIdx.x; i < C::NROW; i += C::THREADS) rml[i] = rm ? rm[i] : 0xffffffffu;
+}
+template <class C>
+__device__ __forceinline__ uint32_t row_mask_of(const uint32_t* rml, int k) {
+  return rml[(int)threadIdx.x / C::CW + k * (C::THREADS / C::CW)];
+}
 template <class C>
 struct TileRegs {
-  static constexpr int PER = C::NROW * C::CW / C::THREADS;
-  static_assert(PER * C::THREADS == C::NROW * C::CW, "tile must divide evenly over the threads");
+  static constexpr int PER = TileOffs<C>::PER;
   float2 t[PER];
-  __device__ __forceinline__ void fetch(const float2* __restrict__ src, int NX, int c0) {
+  __device__ __forceinline__ void fetch(const PlaneBuf& b, const TileOffs<C>& o) {
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int e = threadIdx.x + k * C::THREADS;
-      const int i = e / C::CW, c = e % C::CW, col = c0 + c;
-      t[k] = col < NX ? src[(size_t)i * NX + col] : czero();
-    }
+    for (int k = 0; k < PER; ++k) t[k] = b.load(o.at(k));
   }
-  // rows whose mask bit is clear are known to be zero and are not loaded
-  __device__ __forceinline__ void fetch_masked(const float2* __restrict__ src, int NX, int c0,
-                                               const uint32_t* __restrict__ rm, uint32_t bit) {
+  // rows whose mask bit is clear are known to be zero and are not loaded;
+  // rmk holds the presence mask of every tile row
+  __device__ __forceinline__ void fetch_masked(const PlaneBuf& b, const TileOffs<C>& o, const uint32_t* rmk,
+                                               uint32_t bit) {
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int e = threadIdx.x + k * C::THREADS;
-      const int i = e / C::CW, c = e % C::CW, col = c0 + c;
-      t[k] = (col < NX && (rm[i] & bit)) ? src[(size_t)i * NX + col] : czero();
-    }
+    for (int k = 0; k < PER; ++k) t[k] = b.load((row_mask_of<C>(rmk, k) & bit) ? o.at(k) : kOff);
   }
   __device__ __forceinline__ void commit(float2* tile) const {
 #pragma unroll
@@ -302,26 +355,24 @@ struct TileRegs {
   }
 };
 template <class C>
-__device__ __forceinline__ void tile_load(float2* tile, const float2* __restrict__ src, int NX, int c0) {
+__device__ __forceinline__ void tile_load(float2* tile, const PlaneBuf& b, const TileOffs<C>& o) {
   TileRegs<C> r;
-  r.fetch(src, NX, c0);
+  r.fetch(b, o);
   r.commit(tile);
 }
 // store only the rows whose mask bit is set (rows that a later pass reads)
 template <class C>
-__device__ __forceinline__ void tile_store_masked(const float2* tile, float2* __restrict__ dst, int NX, int c0,
-                                                  const uint32_t* __restrict__ rm, uint32_t bit) {
-  constexpr int PER = C::NROW * C::CW / C::THREADS;
+__device__ __forceinline__ void tile_store_masked(const float2* tile, const PlaneBuf& b, const TileOffs<C>& o,
+                                                  const uint32_t* rmk, uint32_t bit) {
 #pragma unroll
-  for (int k = 0; k < PER; ++k) {
+  for (int k = 0; k < TileOffs<C>::PER; ++k) {
     const int e = threadIdx.x + k * C::THREADS;
-    const int i = e / C::CW, col = c0 + e % C::CW;
-    if (col < NX && (rm[i] & bit)) dst[(size_t)i * NX + col] = tile[i * C::CWP + e % C::CW];
+    b.store(tile[(e / C::CW) * C::CWP + e % C::CW], (row_mask_of<C>(rmk, k) & bit) ? o.at(k) : kOff);
   }
 }
 template <class C>
-__device__ __forceinline__ void tile_store(const float2* tile, float2* __restrict__ dst, int NX, int c0, int add) {
-  constexpr int PER = C::NROW * C::CW / C::THREADS;
+__device__ __forceinline__ void tile_store(const float2* tile, const PlaneBuf& b, const TileOffs<C>& o, int add) {
+  constexpr int PER = TileOffs<C>::PER;
   float2 t[PER];
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
@@ -330,18 +381,10 @@ __device__ __forceinline__ void tile_store(const float2* tile, float2* __restric
   }
   if (add) {
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int e = threadIdx.x + k * C::THREADS;
-      const int col = c0 + e % C::CW;
-      if (col < NX) t[k] = cadd(t[k], dst[(size_t)(e / C::CW) * NX + col]);
-    }
+    for (int k = 0; k < PER; ++k) t[k] = cadd(t[k], b.load(o.at(k)));
   }
 #pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int e = threadIdx.x + k * C::THREADS;
-    const int col = c0 + e % C::CW;
-    if (col < NX) dst[(size_t)(e / C::CW) * NX + col] = t[k];
-  }
+  for (int k = 0; k < PER; ++k) b.store(t[k], o.at(k));
 }
 // this wave's columns at the first-stage positions of radix R, from the tile
 template <int N, class C, int R>
@@ -365,7 +408,7 @@ __device__ __forceinline__ void tile_write(float2* tile, const float2* v, int wa
 
 // out = (add ? out : 0) + scale * F_y(in) on each plane (plain 2-D FFT column pass)
 template <int N, bool INV>
-__global__ __launch_bounds__(ColCfg<N>::THREADS) void k_cols(KArgs a) {
+__global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(ColCfg<N>::WPE))) void k_cols(KArgs a) {
   using C = ColCfg<N>;
   using F = WaveFft<N, C::NB, false, INV, PadLds<N>>;
   __shared__ float2 smem[C::SMEM];
@@ -373,7 +416,8 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_cols(KArgs a) {
   const int p = blockIdx.y, c0 = blockIdx.x * C::CW;
   const size_t pb = (size_t)p * a.NX * N;
   float2* priv = smem + wave * C::PRIV;
-  tile_load<C>(smem, a.in + pb, a.NX, c0);
+  const TileOffs<C> to(a.NX, c0);
+  tile_load<C>(smem, PlaneBuf(a.in + pb, plane_bytes(a)), to);
   __syncthreads();
   float2 v[C::V];
   tile_read<N, C, F::P::first()>(v, smem, wave, lane);
@@ -385,12 +429,12 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_cols(KArgs a) {
   __syncthreads();
   tile_write<N, C, F::P::last()>(smem, v, wave, lane);
   __syncthreads();
-  tile_store<C>(smem, a.out + pb, a.NX, c0, a.add);
+  tile_store<C>(smem, PlaneBuf(a.out + pb, plane_bytes(a)), to, a.add);
 }
 
 // cols_fanout: W_r = F_y^-1[M_r * F_y(K)] / NY for every term r (M_r: term_mult<MODE>).
 template <int N, int MODE>
-__global__ __launch_bounds__(ColCfg<N>::THREADS) void k_cols_fanout(KArgs a) {
+__global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(ColCfg<N>::WPE))) void k_cols_fanout(KArgs a) {
   using C = ColCfg<N>;
   using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
   using Fi = WaveFft<N, C::NB, false, true, PadLds<N>>;
@@ -400,7 +444,8 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_cols_fanout(KArgs a) {
   const int p = blockIdx.y, c0 = blockIdx.x * C::CW;
   const size_t pb = (size_t)p * a.NX * N;
   float2* priv = smem + wave * C::PRIV;
-  tile_load<C>(smem, a.K + pb, a.NX, c0);
+  const TileOffs<C> to(a.NX, c0);
+  tile_load<C>(smem, PlaneBuf(a.K + pb, plane_bytes(a)), to);
   __syncthreads();
   float2 spec[C::V];
   tile_read<N, C, F::P::first()>(spec, smem, wave, lane);
@@ -412,19 +457,17 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_cols_fanout(KArgs a) {
   const int colw = c0 + wave * C::NB;
   const uint32_t* rm = (MODE != 2 && a.rowmask) ? a.rowmask + iw * a.NY : nullptr;
   const uint32_t pm = (MODE != 2 && a.planemask) ? a.planemask[iw] : 0xffffffffu;
+  __shared__ uint32_t rmk[C::NROW];
+  tile_row_masks<C>(rmk, rm);
+  float k2v[C::V];
+  column_k2<N, C, F::P::last()>(k2v, a, colw, lane);
   for (int r = 0; r < a.nterm; ++r) {
     if (r < 32 && !((pm >> r) & 1u)) continue;  // reference absent from the plane: W_r is never read
     const TermMult<MODE> tm(a, r, iw, w2);
     float2 x[C::V];
     auto mult = [&](auto real) {
 #pragma unroll
-      for (int e = 0; e < C::V; ++e) {
-        int b, i;
-        edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
-        const int col = colw + b;
-        const float2 ph = tm.template at<decltype(real)::value>(a.kx[col < a.NX ? col : 0], a.ky[i]);
-        x[e] = cscale(cmul(spec[e], ph), s2);
-      }
+      for (int e = 0; e < C::V; ++e) x[e] = cscale(cmul(spec[e], tm.template at<decltype(real)::value>(k2v[e])), s2);
     };
     if (tm.real) mult(std::true_type{});
     else mult(std::false_type{});
@@ -432,15 +475,16 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_cols_fanout(KArgs a) {
     __syncthreads();
     tile_write<N, C, Fi::P::last()>(smem, x, wave, lane);
     __syncthreads();
-    if (rm) tile_store_masked<C>(smem, a.W + (size_t)r * a.wstride + pb, a.NX, c0, rm, r < 32 ? 1u << r : 0xffffffffu);
-    else tile_store<C>(smem, a.W + (size_t)r * a.wstride + pb, a.NX, c0, 0);
+    const PlaneBuf wb(a.W + (size_t)r * a.wstride + pb, plane_bytes(a));
+    if (rm) tile_store_masked<C>(smem, wb, to, rmk, r < 32 ? 1u << r : 0xffffffffu);
+    else tile_store<C>(smem, wb, to, 0);
     __syncthreads();
   }
 }
 
 // cols_fanin: K = F_y^-1[sum_r M_r * F_y(W_r)] / NY (M_r: term_mult<MODE>).
 template <int N, int MODE>
-__global__ __launch_bounds__(ColCfg<N>::THREADS) void k_cols_fanin(KArgs a) {
+__global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(ColCfg<N>::WPE))) void k_cols_fanin(KArgs a) {
   using C = ColCfg<N>;
   using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
   using Fi = WaveFft<N, C::NB, false, true, PadLds<N>>;
@@ -450,6 +494,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_cols_fanin(KArgs a) {
   const int p = blockIdx.y, c0 = blockIdx.x * C::CW;
   const size_t pb = (size_t)p * a.NX * N;
   float2* priv = smem + wave * C::PRIV;
+  const TileOffs<C> to(a.NX, c0);
   const int iw = plane_iw(a, p);
   const float w2 = a.w2[iw];
   const int colw = c0 + wave * C::NB;
@@ -461,19 +506,22 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_cols_fanin(KArgs a) {
   const uint32_t live = a.nterm >= 32 ? pm : (pm & ((1u << a.nterm) - 1u));
   const bool masked = rm != nullptr;
   const int rend = masked ? 32 : a.nterm;
-  auto fetch = [&](TileRegs<C>& tr, int r) {
-    if (masked) tr.fetch_masked(a.W + (size_t)r * a.wstride + pb, a.NX, c0, rm, 1u << r);
-    else tr.fetch(a.W + (size_t)r * a.wstride + pb, a.NX, c0);
-  };
-  TileRegs<C> next;
+  __shared__ uint32_t rmk[C::NROW];
+  tile_row_masks<C>(rmk, rm);
+  __syncthreads();
+  float k2v[C::V];
+  column_k2<N, C, F::P::last()>(k2v, a, colw, lane);
   int r = masked ? next_bit(live, -1) : 0;
-  if (r < rend) fetch(next, r);
   while (r < rend) {
-    next.commit(smem);
+    {
+      TileRegs<C> cur;
+      const PlaneBuf wb(a.W + (size_t)r * a.wstride + pb, plane_bytes(a));
+      if (masked) cur.fetch_masked(wb, to, rmk, 1u << r);
+      else cur.fetch(wb, to);
+      cur.commit(smem);
+    }
     __syncthreads();
-    // the next present term's loads are in flight during this term's transform
     const int rn = masked ? next_bit(live, r) : r + 1;
-    if (rn < rend) fetch(next, rn);
     float2 v[C::V];
     tile_read<N, C, F::P::first()>(v, smem, wave, lane);
     __syncthreads();
@@ -481,13 +529,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_cols_fanin(KArgs a) {
     const TermMult<MODE> tm(a, r, iw, w2);
     auto mult = [&](auto real) {
 #pragma unroll
-      for (int e = 0; e < C::V; ++e) {
-        int b, i;
-        edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
-        const int col = colw + b;
-        const float2 ph = tm.template at<decltype(real)::value>(a.kx[col < a.NX ? col : 0], a.ky[i]);
-        acc[e] = cadd(acc[e], cmul(v[e], ph));
-      }
+      for (int e = 0; e < C::V; ++e) acc[e] = cadd(acc[e], cmul(v[e], tm.template at<decltype(real)::value>(k2v[e])));
     };
     if (tm.real) mult(std::true_type{});
     else mult(std::false_type{});
@@ -501,7 +543,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) void k_cols_fanin(KArgs a) {
   __syncthreads();
   tile_write<N, C, Fi::P::last()>(smem, acc, wave, lane);
   __syncthreads();
-  tile_store<C>(smem, a.K + pb, a.NX, c0, 0);
+  tile_store<C>(smem, PlaneBuf(a.K + pb, plane_bytes(a)), to, 0);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -542,6 +584,23 @@ __global__ __launch_bounds__(kThreads) void k_planemask(const uint32_t* __restri
   for (int y = 0; y < NY; ++y) m |= rowmask[p * NY + y];
   planemask[p] = m;
 }
+// number of present (row, reference) pairs of every plane
+__global__ __launch_bounds__(kThreads) void k_present_count(const uint32_t* __restrict__ rowmask, int* cnt, int NY,
+                                                            long long nplanes, uint32_t refs) {
+  const long long p = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (p >= nplanes) return;
+  int c = 0;
+  for (int y = 0; y < NY; ++y) c += __builtin_popcount(rowmask[p * NY + y] & refs);
+  cnt[p] = c;
+}
+hipError_t launch_present_count(const uint32_t* rowmask, int* cnt, int NY, long long nplanes, int nref,
+                                hipStream_t st) {
+  const uint32_t refs = nref >= 32 ? 0xffffffffu : ((1u << nref) - 1u);
+  hipLaunchKernelGGL(k_present_count, dim3((unsigned)((nplanes + kThreads - 1) / kThreads)), dim3(kThreads), 0, st,
+                     rowmask, cnt, NY, nplanes, refs);
+  return hipGetLastError();
+}
+
 hipError_t launch_presence(const uint8_t* labels, uint32_t* rowmask, uint32_t* planemask, int NX, int NY,
                            long long nplanes, hipStream_t st) {
   const long long nrows = nplanes * NY;
