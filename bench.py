@@ -11,6 +11,16 @@ step (NX*NY*NW*NS points each) per second.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2] [--no-cpu]
 
 Prints one JSON line.
+
+Profiling helpers (the summaries under profiles/ are made this way):
+
+    --copy-probe       run a plain device copy of COPY_PROBE_BYTES first, so a
+                       counter run can scale FETCH_SIZE / WRITE_SIZE to bytes
+    --summarize-stats CSV OUT        kernel_stats.csv of `rocprofv3 --stats`
+                                     -> markdown table OUT
+    --summarize-traffic FETCH WRITE OUT   counter_collection.csv of two
+                                     `rocprofv3 --pmc` runs -> markdown OUT,
+                                     read back by the bench for `traffic`
 """
 from __future__ import annotations
 
@@ -26,30 +36,93 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+COPY_PROBE_BYTES = 1 << 28          # bytes read (and written) by the copy probe
+COPY_PROBE_TAG = "MulFunctor"       # the probe is y = 1.0 * x over float32
 
 
-def _device_inputs(w, dev):
-    """Upload a workload and expand the frequency-invariant arrays over NW on the GPU."""
-    import torch
-    nz, NW = w.nz, w.NW
-    s3 = torch.from_numpy(w.slow3.astype(np.float32)).to(dev)
-    slow = torch.empty((nz, NW, w.ny, w.nx), dtype=torch.complex64, device=dev)
-    slow.real.copy_(s3[:, None].expand(nz, NW, w.ny, w.nx))
-    slow.imag.zero_()
-    d3 = torch.from_numpy(w.dslow3.astype(np.float32)).to(dev)
-    ds = torch.empty_like(slow)
-    ds.real.copy_(d3[:, None].expand(nz, NW, w.ny, w.nx))
-    ds.imag.zero_()
-    sref = torch.from_numpy(w.sref1).to(dev).expand(nz, w.nref, NW).contiguous()
-    labels = torch.from_numpy(w.labels1).to(dev).expand(nz, NW, w.NY, w.NX).contiguous()
-    wav = torch.from_numpy(w.wavelet).to(dev)
-    del s3, d3
-    return slow, ds, sref, labels, wav
+def pass_kernel(name, w):
+    """Kernel (demangled name prefix) that a timed pass launches, for the c2-shaped PSPI/Born path."""
+    nx, ny = w.NX, w.NY
+    return {"cols_fanout": f"fwix::k_cols_fanout<{ny}, 0>", "cols_fanin": f"fwix::k_cols_fanin<{ny}, 1>",
+            "born_cols_kfac": f"fwix::k_cols_fanout<{ny}, 2>", "born_cols_kfac_adj": f"fwix::k_cols_fanin<{ny}, 2>",
+            "rows_select": f"fwix::k_rows_select<{nx}>", "rows_fanout": f"fwix::k_rows_fanout<{nx}>",
+            "born_rows_scatter": f"fwix::k_rows_scatter<{nx}, true>",
+            "born_rows_inv_scatter_adj": f"fwix::k_rows_scatter_adj<{nx}, true>",
+            "born_rows_adj_ic": f"fwix::k_rows_adj_out<{nx}, true>",
+            "born_rows_ic": f"fwix::k_rows_ic_fwd<{nx}>"}.get(name)
 
 
-def _dims(w):
-    from pyfwix_amd._lib import make_dims
-    return make_dims(w.NX, w.NY, w.NW, w.NS, w.nx, w.ny, 0.0, w.dx, 0.0, w.dy, w.ow, w.dw, 0.0, w.dz, w.nz)
+def _csv_rows(path):
+    import csv
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _short(name):
+    name = name.split("(")[0]
+    return name[5:] if name.startswith("void ") else name
+
+
+def summarize_stats(csv_path, out_path, command, top=16):
+    rows = _csv_rows(csv_path)
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    lines = ["# Kernel time, c2 Born forward + adjoint (512x512 padded, 64 frequencies, 200 depths)", "",
+             "Command (from the repository root):", "", "```", command, "```", "",
+             f"Total kernel time {tot / 1e6:.1f} ms over {sum(int(r['Calls']) for r in rows)} dispatches.", "",
+             "| kernel | calls | total ms | avg us | min us | max us | % |", "|---|---|---|---|---|---|---|"]
+    for r in rows[:top]:
+        lines.append(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.2f} | "
+                     f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} | "
+                     f"{float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.1f} |")
+    open(out_path, "w").write("\n".join(lines) + "\n")
+
+
+def _counter_means(csv_path, counter):
+    acc = {}
+    for r in _csv_rows(csv_path):
+        if r["Counter_Name"] != counter:
+            continue
+        k = _short(r["Kernel_Name"])
+        n, v = acc.get(k, (0, 0.0))
+        acc[k] = (n + 1, v + float(r["Counter_Value"]))
+    return {k: (n, v / n) for k, (n, v) in acc.items()}
+
+
+def summarize_traffic(fetch_csv, write_csv, out_path, command_fetch, command_write):
+    fetch = _counter_means(fetch_csv, "FETCH_SIZE")
+    write = _counter_means(write_csv, "WRITE_SIZE")
+    probe = [k for k in fetch if COPY_PROBE_TAG in k]
+    if not probe or not any(COPY_PROBE_TAG in k for k in write):
+        raise SystemExit(f"copy probe kernel ({COPY_PROBE_TAG}) not found in the counter runs")
+    kib = COPY_PROBE_BYTES / 1024.0
+    ff = kib / fetch[probe[0]][1]
+    wf = kib / write[[k for k in write if COPY_PROBE_TAG in k][0]][1]
+    lines = ["# HBM traffic per dispatch, c2 Born forward + adjoint", "",
+             "Two counter runs (from the repository root):", "", "```", command_fetch, command_write, "```", "",
+             f"Calibration: a plain copy of {COPY_PROBE_BYTES} bytes (`{COPY_PROBE_TAG}` kernel, 1.0 * x) reads "
+             f"{fetch[probe[0]][1]:.0f} KiB (FETCH_SIZE) and writes {write[[k for k in write if COPY_PROBE_TAG in k][0]][1]:.0f} KiB "
+             f"(WRITE_SIZE); scale factors fetch {ff:.4f}, write {wf:.4f} turn the counters into bytes.", "",
+             "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | traffic bytes (scaled) |",
+             "|---|---|---|---|---|"]
+    for k, (n, f) in sorted(fetch.items(), key=lambda kv: -kv[1][1] * kv[1][0]):
+        if k not in write or COPY_PROBE_TAG in k:
+            continue
+        wv = write[k][1]
+        lines.append(f"| `{k}` | {n} | {f:.1f} | {wv:.1f} | {(f * ff + wv * wf) * 1024:.0f} |")
+    open(out_path, "w").write("\n".join(lines) + "\n")
+
+
+def traffic_for(config, kernel):
+    """Scaled HBM bytes per dispatch of `kernel` from profiles/traffic_<config>.md, or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{config}.md")
+    if kernel is None or not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0] == f"`{kernel}`":
+            return float(cells[4])
+    return None
 
 
 def _step_alg_bytes(w, steps_bg, steps_fwd, steps_adj):
@@ -127,7 +200,22 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
+    ap.add_argument("--copy-probe", action="store_true", help="run the counter calibration copy first")
+    ap.add_argument("--pass-sample", type=int, default=8,
+                    help="time every n-th launch of each pass with HIP events inside the timed region")
+    ap.add_argument("--no-pass-timing", action="store_true",
+                    help="no event markers at all (counter-collection runs)")
+    ap.add_argument("--summarize-stats", nargs=2, metavar=("CSV", "OUT"))
+    ap.add_argument("--summarize-traffic", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT"))
+    ap.add_argument("--command", action="append", default=[], help="profiling command recorded in a summary")
     args = ap.parse_args()
+    if args.summarize_stats:
+        summarize_stats(*args.summarize_stats, command=" ".join(args.command))
+        return
+    if args.summarize_traffic:
+        cmds = args.command + ["", ""]
+        summarize_traffic(*args.summarize_traffic, command_fetch=cmds[0], command_write=cmds[1])
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); shots and "
               "frequency bands would shard across GPUs with one image all-reduce, which is not built here",
@@ -137,13 +225,20 @@ def main():
     import torch
     from pyfwix_amd import _lib
     from pyfwix_amd.ops import Born
-    from pyfwix_amd.workloads import CONFIGS, born_workload
+    from pyfwix_amd.workloads import CONFIGS, born_workload, device_inputs, workload_dims
 
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        x = torch.ones(COPY_PROBE_BYTES // 4, dtype=torch.float32, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(2):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        del x, y
     cfg = CONFIGS[args.config]
     w = born_workload(args.config, **cfg)
-    slow, ds, sref, labels, wav = _device_inputs(w, dev)
-    dims = _dims(w)
+    slow, ds, sref, labels, wav = device_inputs(w, dev)
+    dims = workload_dims(w)
     born = Born(dims, nref=w.nref, taperx=w.taper, tapery=w.taper, taylor=1, eps_scat=0.04, ckpt_every=1)
     born.set_refs(sref, labels)
     born.set_slow(slow)
@@ -163,28 +258,44 @@ def main():
         step()
     torch.cuda.synchronize()
     s0 = born.steps
-    _lib.timing_reset()
-    _lib.timing_enable(1 << 16)
+    timed = not args.no_pass_timing
+    if timed:
+        _lib.timing_reset()
+        _lib.timing_select(None, max(1, args.pass_sample))
+        _lib.timing_enable(1 << 15)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    _lib.timing_disable()
-    passes = _lib.timing_collect()
+    passes = {}
+    ev_us = None
+    if timed:
+        _lib.timing_disable()
+        passes = _lib.timing_collect()
+        _lib.timing_select(None, 1)
+        ev_us = _lib.timing_overhead(torch.cuda.current_stream().cuda_stream, 200)
     steps_exec = born.steps - s0
     per = steps_exec // args.steps
     value = steps_exec * w.points / wall / 1e9
     ms_step = wall / args.steps * 1e3
 
-    dom_name, dom = max(passes.items(), key=lambda kv: kv[1]["ms"])
-    achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.config}.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get(dom_name)
+    roof = None
+    if passes:
+        # estimated total time of each pass: mean recorded launch x all launches
+        est = {k: v["ms"] / v["launches"] * v["total_launches"] for k, v in passes.items()}
+        dom_name = max(est, key=est.get)
+        dom = passes[dom_name]
+        avg_us = dom["ms"] * 1e3 / dom["launches"]
+        per_launch = dom["bytes"] / dom["launches"]
+        achieved = per_launch / (avg_us * 1e-6) / 1e9
+        roof = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_for(args.config, pass_kernel(dom_name, w)),
+                "alg_bytes_per_launch": round(per_launch), "avg_us": round(avg_us, 2),
+                "launches_timed": dom["launches"], "launches": dom["total_launches"],
+                "empty_event_pair_us": round(ev_us, 2)}
     nz1 = w.nz - 1
     alg_total = _step_alg_bytes(w, nz1, nz1, nz1) * args.steps
     res = {
@@ -204,17 +315,17 @@ def main():
                                f"included", "nx": w.nx, "ny": w.ny, "nz": w.nz, "NX": w.NX, "NY": w.NY,
                    "nw": w.NW, "shots": w.NS, "nref": w.nref, "taper": w.taper, "receivers": nrec,
                    "depth_steps_per_step": per, "checkpointing": "all depths in HBM"},
-        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "launches": dom["launches"], "avg_us": round(dom["ms"] * 1e3 / dom["launches"], 2)},
+        "roofline": roof,
         "step_roofline": {"achieved": round(alg_total / wall / 1e9, 1), "unit": "GB/s",
                           "frac": round(alg_total / wall / 1e9 / HBM_PEAK_GBS, 4),
                           "bytes": "16P+labels per background step (+8P checkpoint), 24P+8m+labels per "
                                    "Born forward step, 24P+16m+labels per adjoint step"},
         "grad_s_per_shot": round(wall / args.steps / w.NS, 4),
-        "passes": {k: {"ms": round(v["ms"], 3), "launches": v["launches"],
+        "passes": {k: {"ms_est": round(v["ms"] / v["launches"] * v["total_launches"], 3),
+                       "launches": v["total_launches"], "timed": v["launches"],
+                       "avg_us": round(v["ms"] * 1e3 / v["launches"], 2),
                        "GBs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)} for k, v in
-                   sorted(passes.items(), key=lambda kv: -kv[1]["ms"])},
+                   sorted(passes.items(), key=lambda kv: -kv[1]["ms"] / kv[1]["launches"] * kv[1]["total_launches"])},
     }
     del born, grad, data
     if not args.no_plain:

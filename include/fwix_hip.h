@@ -201,6 +201,22 @@ int fwix_born_model(fwix_op op, int add, fwix_c64* data, fwix_stream s);
 int fwix_born_forward(fwix_op op, int add, const fwix_c64* dslow, fwix_c64* data, fwix_stream s);
 int fwix_born_adjoint(fwix_op op, int add, fwix_c64* dslow, const fwix_c64* data, fwix_stream s);
 int fwix_born_migrate(fwix_op op, int add, fwix_c64* image, const fwix_c64* data, fwix_stream s);
+/* Up-going reflection channel of the background modeling
+ * (propagator/src/Propagator.cpp:185-208, Reflect.h:20-36, reflect.cu:8-47):
+ * V = 0; for z = nz-1..0: V <- P_z V; V += refl_z * U_z on the model region;
+ * data (+)= Rec_z^T V.  With refl = fwix_reflect_coef(slow, density) and
+ * fwix_born_model's direct recording, this is Propagator::forward's data.
+ * It is linear in refl (nz, NW, ny, nx); reflect_adjoint is its adjoint
+ * (refl_z (+)= sum_s conj(U_z) mu_z with mu_z = Inj_z(data) + P_{z-1}^H mu_{z-1})
+ * and reflect_migrate the migration image of surface data,
+ * image_z (+)= sum_{w,s} conj(U_z) mu_z, image (nz, ny, nx). */
+int fwix_born_reflect_forward(fwix_op op, int add, const fwix_c64* refl, fwix_c64* data, fwix_stream s);
+int fwix_born_reflect_adjoint(fwix_op op, int add, fwix_c64* refl, const fwix_c64* data, fwix_stream s);
+int fwix_born_reflect_migrate(fwix_op op, int add, fwix_c64* image, const fwix_c64* data, fwix_stream s);
+/* Reflection coefficients r_z = (rho_{z+1} sqrt(s_z) - rho_z sqrt(s_{z+1})) /
+ * (rho_{z+1} sqrt(s_z) + rho_z sqrt(s_{z+1})), principal square roots, r_{nz-1} = 0
+ * (reflect.cu:42-45, Reflect.h:29-36); slow, den, refl (nz, NW, ny, nx). */
+int fwix_reflect_coef(const fwix_dims* d, const fwix_c64* slow, const fwix_c64* den, fwix_c64* refl, fwix_stream s);
 /* depth steps (PSPI applications of one wavefield) executed so far, -1 on a bad handle */
 long long fwix_born_steps(fwix_op op);
 
@@ -210,8 +226,17 @@ long long fwix_born_steps(fwix_op op);
  * of HIP events on its launch stream; fwix_timing_collect synchronizes them
  * and accumulates per pass: milliseconds, launches and compulsory HBM bytes
  * (what the pass must read and write).  fwix_timing_report writes
- * {"pass": {"ms": .., "launches": .., "bytes": ..}, ...} as JSON. */
+ * {"pass": {"ms": .., "launches": .., "bytes": .., "total_launches": ..}, ...}
+ * as JSON (ms, launches and bytes of the recorded launches; total_launches
+ * counts every launch of the pass since the last reset).
+ * fwix_timing_select restricts recording to one pass name (null: all
+ * passes) and to every `every`-th launch of each pass, so a timed region
+ * carries few event markers.  fwix_timing_overhead records `pairs` empty event
+ * pairs on a stream and returns their mean elapsed time in microseconds
+ * (the fixed cost inside every recorded pair). */
 int fwix_timing_enable(int on);
+int fwix_timing_select(const char* pass, int every);
+int fwix_timing_overhead(fwix_stream s, int pairs, float* us);
 int fwix_timing_collect(void);
 int fwix_timing_reset(void);
 int fwix_timing_report(char* buf, int len);

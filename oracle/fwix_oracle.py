@@ -32,6 +32,7 @@ __all__ = [
     "injection_weights", "injection_forward", "injection_adjoint",
     "downward_forward", "downward_adjoint", "upward_forward", "upward_adjoint",
     "model_background", "born_forward", "born_adjoint", "migrate",
+    "reflect_coef", "reflect_forward", "reflect_adjoint", "reflect_migrate",
     "SCATTER_COEF", "set_fft_workers",
 ]
 
@@ -515,3 +516,58 @@ def migrate(data, bg, rec, sref, labels, g: Grid, dz, eps=0.0, tx=0, ty=0):
         img[z] = ic_adjoint(lam, bg[z], g.nx, g.ny).sum(axis=0)
         lam = injection_forward(data, *rec, g, z, wfld=lam, add=True, shape=shape)
     return img
+
+
+def reflect_coef(slow, den):
+    """Reflection coefficient per depth (``propagator/src/reflect.cu:42-45``):
+
+    ``r_z = (rho_{z+1} sqrt(s_z) - rho_z sqrt(s_{z+1})) / (rho_{z+1} sqrt(s_z) + rho_z sqrt(s_{z+1}))``
+    with principal square roots, and r_{nz-1} = 0 because the last slice is
+    paired with itself (``Reflect.h:29-36``).  Arrays (nz, NW, ny, nx).
+    """
+    s0 = np.asarray(slow, np.complex128)
+    r0 = np.asarray(den, np.complex128)
+    s1 = np.concatenate([s0[1:], s0[-1:]])
+    r1 = np.concatenate([r0[1:], r0[-1:]])
+    a = r1 * np.sqrt(s0)
+    b = r0 * np.sqrt(s1)
+    return (a - b) / (a + b)
+
+
+def reflect_forward(refl, bg, rec, sref, labels, g: Grid, dz, eps=0.0, tx=0, ty=0):
+    """Up-going reflection recording of the background (``Propagator.cpp:185-208``).
+
+    V = 0; for z = nz-1..0: V <- P_z V; V += refl_z * U_z (``Reflect::cu_forward``);
+    d += Rec_z^T V.  Linear in ``refl`` (nz, NW, ny, nx); ``bg`` = [U_z].
+    """
+    nz = sref.shape[0]
+    V = np.zeros_like(bg[0])
+    data = np.zeros((len(rec[0]), g.NW), V.dtype)
+    for z in range(nz - 1, -1, -1):
+        if z < nz - 1:
+            V = pspi_forward(V, sref[z], labels[z], g, dz, eps, tx, ty)
+        V = ic_forward(refl[z], bg[z], d=V, add=True)
+        data = injection_adjoint(V, *rec, g, z, traces=data, add=True)
+    return data
+
+
+def reflect_adjoint(data, bg, rec, sref, labels, g: Grid, dz, eps=0.0, tx=0, ty=0):
+    """Adjoint of ``reflect_forward`` in the reflectivity: mu_0 = Inj_0(d),
+    mu_z = Inj_z(d) + P_{z-1}^adj mu_{z-1}; refl_z = sum_s conj(U_z) mu_z."""
+    nz = sref.shape[0]
+    mu = np.zeros_like(bg[0])
+    shape = mu.shape
+    out = np.zeros((nz, g.NW, g.ny, g.nx), mu.dtype)
+    for z in range(nz):
+        if z > 0:
+            mu = pspi_adjoint(mu, sref[z - 1], labels[z - 1], g, dz, eps, tx, ty)
+        mu = injection_forward(data, *rec, g, z, wfld=mu, add=True, shape=shape)
+        out[z] = ic_adjoint(mu, bg[z], g.nx, g.ny)
+    return out
+
+
+def reflect_migrate(data, bg, rec, sref, labels, g: Grid, dz, eps=0.0, tx=0, ty=0):
+    """Migration image of surface data, ``I_z = sum_w sum_s conj(U_z) * mu_z``:
+    the source wavefield U correlated with the receiver wavefield mu
+    extrapolated downward (the frequency sum of ``reflect_adjoint``)."""
+    return reflect_adjoint(data, bg, rec, sref, labels, g, dz, eps, tx, ty).sum(axis=1)

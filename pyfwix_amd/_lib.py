@@ -97,7 +97,13 @@ _SIGS = {
     "fwix_born_forward": [_P, _I, _P, _P, _P],
     "fwix_born_adjoint": [_P, _I, _P, _P, _P],
     "fwix_born_migrate": [_P, _I, _P, _P, _P],
+    "fwix_born_reflect_forward": [_P, _I, _P, _P, _P],
+    "fwix_born_reflect_adjoint": [_P, _I, _P, _P, _P],
+    "fwix_born_reflect_migrate": [_P, _I, _P, _P, _P],
+    "fwix_reflect_coef": [_D, _P, _P, _P, _P],
     "fwix_timing_enable": [_I],
+    "fwix_timing_select": [C.c_char_p, _I],
+    "fwix_timing_overhead": [C.c_void_p, _I, C.POINTER(C.c_float)],
     "fwix_timing_collect": [],
     "fwix_timing_reset": [],
     "fwix_timing_report": [C.c_char_p, _I],
@@ -188,3 +194,15 @@ def timing_collect() -> dict:
 
 def timing_reset() -> None:
     call("fwix_timing_reset")
+
+
+def timing_select(pass_name: str | None = None, every: int = 1) -> None:
+    """Record only ``pass_name`` (None: every pass), every ``every``-th launch of it."""
+    call("fwix_timing_select", None if pass_name is None else pass_name.encode(), int(every))
+
+
+def timing_overhead(stream: int, pairs: int = 200) -> float:
+    """Mean elapsed microseconds of an empty event pair on ``stream``."""
+    us = C.c_float()
+    call("fwix_timing_overhead", C.c_void_p(stream), int(pairs), C.byref(us))
+    return float(us.value)

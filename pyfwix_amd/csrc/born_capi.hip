@@ -400,6 +400,59 @@ struct BornOp : fwix_op_s {
       rec.forward(z, data, l, st);  // adjoint source (ExtendedBorn.cpp:195-196)
     }
   }
+
+  // Up-going reflection recording of the background (Propagator.cpp:185-208):
+  // V = 0; for z = nz-1..0: V <- P_z V; V += refl_z * U_z (model region);
+  // data += Rec_z^T V.  Linear in refl (nz, NW, ny, nx).
+  void reflect_forward(int add, const float2* refl, float2* data, hipStream_t st) {
+    const size_t n = P(), mz = (size_t)d.NW * d.ny * d.nx;
+    if (!add) hip_check(launch_zero(data, (size_t)nrec * d.NW, st), "zero data");
+    float2* v = (float2*)dU.p;
+    hip_check(launch_zero(v, n, st), "zero V");
+    for (int z = d.nz - 1; z >= 0; --z) {
+      if (z + 1 < d.nz) {  // V is still zero at the bottom
+        eng.apply(false, z, 0, v, v, st);
+        ++steps;
+      }
+      const float2* bg = background(z, st);
+      {
+        PassTimer t("reflect_ic", 24.0 * (double)n + 8.0 * mz, st);
+        hip_check(launch_ic_fwd(refl + (size_t)z * mz, v, bg, d.NX, d.NY, d.NW, d.NS, d.nx, d.ny, 1, st), "reflect");
+      }
+      rec.adjoint(z, data, v, st);
+    }
+  }
+
+  // Adjoint of reflect_forward in refl: mu_0 = Inj_0(data),
+  // mu_z = Inj_z(data) + P_{z-1}^H mu_{z-1};  refl_z (+)= sum_s conj(U_z) mu_z.
+  // With image != null: image_z (+)= sum_{w,s} conj(U_z) mu_z, the migration
+  // image of surface data (source wavefield U, receiver wavefield mu).
+  void reflect_adjoint(int add, float2* refl, float2* image, const float2* data, hipStream_t st) {
+    const size_t n = P(), mz = (size_t)d.NW * d.ny * d.nx;
+    if (!add) {
+      if (refl) hip_check(launch_zero(refl, mz * d.nz, st), "zero model");
+      if (image) hip_check(launch_zero(image, (size_t)d.ny * d.nx * d.nz, st), "zero image");
+    }
+    float2* mu = (float2*)lam.p;
+    hip_check(launch_zero(mu, n, st), "zero mu");
+    for (int z = 0; z < d.nz; ++z) {
+      if (z > 0) {
+        eng.apply(true, z - 1, 0, mu, mu, st);
+        ++steps;
+      }
+      rec.forward(z, data, mu, st);
+      const float2* bg = background(z, st);
+      if (image) {
+        PassTimer t("reflect_image", 16.0 * (double)n + 16.0 * d.nx * d.ny, st);
+        hip_check(launch_image(bg, mu, image + (size_t)z * d.ny * d.nx, d.NX, d.NY, d.NW, d.NS, d.nx, d.ny, 1, st),
+                  "reflect image");
+      } else {
+        PassTimer t("reflect_ic_adj", 16.0 * (double)d.nx * d.ny * d.NW * d.NS + 16.0 * mz, st);
+        hip_check(launch_ic_adj(refl + (size_t)z * mz, mu, bg, d.NX, d.NY, d.NW, d.NS, d.nx, d.ny, 1, st),
+                  "reflect adjoint");
+      }
+    }
+  }
 };
 
 }  // namespace
@@ -636,6 +689,33 @@ int fwix_born_migrate(fwix_op op, int add, fwix_c64* image, const fwix_c64* data
   return guard([&] {
     if (!image) throw Error("null image");
     as<BornOp>(op, KIND_BORN)->adjoint(add, nullptr, (float2*)image, (const float2*)data, S(s));
+  });
+}
+int fwix_born_reflect_forward(fwix_op op, int add, const fwix_c64* refl, fwix_c64* data, fwix_stream s) {
+  return guard([&] {
+    if (!refl || !data) throw Error("null reflectivity or data");
+    as<BornOp>(op, KIND_BORN)->reflect_forward(add, (const float2*)refl, (float2*)data, S(s));
+  });
+}
+int fwix_born_reflect_adjoint(fwix_op op, int add, fwix_c64* refl, const fwix_c64* data, fwix_stream s) {
+  return guard([&] {
+    if (!refl || !data) throw Error("null reflectivity or data");
+    as<BornOp>(op, KIND_BORN)->reflect_adjoint(add, (float2*)refl, nullptr, (const float2*)data, S(s));
+  });
+}
+int fwix_born_reflect_migrate(fwix_op op, int add, fwix_c64* image, const fwix_c64* data, fwix_stream s) {
+  return guard([&] {
+    if (!image || !data) throw Error("null image or data");
+    as<BornOp>(op, KIND_BORN)->reflect_adjoint(add, nullptr, (float2*)image, (const float2*)data, S(s));
+  });
+}
+int fwix_reflect_coef(const fwix_dims* d, const fwix_c64* slow, const fwix_c64* den, fwix_c64* refl, fwix_stream s) {
+  return guard([&] {
+    validate_dims(d, true);
+    if (!slow || !den || !refl) throw Error("null slowness, density or output");
+    const size_t slice = (size_t)d->NW * d->ny * d->nx;
+    hip_check(launch_reflect_coef((const float2*)slow, (const float2*)den, (float2*)refl, d->nz, slice, S(s)),
+              "reflection coefficients");
   });
 }
 long long fwix_born_steps(fwix_op op) {

@@ -345,6 +345,31 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj_out(KArgs a) {
   }
 }
 
+// ------------------------------------------------------------------ reflection coefficients
+// principal square root of a complex number: sqrt((|z|+x)/2) + i sign(y) sqrt((|z|-x)/2)
+__device__ __forceinline__ float2 csqrt_principal(float2 z) {
+  const float m = hypotf(z.x, z.y);
+  const float re = sqrtf(fmaxf(0.5f * (m + z.x), 0.f));
+  const float im = copysignf(sqrtf(fmaxf(0.5f * (m - z.x), 0.f)), z.y);
+  return make_float2(re, im);
+}
+__device__ __forceinline__ float2 cdiv(float2 a, float2 b) {
+  const float q = 1.f / fmaf(b.x, b.x, b.y * b.y);
+  return make_float2(fmaf(a.x, b.x, a.y * b.y) * q, fmaf(a.y, b.x, -a.x * b.y) * q);
+}
+// r_z = (rho_{z+1} sqrt(s_z) - rho_z sqrt(s_{z+1})) / (rho_{z+1} sqrt(s_z) + rho_z sqrt(s_{z+1})),
+// r_{nz-1} = 0 (the last slice is paired with itself); arrays (nz, NW, ny, nx)
+__global__ __launch_bounds__(kThreads) void k_reflect_coef(const float2* __restrict__ slow, const float2* __restrict__ den,
+                                                           float2* __restrict__ r, int nz, size_t slice) {
+  const size_t n = (size_t)nz * slice;
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+    if (i + slice >= n) { r[i] = czero(); continue; }
+    const float2 a = cmul(den[i + slice], csqrt_principal(slow[i]));
+    const float2 b = cmul(den[i], csqrt_principal(slow[i + slice]));
+    r[i] = cdiv(csub(a, b), cadd(a, b));
+  }
+}
+
 // ------------------------------------------------------------------ point-wise Born helpers
 // img (+)= sum_{s, w} conj(U) * R over the model region: one thread per (y, x),
 // fixed summation order (the reduction of the migration image).
@@ -484,6 +509,11 @@ hipError_t launch_rows_adj_out(bool ic, const KArgs& a, hipStream_t st) {
     else hipLaunchKernelGGL((k_rows_adj_out<N, false>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
     return hipGetLastError();
   });
+}
+hipError_t launch_reflect_coef(const float2* slow, const float2* den, float2* r, int nz, size_t slice, hipStream_t st) {
+  hipLaunchKernelGGL(k_reflect_coef, bflat((long long)nz * (long long)slice), dim3(kThreads), 0, st, slow, den, r, nz,
+                     slice);
+  return hipGetLastError();
 }
 hipError_t launch_image(const float2* U, const float2* R, float2* img, int NX, int NY, int NW, int NS, int nx,
                         int ny, int add, hipStream_t st) {

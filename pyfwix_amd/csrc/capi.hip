@@ -33,10 +33,13 @@ struct EvRec {
 };
 struct PassStat {
   double ms = 0, bytes = 0;
-  long long n = 0;
+  long long n = 0;  // recorded launches
 };
 std::mutex g_tmu;
 bool g_timing = false;
+std::string g_only;        // time only this pass ("" = every pass)
+long long g_every = 1;     // ... and only every g_every-th launch of each pass
+std::map<std::string, long long> g_seen;  // launches per pass while enabled
 std::vector<EvRec> g_pending;
 std::vector<hipEvent_t> g_pool;
 std::map<std::string, PassStat> g_stat;
@@ -56,6 +59,9 @@ hipEvent_t take_event() {
 PassTimer::PassTimer(const char* name, double bytes, hipStream_t s) {
   if (!g_timing) return;
   std::lock_guard<std::mutex> g(g_tmu);
+  const long long k = g_seen[name]++;
+  if (!g_only.empty() && g_only != name) return;
+  if (k % g_every) return;
   EvRec r{name, bytes, take_event(), take_event()};
   hip_check(hipEventRecord(r.a, s), "hipEventRecord");
   slot = (int)g_pending.size();
@@ -635,6 +641,35 @@ int fwix_timing_enable(int on) {
     g_timing = on > 0;
   });
 }
+int fwix_timing_select(const char* pass, int every) {
+  return guard([&] {
+    if (every < 1) throw Error("every must be >= 1");
+    std::lock_guard<std::mutex> g(g_tmu);
+    g_only = pass ? pass : "";
+    g_every = every;
+  });
+}
+int fwix_timing_overhead(fwix_stream s, int pairs, float* us) {
+  return guard([&] {
+    if (pairs < 1 || !us) throw Error("pairs must be >= 1 and us non-null");
+    hipStream_t st = static_cast<hipStream_t>(s);
+    std::vector<hipEvent_t> ev(2 * (size_t)pairs);
+    for (auto& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+    for (int i = 0; i < pairs; ++i) {
+      hip_check(hipEventRecord(ev[2 * i], st), "hipEventRecord");
+      hip_check(hipEventRecord(ev[2 * i + 1], st), "hipEventRecord");
+    }
+    double tot = 0;
+    for (int i = 0; i < pairs; ++i) {
+      hip_check(hipEventSynchronize(ev[2 * i + 1]), "hipEventSynchronize");
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]), "hipEventElapsedTime");
+      tot += ms;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    *us = (float)(tot * 1e3 / pairs);
+  });
+}
 int fwix_timing_collect(void) {
   return guard([&] {
     std::lock_guard<std::mutex> g(g_tmu);
@@ -656,6 +691,7 @@ int fwix_timing_reset(void) {
   return guard([&] {
     std::lock_guard<std::mutex> g(g_tmu);
     g_stat.clear();
+    g_seen.clear();
   });
 }
 int fwix_timing_report(char* buf, int len) {
@@ -664,9 +700,12 @@ int fwix_timing_report(char* buf, int len) {
     std::string out = "{";
     bool first = true;
     for (auto& kv : g_stat) {
-      char tmp[256];
-      std::snprintf(tmp, sizeof(tmp), "%s\"%s\": {\"ms\": %.6f, \"launches\": %lld, \"bytes\": %.0f}",
-                    first ? "" : ", ", kv.first.c_str(), kv.second.ms, kv.second.n, kv.second.bytes);
+      char tmp[320];
+      auto it = g_seen.find(kv.first);
+      const long long total = it == g_seen.end() ? kv.second.n : it->second;
+      std::snprintf(tmp, sizeof(tmp),
+                    "%s\"%s\": {\"ms\": %.6f, \"launches\": %lld, \"bytes\": %.0f, \"total_launches\": %lld}",
+                    first ? "" : ", ", kv.first.c_str(), kv.second.ms, kv.second.n, kv.second.bytes, total);
       out += tmp;
       first = false;
     }

@@ -48,6 +48,7 @@ hipError_t launch_rows_ic_fwd(const KArgs& a, hipStream_t st);
 hipError_t launch_rows_scatter(bool born, const KArgs& a, hipStream_t st);
 hipError_t launch_rows_scatter_adj(bool fused, const KArgs& a, hipStream_t st);
 hipError_t launch_rows_adj_out(bool ic, const KArgs& a, hipStream_t st);
+hipError_t launch_reflect_coef(const float2* slow, const float2* den, float2* r, int nz, size_t slice, hipStream_t st);
 hipError_t launch_image(const float2* U, const float2* R, float2* img, int NX, int NY, int NW, int NS, int nx,
                         int ny, int add, hipStream_t st);
 hipError_t launch_ic_fwd(const float2* img, float2* D, const float2* U, int NX, int NY, int NW, int NS, int nx, int ny,

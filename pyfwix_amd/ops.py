@@ -17,7 +17,7 @@ from . import _lib
 from ._lib import FWIX_NSPS, FWIX_PSPI, FWIX_PSPI_PLAIN, Handle, call, create, default_params, make_dims
 
 __all__ = ["FFT2d", "PhaseShift", "Selector", "Taper", "OneStep", "Injection", "Scatter",
-           "ImagingCondition", "OneWay", "Born", "ptr", "stream",
+           "ImagingCondition", "OneWay", "Born", "reflect_coef", "ptr", "stream",
            "make_dims", "default_params", "FWIX_PSPI", "FWIX_NSPS", "FWIX_PSPI_PLAIN"]
 
 
@@ -367,6 +367,40 @@ class Born:
         call("fwix_born_migrate", self._h.h, int(bool(add)), ptr(image), ptr(data), stream())
         return image
 
+    def reflect_forward(self, add, refl, data):
+        """data (+)= up-going recording of refl * U_z (``Propagator.cpp:185-208``)."""
+        d = self.dims
+        _check(refl, (d.nz, d.NW, d.ny, d.nx), name="reflectivity")
+        _check(data, (self.nrec, d.NW), name="data")
+        call("fwix_born_reflect_forward", self._h.h, int(bool(add)), ptr(refl), ptr(data), stream())
+        return data
+
+    def reflect_adjoint(self, add, refl, data):
+        d = self.dims
+        _check(refl, (d.nz, d.NW, d.ny, d.nx), name="reflectivity")
+        _check(data, (self.nrec, d.NW), name="data")
+        call("fwix_born_reflect_adjoint", self._h.h, int(bool(add)), ptr(refl), ptr(data), stream())
+        return refl
+
+    def reflect_migrate(self, add, image, data):
+        """image_z (+)= sum_{w,s} conj(U_z) mu_z: migration of surface data."""
+        d = self.dims
+        _check(image, (d.nz, d.ny, d.nx), name="image")
+        _check(data, (self.nrec, d.NW), name="data")
+        call("fwix_born_reflect_migrate", self._h.h, int(bool(add)), ptr(image), ptr(data), stream())
+        return image
+
     @property
     def steps(self) -> int:
         return int(_lib.lib.fwix_born_steps(self._h.h))
+
+
+def reflect_coef(dims, slow: torch.Tensor, den: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+    """Reflection coefficients of a (slowness, density) model (``reflect.cu:42-45``), (nz, NW, ny, nx)."""
+    shape = (dims.nz, dims.NW, dims.ny, dims.nx)
+    _check(slow, shape, name="slowness")
+    _check(den, shape, name="density")
+    out = torch.empty_like(slow) if out is None else out
+    _check(out, shape, name="reflectivity")
+    call("fwix_reflect_coef", C.byref(dims), ptr(slow), ptr(den), ptr(out), stream())
+    return out

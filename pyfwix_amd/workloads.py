@@ -65,13 +65,18 @@ def _velocity(nx, ny, nz, dx, dy, dz):
 
 def born_workload(name: str, nx: int, nz: int, nw: int, fmin: float, fmax: float, nref: int = 4,
                   taper: int = 20, d: float = 10.0, seed: int = 2, sigma: float = 3.0,
-                  amp: float = 0.05, reflectors: bool = False) -> Workload:
+                  amp: float = 0.05, reflectors: bool = False, rec_depth: float = 0.0) -> Workload:
     """Born forward + migration workload on an nx^2 x nz grid padded to 2nx.
 
-    One Ricker 15 Hz source at the surface centre, receivers on every surface
-    node.  The perturbation is a seeded Gaussian random field smoothed with
-    ``sigma`` cells, ``amp`` * s of amplitude, or (``reflectors``) three flat
-    +-5 % slowness reflectors at 1/4, 1/2, 3/4 of the depth.
+    One Ricker 15 Hz source at the surface centre, receivers on every node of
+    the plane z = ``rec_depth`` (the surface by default).  The perturbation is
+    a seeded Gaussian random field smoothed with ``sigma`` cells, ``amp`` * s
+    of amplitude, or (``reflectors``) three flat +-5 % slowness reflectors at
+    1/4, 1/2, 3/4 of the depth.
+
+    The down-scattering Born channel records the scattered field where it
+    arrives, below the scatterers: its data on the surface are zero, so a
+    transmission workload puts the receivers on the deepest plane.
     """
     from scipy.ndimage import gaussian_filter
     rng = np.random.default_rng(seed)
@@ -90,21 +95,108 @@ def born_workload(name: str, nx: int, nz: int, nw: int, fmin: float, fmax: float
            np.array([0.0], np.float32), np.array([0], np.int32))
     gx, gy = np.meshgrid(np.arange(nx) * d, np.arange(ny) * d)
     rec = (gx.ravel().astype(np.float32), gy.ravel().astype(np.float32),
-           np.zeros(nx * ny, np.float32), np.zeros(nx * ny, np.int32))
+           np.full(nx * ny, rec_depth, np.float32), np.zeros(nx * ny, np.int32))
+    extra = {}
     if reflectors:
         ds = np.zeros_like(s)
+        refl = np.zeros_like(s)
         for k, sign in zip((nz // 4, nz // 2, 3 * nz // 4), (1.0, -1.0, 1.0)):
             ds[k] = sign * 0.05 * s[k]
+            refl[k] = sign * 0.05          # reflectivity of the same three flat reflectors
+        extra["refl3"] = refl
     else:
         g = gaussian_filter(rng.standard_normal((nz, ny, nx)), sigma)
         ds = amp * s * g / g.std()
     return Workload(name, nx, ny, nz, NX, NY, nw, 1, d, d, d, fmin, dw, nref, taper, s, sref1, labels1,
-                    src, rec, wav, ds)
+                    src, rec, wav, ds, extra)
+
+
+def salt_velocity(nx, ny, nz, d, rng, layers: int = 8, vmin: float = 1500.0, vmax: float = 4500.0,
+                  fluct: float = 0.03):
+    """Layered vmin..vmax model + a vmax ellipsoid ("salt") + seeded fluctuation of ``fluct`` (m/s).
+
+    Layers are equal-thickness, velocity rising linearly from vmin to vmax;
+    the ellipsoid is centred laterally at 55 % of the depth with semi-axes
+    (0.25 nx, 0.2 ny, 0.12 nz) cells; the fluctuation is a Gaussian random
+    field smoothed over 2 cells, scaled to ``fluct`` relative standard
+    deviation, and is not applied inside the salt.
+    """
+    from scipy.ndimage import gaussian_filter
+    z = np.arange(nz)[:, None, None]
+    y = np.arange(ny)[None, :, None]
+    x = np.arange(nx)[None, None, :]
+    layer = np.minimum(z * layers // nz, layers - 1)
+    v = np.broadcast_to(vmin + (vmax - vmin) * layer / (layers - 1), (nz, ny, nx)).astype(np.float64)
+    r = (((x - (nx - 1) / 2) / (0.25 * nx)) ** 2 + ((y - (ny - 1) / 2) / (0.2 * ny)) ** 2
+         + ((z - 0.55 * (nz - 1)) / (0.12 * nz)) ** 2)
+    salt = r <= 1.0
+    g = gaussian_filter(rng.standard_normal((nz, ny, nx)), 2.0)
+    v = v * (1.0 + fluct * g / g.std())
+    return np.where(salt, vmax, v)
+
+
+def nsps_workload(name: str, nx: int, nz: int, nw: int, fmin: float, fmax: float, nref: int = 8,
+                  taper: int = 20, d: float = 10.0, seed: int = 3) -> Workload:
+    """One-way propagation in a strongly heterogeneous model (salt_velocity) on an
+    nx^2 x nz grid padded to 2nx; one Ricker 15 Hz source at the surface centre."""
+    rng = np.random.default_rng(seed)
+    ny = nx
+    NX = NY = 2 * nx
+    s = 1.0 / salt_velocity(nx, ny, nz, d, rng) ** 2
+    sref1 = np.empty((nz, nref, 1), np.complex64)
+    labels1 = np.empty((nz, 1, NY, NX), np.uint8)
+    for z in range(nz):
+        sref1[z], labels1[z] = select_refs_depth(s[z][None].astype(np.complex64), nref, NX - nx, NY - ny)
+    dw = (fmax - fmin) / (nw - 1) if nw > 1 else 1.0
+    f = fmin + dw * np.arange(nw)
+    wav = ricker_spectrum(f)[None].astype(np.complex64)
+    src = (np.array([nx // 2 * d], np.float32), np.array([ny // 2 * d], np.float32),
+           np.array([0.0], np.float32), np.array([0], np.int32))
+    gx, gy = np.meshgrid(np.arange(nx) * d, np.arange(ny) * d)
+    rec = (gx.ravel().astype(np.float32), gy.ravel().astype(np.float32),
+           np.zeros(nx * ny, np.float32), np.zeros(nx * ny, np.int32))
+    return Workload(name, nx, ny, nz, NX, NY, nw, 1, d, d, d, fmin, dw, nref, taper, s, sref1, labels1,
+                    src, rec, wav, np.zeros_like(s))
 
 
 CONFIGS = {
     # plain PSPI shot migration (CPU-runnable size): 128^2 x 100, 32 freqs 5-30 Hz, nref 4
     "c1": dict(nx=128, nz=100, nw=32, fmin=5.0, fmax=30.0, nref=4, taper=20, seed=1, reflectors=True),
-    # Born forward + adjoint: 256^2 x 200, 64 freqs 3-40 Hz, smoothed 5 % random perturbation
-    "c2": dict(nx=256, nz=200, nw=64, fmin=3.0, fmax=40.0, nref=4, taper=20, seed=2),
+    # Born forward + adjoint: 256^2 x 200, 64 freqs 3-40 Hz, smoothed 5 % random perturbation,
+    # receivers on the deepest plane (transmission)
+    "c2": dict(nx=256, nz=200, nw=64, fmin=3.0, fmax=40.0, nref=4, taper=20, seed=2, rec_depth=1990.0),
 }
+# NSPS propagation in a layered + salt model: 384^2 x 300, 100 freqs 2-50 Hz, nref 8
+NSPS_CONFIGS = {
+    "c3": dict(nx=384, nz=300, nw=100, fmin=2.0, fmax=50.0, nref=8, taper=20, seed=3),
+}
+
+
+def workload_dims(w: Workload):
+    """fwix_dims of a workload (origins at 0)."""
+    from ._lib import make_dims
+    return make_dims(w.NX, w.NY, w.NW, w.NS, w.nx, w.ny, 0.0, w.dx, 0.0, w.dy, w.ow, w.dw, 0.0, w.dz, w.nz)
+
+
+def device_inputs(w: Workload, dev):
+    """Upload a workload and expand its frequency-invariant arrays over NW on the GPU.
+
+    Returns slowness and dslow (nz, NW, ny, nx) complex64, reference
+    slownesses (nz, nref, NW), labels (nz, NW, NY, NX) uint8 and the
+    wavelet (nsrc, NW).
+    """
+    import torch
+    nz, NW = w.nz, w.NW
+    s3 = torch.from_numpy(w.slow3.astype(np.float32)).to(dev)
+    slow = torch.empty((nz, NW, w.ny, w.nx), dtype=torch.complex64, device=dev)
+    slow.real.copy_(s3[:, None].expand(nz, NW, w.ny, w.nx))
+    slow.imag.zero_()
+    d3 = torch.from_numpy(w.dslow3.astype(np.float32)).to(dev)
+    ds = torch.empty_like(slow)
+    ds.real.copy_(d3[:, None].expand(nz, NW, w.ny, w.nx))
+    ds.imag.zero_()
+    sref = torch.from_numpy(w.sref1).to(dev).expand(nz, w.nref, NW).contiguous()
+    labels = torch.from_numpy(w.labels1).to(dev).expand(nz, NW, w.NY, w.NX).contiguous()
+    wav = torch.from_numpy(w.wavelet).to(dev)
+    del s3, d3
+    return slow, ds, sref, labels, wav

@@ -240,3 +240,42 @@ def test_born_checkpointing_is_exact_and_deterministic(dev):
     m = _z((g.nz, g.NW, g.ny, g.nx), dev)
     born.adjoint(False, m, _t(dd, dev))
     assert torch.equal(m, outs[0][0])
+
+
+@pytest.mark.parametrize("ck", [1, 3])
+def test_reflect_channel_matches_oracle(dev, ck):
+    """Up-going reflection recording, its adjoint and the surface-data migration image."""
+    import torch
+    g, born, slow, sref, labels, src, rec, wav, rng = _born_setup(dev, ckpt=ck, seed=21)
+    s64, lab = sref.astype(np.complex128), labels.astype(np.int64)
+    born.model()
+    _, bg = O.model_background(wav, src, rec, s64, lab, g, g.dz, 0.0, 4, 4)
+    refl = crand(rng, (g.nz, g.NW, g.ny, g.nx)) * 0.05
+    d = _z((born.nrec, g.NW), dev)
+    born.reflect_forward(False, _t(refl, dev), d)
+    ref_d = O.reflect_forward(refl, bg, rec, s64, lab, g, g.dz, 0.0, 4, 4)
+    assert rel(d.cpu().numpy(), ref_d) < TOL_WFLD
+    dd = crand(rng, (born.nrec, g.NW))
+    m = _z(refl.shape, dev)
+    born.reflect_adjoint(False, m, _t(dd, dev))
+    ref_m = O.reflect_adjoint(dd, bg, rec, s64, lab, g, g.dz, 0.0, 4, 4)
+    assert rel(m.cpu().numpy(), ref_m) < TOL_WFLD
+    assert dot_err(refl, d.cpu().numpy(), dd, m.cpu().numpy()) < TOL_DOT
+    img = _z((g.nz, g.ny, g.nx), dev)
+    born.reflect_migrate(False, img, _t(dd, dev))
+    assert rel(img.cpu().numpy(), ref_m.sum(axis=1)) < TOL_WFLD
+    # add = true accumulates
+    born.reflect_forward(True, _t(refl, dev), d)
+    assert rel(d.cpu().numpy(), 2 * ref_d) < TOL_WFLD
+    torch.cuda.synchronize()
+
+
+def test_reflect_coef_matches_oracle(dev):
+    from pyfwix_amd.ops import reflect_coef
+    rng = np.random.default_rng(4)
+    g = grid(48, 40, 3, 1, nx=40, ny=32, nz=5)
+    v = rng.uniform(1500, 4500, (g.nz, g.NW, g.ny, g.nx))
+    s = (1.0 / v ** 2) * (1 - 0.03j * rng.random(v.shape))
+    rho = rng.uniform(1.0, 2.5, v.shape) * (1 + 0.01j * rng.standard_normal(v.shape))
+    r = reflect_coef(dims_of(g), _t(s, dev), _t(rho, dev))
+    assert rel(r.cpu().numpy(), O.reflect_coef(s.astype(np.complex64), rho.astype(np.complex64))) < 1e-5

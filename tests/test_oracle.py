@@ -256,3 +256,52 @@ def test_recursions_and_born_dot_products():
     # the migration image is the Born adjoint's imaging core: with S = identity
     # it equals sum_w of the IC adjoint, a real identity we can check on one depth
     assert img.shape == (g.nz, g.ny, g.nx) and np.all(np.isfinite(img))
+
+
+def test_reflect_coef_is_the_normal_incidence_impedance_contrast():
+    """r = (Z_{z+1} - Z_z) / (Z_{z+1} + Z_z) with impedance Z = rho * v = rho / sqrt(s)."""
+    rng = np.random.default_rng(3)
+    v = rng.uniform(1500, 4500, (5, 2, 3, 4))
+    rho = rng.uniform(1.0, 2.5, v.shape)
+    r = O.reflect_coef(1.0 / v ** 2, rho)
+    Z = rho * v
+    ref = (Z[1:] - Z[:-1]) / (Z[1:] + Z[:-1])
+    assert np.max(np.abs(r[:-1] - ref)) < 1e-13
+    assert np.all(r[-1] == 0)                                   # last slice paired with itself
+    assert np.all(O.reflect_coef(np.full(v.shape, 4e-7), rho * 0 + 1.7) == 0)   # no contrast
+    # complex slowness: principal square roots, and swapping the layers flips the sign
+    s = (1.0 / v ** 2) * (1 - 0.02j)
+    sw = O.reflect_coef(np.stack([s[1], s[0]]), np.stack([rho[1], rho[0]]))
+    assert np.max(np.abs(sw[0] + O.reflect_coef(s[:2], rho[:2])[0])) < 1e-13
+
+
+def test_reflect_channel_single_reflector_and_dot():
+    rng = np.random.default_rng(8)
+    g = grid(16, 12, 2, 1, nx=12, ny=9, nz=4, ow=6.0, dw=3.0)
+    slow, sref, labels = _refs(g, rng, 2)
+    src = (np.array([55.0], np.float32), np.array([45.0], np.float32), np.array([2.0], np.float32), np.array([0]))
+    rx, ry = np.meshgrid(np.arange(1, 11) * 10.0, np.arange(1, 8) * 10.0)
+    rec = (rx.ravel().astype(np.float32), ry.ravel().astype(np.float32), np.zeros(rx.size, np.float32),
+           np.zeros(rx.size, int))
+    wav = crand(rng, (1, g.NW))
+    _, bg = O.model_background(wav, src, rec, sref, labels, g, g.dz, 0.0, 2, 2)
+    # a reflector at depth 2 only: d = Rec_2^T V + Rec_1^T P_1 V + Rec_0^T P_0 P_1 V, V = r_2 U_2
+    refl = np.zeros((g.nz, g.NW, g.ny, g.nx), complex)
+    refl[2] = crand(rng, refl[2].shape)
+    V = O.ic_forward(refl[2], bg[2])
+    shape = V.shape
+    d = O.injection_adjoint(V, *rec, g, 2)
+    V = O.pspi_forward(V, sref[1], labels[1], g, g.dz, 0.0, 2, 2)
+    d += O.injection_adjoint(V, *rec, g, 1)
+    V = O.pspi_forward(V, sref[0], labels[0], g, g.dz, 0.0, 2, 2)
+    d += O.injection_adjoint(V, *rec, g, 0)
+    assert rel(O.reflect_forward(refl, bg, rec, sref, labels, g, g.dz, 0.0, 2, 2), d) < 1e-14
+    assert np.linalg.norm(d) > 0                                # surface receivers see the reflection
+    # dot test of the pair and the image as the frequency sum of the adjoint
+    m = crand(rng, refl.shape)
+    dd = crand(rng, (rx.size, g.NW))
+    Am = O.reflect_forward(m, bg, rec, sref, labels, g, g.dz, 0.0, 2, 2)
+    Atd = O.reflect_adjoint(dd, bg, rec, sref, labels, g, g.dz, 0.0, 2, 2)
+    assert dot_err(m, Am, dd, Atd) < 1e-12
+    img = O.reflect_migrate(dd, bg, rec, sref, labels, g, g.dz, 0.0, 2, 2)
+    assert rel(img, Atd.sum(axis=1)) < 1e-14 and shape == (g.NS, g.NW, g.NY, g.NX)
