@@ -203,6 +203,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="run the counter calibration copy first")
     ap.add_argument("--pass-sample", type=int, default=8,
                     help="time every n-th launch of each pass with HIP events inside the timed region")
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="planes per launch of the fused passes (0: the library's choice)")
     ap.add_argument("--no-pass-timing", action="store_true",
                     help="no event markers at all (counter-collection runs)")
     ap.add_argument("--summarize-stats", nargs=2, metavar=("CSV", "OUT"))
@@ -239,7 +241,8 @@ def main():
     w = born_workload(args.config, **cfg)
     slow, ds, sref, labels, wav = device_inputs(w, dev)
     dims = workload_dims(w)
-    born = Born(dims, nref=w.nref, taperx=w.taper, tapery=w.taper, taylor=1, eps_scat=0.04, ckpt_every=1)
+    born = Born(dims, nref=w.nref, taperx=w.taper, tapery=w.taper, taylor=1, eps_scat=0.04, ckpt_every=1,
+                planes_per_chunk=args.chunk)
     born.set_refs(sref, labels)
     born.set_slow(slow)
     born.set_sources(*w.src)
@@ -314,7 +317,8 @@ def main():
         "config": {"workload": f"{args.config}: Born forward + adjoint (migration), background modeling "
                                f"included", "nx": w.nx, "ny": w.ny, "nz": w.nz, "NX": w.NX, "NY": w.NY,
                    "nw": w.NW, "shots": w.NS, "nref": w.nref, "taper": w.taper, "receivers": nrec,
-                   "depth_steps_per_step": per, "checkpointing": "all depths in HBM"},
+                   "receiver_depth_m": float(w.rec[2][0]), "depth_steps_per_step": per,
+                   "planes_per_chunk": args.chunk or "auto", "checkpointing": "all depths in HBM"},
         "roofline": roof,
         "step_roofline": {"achieved": round(alg_total / wall / 1e9, 1), "unit": "GB/s",
                           "frac": round(alg_total / wall / 1e9 / HBM_PEAK_GBS, 4),

@@ -89,7 +89,8 @@ class RefSampler:
     def __init__(self, slow, par):
         par = par if isinstance(par, paramObj) else paramObj(par)
         self.cppMode = _RefSampler(_slow_array(slow), par.getInt("nref"), par.getInt("padx", 0),
-                                   par.getInt("pady", 0))
+                                   par.getInt("pady", 0), par.getString("ref_method", "quantile"),
+                                   par.getInt("ref_seed", 0))
 
     def get_ref_slow(self, iz, iref):
         return self.cppMode.get_ref_slow(iz, iref)

@@ -1,7 +1,10 @@
 """Build libpyfwix_hip.so for gfx950 with hipcc (no rocFFT, no torch extension).
 
 Each ``csrc/*.hip`` file is compiled to an object in ``pyfwix_amd/_build/`` in
-parallel, then linked in-tree as ``pyfwix_amd/libpyfwix_hip.so``.
+parallel, then linked in-tree as ``pyfwix_amd/libpyfwix_hip.so``.  The C++
+operator-layer test driver (``tests/cpp/test_operators.cpp`` over
+``include/fwix_operators.hpp``) is linked against it as
+``tests/cpp/test_operators``.
 
     python -m pyfwix_amd.build            # build (always recompiles)
 """
@@ -55,7 +58,24 @@ def build(verbose: bool = True, jobs: int | None = None) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB} ({LIB.stat().st_size // 1024} KiB) from {len(srcs)} sources")
+    build_cpp_tests(verbose)
     return LIB
+
+
+def build_cpp_tests(verbose: bool = True) -> Path | None:
+    src = ROOT / "tests" / "cpp" / "test_operators.cpp"
+    if not src.exists():
+        return None
+    exe = src.with_suffix("")
+    cmd = [HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}", "-I", str(ROOT / "include"), str(src),
+           "-L", str(PKG), "-lpyfwix_hip", "-Wl,-rpath,$ORIGIN/../../pyfwix_amd", "-o", str(exe),
+           "-Wno-unused-command-line-argument"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"C++ operator test driver failed to build:\n{res.stdout}\n{res.stderr}")
+    if verbose:
+        print(f"built {exe}")
+    return exe
 
 
 if __name__ == "__main__":

@@ -711,7 +711,8 @@ int fwix_born_reflect_migrate(fwix_op op, int add, fwix_c64* image, const fwix_c
 }
 int fwix_reflect_coef(const fwix_dims* d, const fwix_c64* slow, const fwix_c64* den, fwix_c64* refl, fwix_stream s) {
   return guard([&] {
-    validate_dims(d, true);
+    validate_dims(d, false);  // point-wise: any lateral size
+    if (d->nz < 1) throw Error("dims: nz must be >= 1");
     if (!slow || !den || !refl) throw Error("null slowness, density or output");
     const size_t slice = (size_t)d->NW * d->ny * d->nx;
     hip_check(launch_reflect_coef((const float2*)slow, (const float2*)den, (float2*)refl, d->nz, slice, S(s)),

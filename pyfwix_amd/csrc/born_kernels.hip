@@ -346,12 +346,14 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj_out(KArgs a) {
 }
 
 // ------------------------------------------------------------------ reflection coefficients
-// principal square root of a complex number: sqrt((|z|+x)/2) + i sign(y) sqrt((|z|-x)/2)
+// principal square root of a complex number without cancellation: the
+// larger part from sqrt((|z| + |x|)/2), the other as y / (2 * that)
 __device__ __forceinline__ float2 csqrt_principal(float2 z) {
   const float m = hypotf(z.x, z.y);
-  const float re = sqrtf(fmaxf(0.5f * (m + z.x), 0.f));
-  const float im = copysignf(sqrtf(fmaxf(0.5f * (m - z.x), 0.f)), z.y);
-  return make_float2(re, im);
+  const float t = sqrtf(0.5f * (m + fabsf(z.x)));
+  if (t == 0.f) return czero();
+  const float o = 0.5f * fabsf(z.y) / t;
+  return z.x >= 0.f ? make_float2(t, copysignf(o, z.y)) : make_float2(o, copysignf(t, z.y));
 }
 __device__ __forceinline__ float2 cdiv(float2 a, float2 b) {
   const float q = 1.f / fmaf(b.x, b.x, b.y * b.y);

@@ -194,19 +194,21 @@ PwArgs Tables::pwargs() const {
 int auto_chunk(const fwix_dims& d, int nterm, int requested) {
   const int planes = d.NW * d.NS;
   if (requested > 0) return requested < planes ? requested : planes;
-  // enough planes that a column pass launches >= 1024 workgroups (4 per CU),
-  // at least what keeps the chunk's intermediates (spectrum + nterm fan-out
-  // planes) within 128 MiB, and at most 2 GiB of scratch
+  // Enough planes that a column pass launches >= 4096 workgroups (16 per CU)
+  // and a row pass >= 16 waves per SIMD slot, so every launch reaches a
+  // steady state of loads overlapping transforms instead of one burst; at
+  // most 4 GiB of scratch (spectrum + nterm fan-out planes).  Chunks are
+  // then balanced so the last one is not a small tail.
   const double per_plane = (double)(1 + nterm) * d.NX * d.NY * 8.0;
   const int cw = fft_size_supported(d.NY) ? col_tile_width(d.NY) : 16;
   const int blocks_per_plane = (d.NX + cw - 1) / cw;
-  int c = (1024 + blocks_per_plane - 1) / blocks_per_plane;
-  const int c_cache = (int)(128.0 * 1024 * 1024 / per_plane);
-  if (c_cache > c) c = c_cache;
-  const int c_max = (int)(2048.0 * 1024 * 1024 / per_plane);
+  int c = (4096 + blocks_per_plane - 1) / blocks_per_plane;
+  const int c_max = (int)(4096.0 * 1024 * 1024 / per_plane);
   if (c > c_max) c = c_max;
   if (c < 1) c = 1;
-  return c < planes ? c : planes;
+  if (c >= planes) return planes;
+  const int nchunks = (planes + c - 1) / c;
+  return (planes + nchunks - 1) / nchunks;
 }
 
 // ------------------------------------------------------------------ one step engine

@@ -51,6 +51,27 @@ struct ColCfg {
   static_assert(NB >= 1 && V >= 4, "column tile too narrow");
 };
 
+// Workgroups are handed to the 8 XCDs round-robin by linear id, and each
+// XCD has its own L2.  A column tile reads CW * 8 bytes of every row; with
+// CW = 8 that is half of a 128-byte line, so two neighbouring tiles share
+// every line they read.  Linear ids L and L + 8 run on the same XCD back to
+// back, so they take the two halves: within each group of 16 ids,
+// 16g + 8h + q gets the logical tile 16g + 2q + h (a trailing partial group
+// keeps the identity).  Logical tile J is column tile J % gridDim.x of plane
+// J / gridDim.x.
+template <class C>
+__device__ __forceinline__ void col_tile(int& plane, int& c0) {
+  const int nb = gridDim.x;
+  const int L = blockIdx.y * nb + blockIdx.x;
+  int J = L;
+  if constexpr (C::CW * 8 < 128) {
+    const int total = nb * gridDim.y;
+    if ((L & ~15) + 16 <= total) J = (L & ~15) + 2 * (L & 7) + ((L >> 3) & 1);
+  }
+  plane = J / nb;
+  c0 = (J - plane * nb) * C::CW;
+}
+
 // ------------------------------------------------------------------ row passes
 // out = (add ? out : 0) + post(scale * F_x(pre * in))   along x, one row per wave
 template <int N, bool INV, bool PRE_TAPER, bool POST_TAPER>
@@ -413,7 +434,8 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   using F = WaveFft<N, C::NB, false, INV, PadLds<N>>;
   __shared__ float2 smem[C::SMEM];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int p = blockIdx.y, c0 = blockIdx.x * C::CW;
+  int p, c0;
+  col_tile<C>(p, c0);
   const size_t pb = (size_t)p * a.NX * N;
   float2* priv = smem + wave * C::PRIV;
   const TileOffs<C> to(a.NX, c0);
@@ -441,7 +463,8 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   static_assert(F::P::last() == Fi::P::first(), "register hand-off needs mirrored plans");
   __shared__ float2 smem[C::SMEM];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int p = blockIdx.y, c0 = blockIdx.x * C::CW;
+  int p, c0;
+  col_tile<C>(p, c0);
   const size_t pb = (size_t)p * a.NX * N;
   float2* priv = smem + wave * C::PRIV;
   const TileOffs<C> to(a.NX, c0);
@@ -491,7 +514,8 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   static_assert(F::P::last() == Fi::P::first(), "register hand-off needs mirrored plans");
   __shared__ float2 smem[C::SMEM];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int p = blockIdx.y, c0 = blockIdx.x * C::CW;
+  int p, c0;
+  col_tile<C>(p, c0);
   const size_t pb = (size_t)p * a.NX * N;
   float2* priv = smem + wave * C::PRIV;
   const TileOffs<C> to(a.NX, c0);

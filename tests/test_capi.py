@@ -92,3 +92,19 @@ def test_workload_c1_geometry():
     assert np.count_nonzero(w.dslow3.reshape(w.nz, -1).any(axis=1)) == 3
     f = np.array([15.0])
     assert abs(abs(ricker_spectrum(f)[0]) - 2 / np.sqrt(np.pi) / 15 * np.exp(-1)) < 1e-12
+
+
+def test_refsampler_kmeans_is_seeded_and_finds_clusters():
+    from pyfwix_amd.refsampler import kmeans_1slice, select_refs
+    rng = np.random.default_rng(5)
+    true = np.array([2.0e-7, 3.0e-7, 4.4e-7]) * (1 - 0.01j)
+    x = np.concatenate([c + 1e-9 * (rng.standard_normal(400) + 1j * rng.standard_normal(400)) for c in true])
+    c = np.sort_complex(kmeans_1slice(x, 3, np.random.default_rng(1), eps=1e-12))
+    assert np.max(np.abs(c - np.sort_complex(true))) < 2e-10
+    s = np.broadcast_to(x.reshape(1, 1, 30, 40), (2, 3, 30, 40)).astype(np.complex64)
+    r1, l1 = select_refs(s, 3, 5, 2, method="kmeans", seed=4)
+    r2, l2 = select_refs(s, 3, 5, 2, method="kmeans", seed=4)
+    assert np.array_equal(r1, r2) and np.array_equal(l1, l2)            # seeded: reproducible
+    d = np.abs(s[0, 0].reshape(-1, 1) - r1[0, :, 0][None])
+    assert np.array_equal(l1[0, 0, :30, :40].ravel(), d.argmin(axis=1))  # nearest-center labels
+    assert np.array_equal(l1[0, 0, 30:, :40], np.repeat(l1[0, 0, 29:30, :40], 2, axis=0))
