@@ -225,10 +225,14 @@ struct WaveFft {
   // on return it holds the spectrum at last-stage output positions
   // (edge_pos<..., P::last()>).  Unnormalized.
   __device__ __forceinline__ static void run(float2* v, float2* lds, const float2* __restrict__ tw, int lane) {
-    // Keep the twiddle loads next to their use: without this fence the
-    // compiler hoists every stage's (loop-invariant) twiddles out of the
-    // per-reference loops of the fused kernels and runs out of registers.
+    // Keep the twiddles and the per-element twiddle / LDS addresses next to
+    // their use: without these fences the compiler hoists every stage's
+    // (loop-invariant) loads and addresses out of the per-reference loops
+    // of the fused kernels and runs out of registers.  The lane index is
+    // made opaque here, so the index arithmetic derived from it is redone
+    // per transform (a few integer ops) instead of held in registers.
     asm volatile("" ::: "memory");
+    asm volatile("" : "+v"(lane));
     stages<0>(v, lds, tw, lane);
   }
 };

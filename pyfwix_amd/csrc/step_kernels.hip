@@ -47,7 +47,7 @@ struct ColCfg {
   // waves per SIMD the register budget is sized for: 4 (two workgroups per
   // CU, 128 VGPRs) where the column transform and its terms fit in 128
   // registers, else 2 (256 VGPRs, no spills)
-  static constexpr int WPE = (!size_has3(N) && N <= 256) ? 4 : 2;
+  static constexpr int WPE = (!size_has3(N) && N <= 512) ? 4 : 2;
   static_assert(NB >= 1 && V >= 4, "column tile too narrow");
 };
 
