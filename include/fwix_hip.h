@@ -201,6 +201,26 @@ int fwix_born_model(fwix_op op, int add, fwix_c64* data, fwix_stream s);
 int fwix_born_forward(fwix_op op, int add, const fwix_c64* dslow, fwix_c64* data, fwix_stream s);
 int fwix_born_adjoint(fwix_op op, int add, fwix_c64* dslow, const fwix_c64* data, fwix_stream s);
 int fwix_born_migrate(fwix_op op, int add, fwix_c64* image, const fwix_c64* data, fwix_stream s);
+/* Background density (nz, NW, ny, nx), borrowed; needed by the up pass and
+ * the extended operator (Propagator.cpp:117-120 set_background_model). */
+int fwix_born_set_density(fwix_op op, const fwix_c64* den_dev);
+/* Up pass of Propagator::forward (Propagator.cpp:181-211): V_nz = 0,
+ * V_z = P_z V_{z+1} + r_z U_z with r from (slowness, density); keeps every
+ * V_z in HBM and records data (+)= Rec_z^T V_z when data != NULL.  Run after
+ * fwix_born_model; together they give the nonlinear modeling's data. */
+int fwix_born_model_up(fwix_op op, int add, fwix_c64* data, fwix_stream s);
+/* ExtendedBorn::forward / adjoint with all four channels
+ * (ExtendedBorn.cpp:80-150): model (dslow, dden), each (nz, NW, ny, nx);
+ * down scattering, down-reflected, up scattering (scatter and PSPI at z-1
+ * with V_z, ds_z) and back scattering through the linearized reflection
+ * coefficient.  The linearized reflection is the exact derivative of
+ * fwix_reflect_coef (lin_reflect.cu:69-79 uses (s1/s0)^(1/4) where the
+ * derivative has (s1/s0)^(1/2)).  Needs fwix_born_model and
+ * fwix_born_model_up; holds two more wavefields per depth in HBM. */
+int fwix_born_extended_forward(fwix_op op, int add, const fwix_c64* dslow, const fwix_c64* dden, fwix_c64* data,
+                               fwix_stream s);
+int fwix_born_extended_adjoint(fwix_op op, int add, fwix_c64* dslow, fwix_c64* dden, const fwix_c64* data,
+                               fwix_stream s);
 /* Up-going reflection channel of the background modeling
  * (propagator/src/Propagator.cpp:185-208, Reflect.h:20-36, reflect.cu:8-47):
  * V = 0; for z = nz-1..0: V <- P_z V; V += refl_z * U_z on the model region;

@@ -33,6 +33,7 @@ __all__ = [
     "downward_forward", "downward_adjoint", "upward_forward", "upward_adjoint",
     "model_background", "born_forward", "born_adjoint", "migrate",
     "reflect_coef", "reflect_forward", "reflect_adjoint", "reflect_migrate",
+    "lin_reflect", "lin_reflect_adjoint", "model_up", "extended_born_forward", "extended_born_adjoint",
     "SCATTER_COEF", "set_fft_workers",
 ]
 
@@ -571,3 +572,145 @@ def reflect_migrate(data, bg, rec, sref, labels, g: Grid, dz, eps=0.0, tx=0, ty=
     the source wavefield U correlated with the receiver wavefield mu
     extrapolated downward (the frequency sum of ``reflect_adjoint``)."""
     return reflect_adjoint(data, bg, rec, sref, labels, g, dz, eps, tx, ty).sum(axis=1)
+
+
+def _pair(a):
+    """(a_z, a_{z+1}) with the last slice paired with itself (``Reflect.h:29-36``)."""
+    a = np.asarray(a)
+    return a, np.concatenate([a[1:], a[-1:]])
+
+
+def _lin_coefs(slow, den, ds_power=1.0):
+    s0, s1 = _pair(np.asarray(slow, np.complex128))
+    r0, r1 = _pair(np.asarray(den, np.complex128))
+    q0, q1 = np.sqrt(s0), np.sqrt(s1)
+    D = r1 * q0 + r0 * q1
+    A = r0 * r1 / D ** 2
+    B = 2.0 * q0 * q1 / D ** 2
+    c = q1 / q0
+    cp = c if ds_power == 1.0 else c ** ds_power
+    return A * cp, A / cp, r0 * B, r1 * B
+
+
+def lin_reflect(dslow, dden, slow, den, ds_power=1.0):
+    """Linearized reflection coefficient per depth (``propagator/src/lin_reflect.cu:9-90``):
+
+    ``L_z = A q ds_z - A q^-1 ds_{z+1} + rho_z B drho_{z+1} - rho_{z+1} B drho_z`` with
+    ``D = rho_{z+1} sqrt(s_z) + rho_z sqrt(s_{z+1})``, ``A = rho_z rho_{z+1} / D^2``,
+    ``B = 2 sqrt(s_z) sqrt(s_{z+1}) / D^2`` and ``q = (sqrt(s_{z+1}) / sqrt(s_z))^ds_power``.
+    ``ds_power = 1`` is the exact derivative of :func:`reflect_coef`; the reference
+    kernel uses 1/2 (``lin_reflect.cu:69-79``).  The last slice is paired with
+    itself, so ``L_{nz-1} = 0``.  Arrays (nz, NW, ny, nx).
+    """
+    a, b, c0, c1 = _lin_coefs(slow, den, ds_power)
+    d0, d1 = _pair(np.asarray(dslow, np.complex128))
+    p0, p1 = _pair(np.asarray(dden, np.complex128))
+    L = a * d0 - b * d1 + c0 * p1 - c1 * p0
+    L[-1] = 0.0
+    return L
+
+
+def lin_reflect_adjoint(g, slow, den, ds_power=1.0):
+    """Adjoint of :func:`lin_reflect`: (dslow, dden) from g (nz, NW, ny, nx)."""
+    a, b, c0, c1 = _lin_coefs(slow, den, ds_power)
+    g = np.array(g, np.complex128)
+    g[-1] = 0.0
+    ds = np.conj(a) * g
+    ds[1:] -= (np.conj(b) * g)[:-1]
+    dd = -np.conj(c1) * g
+    dd[1:] += (np.conj(c0) * g)[:-1]
+    return ds, dd
+
+
+def model_up(bg, slow, den, rec, sref, labels, g: Grid, dz, eps=0.0, tx=0, ty=0):
+    """Up pass of the nonlinear modeling (``Propagator.cpp:181-211``):
+    V = 0; for z = nz-1..0: V <- P_z V; V += r_z * U_z; d += Rec_z^T V; V_z := V.
+    Returns (data, [V_z])."""
+    nz = sref.shape[0]
+    r = reflect_coef(slow, den)
+    V = np.zeros_like(bg[0])
+    data = np.zeros((len(rec[0]), g.NW), V.dtype)
+    vs = [None] * nz
+    for z in range(nz - 1, -1, -1):
+        V = pspi_forward(V, sref[z], labels[z], g, dz, eps, tx, ty)
+        V = ic_forward(r[z], bg[z], d=V, add=True)
+        data = injection_adjoint(V, *rec, g, z, traces=data, add=True)
+        vs[z] = V.copy()
+    return data, vs
+
+
+def extended_born_forward(dslow, dden, bg, vbg, slow, den, rec, sref, labels, g: Grid, dz, eps=0.0, tx=0, ty=0,
+                          eps_s=0.04, ntaylor=1, ds_power=1.0):
+    """The four scattering channels of ``ExtendedBorn::forward`` (``ExtendedBorn.cpp:80-118``),
+    each as its own loop: down scattering (:152-178), down-reflected (:210-238), up
+    scattering (:282-311, scatter and PSPI at z-1 with V_z and ds_z) and back
+    scattering through the linearized reflection (:352-389).  ``bg``/``vbg`` are the
+    down- and up-going background wavefields U_z, V_z."""
+    nz = sref.shape[0]
+    shape = bg[0].shape
+    r = reflect_coef(slow, den)
+    L = lin_reflect(dslow, dden, slow, den, ds_power)
+    data = np.zeros((len(rec[0]), g.NW), np.complex128)
+    P = lambda u, z: pspi_forward(u, sref[z], labels[z], g, dz, eps, tx, ty)  # noqa: E731
+    S = lambda u, z: scatter_forward(u, slow[z], g, dz, eps_s, ntaylor)       # noqa: E731
+    # 1. down scattering, keeping dU_z
+    dU = np.zeros(shape, complex)
+    dus = []
+    for z in range(nz):
+        data = injection_adjoint(dU, *rec, g, z, traces=data, add=True)
+        dus.append(dU.copy())
+        dU = P(dU, z) + P(S(ic_forward(dslow[z], bg[z]), z), z)
+    # 2. down-reflected
+    dV = np.zeros(shape, complex)
+    for z in range(nz - 1, -1, -1):
+        dV = P(dV, z)
+        dV = ic_forward(r[z], dus[z], d=dV, add=True)
+        data = injection_adjoint(dV, *rec, g, z, traces=data, add=True)
+    # 3. up scattering
+    dW = np.zeros(shape, complex)
+    for z in range(nz - 1, -1, -1):
+        data = injection_adjoint(dW, *rec, g, z, traces=data, add=True)
+        if z > 0:
+            dW = P(dW, z - 1) + P(S(ic_forward(dslow[z], vbg[z]), z - 1), z - 1)
+    # 4. back scattering
+    dX = np.zeros(shape, complex)
+    for z in range(nz - 1, -1, -1):
+        dX = P(dX, z)
+        dX = ic_forward(L[z], bg[z], d=dX, add=True)
+        data = injection_adjoint(dX, *rec, g, z, traces=data, add=True)
+    return data
+
+
+def extended_born_adjoint(data, bg, vbg, slow, den, rec, sref, labels, g: Grid, dz, eps=0.0, tx=0, ty=0,
+                          eps_s=0.04, ntaylor=1, ds_power=1.0):
+    """Adjoint of :func:`extended_born_forward` (``ExtendedBorn.cpp:122-150``: channels
+    3, 2, 1, 4 in that order, channel 2 consuming the residual wavefields that channel
+    3 back-propagates).  Returns (dslow, dden)."""
+    nz = sref.shape[0]
+    shape = bg[0].shape
+    r = reflect_coef(slow, den)
+    ds = np.zeros((nz, g.NW, g.ny, g.nx), complex)
+    Pa = lambda u, z: pspi_adjoint(u, sref[z], labels[z], g, dz, eps, tx, ty)  # noqa: E731
+    Sa = lambda u, z: scatter_adjoint(u, slow[z], g, dz, eps_s, ntaylor)       # noqa: E731
+    Inj = lambda u, z: injection_forward(data, *rec, g, z, wfld=u, add=True, shape=shape)  # noqa: E731
+    # 3: mu_z = Inj_z(d) + P_{z-1}^adj mu_{z-1}; ds_z += IC^adj(S_{z-1}^adj P_{z-1}^adj mu_{z-1}, V_z)
+    mu = np.zeros(shape, complex)
+    mus = []
+    for z in range(nz):
+        if z > 0:
+            ds[z] += ic_adjoint(Sa(Pa(mu, z - 1), z - 1), vbg[z], g.nx, g.ny)
+            mu = Pa(mu, z - 1)
+        mu = Inj(mu, z)
+        mus.append(mu.copy())
+    # 2: lam_z = conj(r_z) mu_z + P_z^adj lam_{z+1}; ds_z += IC^adj(S_z^adj P_z^adj lam_{z+1}, U_z)
+    lam = np.zeros(shape, complex)
+    for z in range(nz - 1, -1, -1):
+        ds[z] += ic_adjoint(Sa(Pa(lam, z), z), bg[z], g.nx, g.ny)
+        lam = Pa(lam, z)
+        lam = ic_forward(np.conj(r[z]), mus[z], d=lam, add=True)
+    # 1: the down-scattering adjoint
+    ds += born_adjoint(data, bg, slow, rec, sref, labels, g, dz, eps, tx, ty, eps_s, ntaylor)
+    # 4: g_z = sum_s conj(U_z) mu_z through the adjoint linearized reflection
+    gz = np.stack([ic_adjoint(mus[z], bg[z], g.nx, g.ny) for z in range(nz)])
+    d4s, d4r = lin_reflect_adjoint(gz, slow, den, ds_power)
+    return ds + d4s, d4r

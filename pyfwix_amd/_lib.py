@@ -97,6 +97,10 @@ _SIGS = {
     "fwix_born_forward": [_P, _I, _P, _P, _P],
     "fwix_born_adjoint": [_P, _I, _P, _P, _P],
     "fwix_born_migrate": [_P, _I, _P, _P, _P],
+    "fwix_born_set_density": [_P, _P],
+    "fwix_born_model_up": [_P, _I, _P, _P],
+    "fwix_born_extended_forward": [_P, _I, _P, _P, _P, _P],
+    "fwix_born_extended_adjoint": [_P, _I, _P, _P, _P, _P],
     "fwix_born_reflect_forward": [_P, _I, _P, _P, _P],
     "fwix_born_reflect_adjoint": [_P, _I, _P, _P, _P],
     "fwix_born_reflect_migrate": [_P, _I, _P, _P, _P],

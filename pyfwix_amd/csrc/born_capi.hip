@@ -240,8 +240,11 @@ struct BornOp : fwix_op_s {
   int nsrc = 0, nrec = 0;
   int ckpt = 1;
   DevBuf store, seg, U, dU, lam;
+  DevBuf vstore;  // up-going background V_z of every depth (model_up)
+  DevBuf xstore;  // dU_z (extended forward) or mu_z (extended adjoint) of every depth
+  const float2* den = nullptr;  // (nz, NW, ny, nx) background density
   int seg_first = -1;  // first depth of the recomputed segment held in seg
-  bool have_bg = false;
+  bool have_bg = false, have_up = false;
   long long steps = 0;  // depth steps executed (statistics)
   BornOp() { kind = KIND_BORN; }
 
@@ -302,64 +305,110 @@ struct BornOp : fwix_op_s {
     return a;
   }
 
-  // dU_{z+1} = P_z [ dU_z + S_z(U_z * ds_z) ]   (ExtendedBorn.cpp:152-178)
-  void forward(int add, const float2* dslow, float2* data, hipStream_t st) {
-    if (!slow) throw Error("background slowness not set (call fwix_born_set_slow)");
-    const size_t n = P(), plane = (size_t)d.NX * d.NY;
-    if (!add) hip_check(launch_zero(data, (size_t)nrec * d.NW, st), "zero data");
-    float2* du = (float2*)dU.p;
-    hip_check(launch_zero(du, n, st), "zero dU");
+  size_t MZ() const { return (size_t)d.NW * d.ny * d.nx; }
+
+  // out = P_z[ addin + S_z(bg * ds_z) ] over every plane (Scattering.cpp:43-52,
+  // fused): addin and out may be the same wavefield.  The scattering operator
+  // and the PSPI step use the slowness and references of depth z.
+  void step_fwd(int z, const float2* bg, const float2* dsz, const float2* addin, float2* out, hipStream_t st) {
+    const size_t plane = (size_t)d.NX * d.NY;
     const float sx = (float)(1.0 / std::sqrt((double)d.NX)), sy2 = (float)(1.0 / (double)d.NY);
     const int cw = std::min(eng.chunk, d.NW);
+    for (int is = 0; is < d.NS; ++is)
+      for (int w0 = 0; w0 < d.NW; w0 += cw) {
+        const int p0 = is * d.NW + w0, np = std::min(cw, d.NW - w0);
+        KArgs a = born_args(z, p0, np);
+        const double pb = 8.0 * np * plane, mb = 8.0 * np * d.nx * d.ny;
+        const double wb = 8.0 * d.NX * eng.present_rows(z, p0, np);
+        const int T = par.taylor;
+        a.dslow = dsz;
+        a.bg = bg + (size_t)p0 * plane;
+        a.wstride = (long long)np * plane;
+        if (T >= 1) {
+          a.scale = sx;
+          { PassTimer t("born_rows_ic", 2 * pb + mb, st); hip_check(launch_rows_ic_fwd(a, st), "born ic"); }
+          a.nterm = T; a.scale = sy2;
+          { PassTimer t("born_cols_kfac", (1 + T) * pb, st); hip_check(launch_cols_fanout(2, a, st), "born scatter columns"); }
+        }
+        a.addin = addin + (size_t)p0 * plane;
+        a.nterm = T + 1; a.scale = sx;
+        { PassTimer t("born_rows_scatter", (T + 3) * pb + 2 * mb, st); hip_check(launch_rows_scatter(true, a, st), "born scatter rows"); }
+        a.nterm = par.nref;
+        a.scale = sy2;
+        { PassTimer t("cols_fanout", pb + wb, st); hip_check(launch_cols_fanout(0, a, st), "born pspi columns"); }
+        a.out = out + (size_t)p0 * plane;
+        a.add = 0;
+        a.scale = sx;
+        { PassTimer t("rows_select", wb + pb + pb / 8, st); hip_check(launch_rows_select(a, st), "born pspi rows"); }
+      }
+    ++steps;
+  }
+
+  // lam <- P_z^H lam in place, and gz += conj(bg) S_z^H (P_z^H lam) on the model
+  // region (the adjoint of step_fwd's scattering term, Scattering.cpp:54-61)
+  void step_adj(int z, float2* lam, const float2* bg, float2* gz, hipStream_t st) {
+    const size_t plane = (size_t)d.NX * d.NY;
+    const float sx = (float)(1.0 / std::sqrt((double)d.NX)), sy2 = (float)(1.0 / (double)d.NY);
+    const int cw = std::min(eng.chunk, d.NW);
+    for (int is = 0; is < d.NS; ++is)
+      for (int w0 = 0; w0 < d.NW; w0 += cw) {
+        const int p0 = is * d.NW + w0, np = std::min(cw, d.NW - w0);
+        float2* lc = lam + (size_t)p0 * plane;
+        KArgs a = born_args(z, p0, np);
+        a.wstride = (long long)np * plane;
+        const double pb = 8.0 * np * plane, mb = 8.0 * np * d.nx * d.ny;
+        const double wb = 8.0 * d.NX * eng.present_rows(z, p0, np);
+        const int T = par.taylor;
+        // mu = P_z^H lam (PSPI adjoint, written in place) fused with S_z^H's first row pass
+        a.in = lc; a.scale = sx;
+        { PassTimer t("rows_fanout", pb + pb / 8 + wb, st); hip_check(launch_rows_fanout(a, st), "born adjoint pspi rows"); }
+        a.nterm = par.nref; a.scale = sy2;
+        { PassTimer t("cols_fanin", wb + pb, st); hip_check(launch_cols_fanin(1, a, st), "born adjoint pspi columns"); }
+        a.out = lc; a.nterm = T + 1; a.scale = sx;
+        { PassTimer t("born_rows_inv_scatter_adj", (2 + T) * pb + mb, st); hip_check(launch_rows_scatter_adj(true, a, st), "born adjoint inverse + scatter rows"); }
+        if (T >= 1) {
+          a.nterm = T; a.scale = sy2;
+          { PassTimer t("born_cols_kfac_adj", (1 + T) * pb, st); hip_check(launch_cols_fanin(2, a, st), "born adjoint scatter columns"); }
+        }
+        // gz += conj(bg) S_z^H mu
+        a.in = lc; a.bg = bg + (size_t)p0 * plane;
+        a.aux = gz;
+        a.add = 1; a.nterm = T + 1; a.scale = sx;
+        { PassTimer t("born_rows_adj_ic", 3 * pb + 3 * mb, st); hip_check(launch_rows_adj_out(true, a, st), "born adjoint imaging"); }
+      }
+    ++steps;
+  }
+
+  // dU_{z+1} = P_z [ dU_z + S_z(U_z * ds_z) ]   (ExtendedBorn.cpp:152-178); with
+  // keep, dU_z of every depth is left in xstore (the down-reflected channel's input)
+  void forward(int add, const float2* dslow, float2* data, hipStream_t st, bool keep = false) {
+    if (!slow) throw Error("background slowness not set (call fwix_born_set_slow)");
+    const size_t n = P();
+    if (!add) hip_check(launch_zero(data, (size_t)nrec * d.NW, st), "zero data");
+    float2* du = keep ? (float2*)xstore.p : (float2*)dU.p;
+    hip_check(launch_zero(du, n, st), "zero dU");
     for (int z = 0; z < d.nz; ++z) {
       if (z > 0) rec.adjoint(z, data, du, st);
       if (z + 1 == d.nz) break;  // dU_nz is never recorded
-      const float2* bg = background(z, st);
-      for (int is = 0; is < d.NS; ++is)
-        for (int w0 = 0; w0 < d.NW; w0 += cw) {
-          const int p0 = is * d.NW + w0, np = std::min(cw, d.NW - w0);
-          KArgs a = born_args(z, p0, np);
-          const double pb = 8.0 * np * plane, mb = 8.0 * np * d.nx * d.ny;
-          const double wb = 8.0 * d.NX * eng.present_rows(z, p0, np);
-          const int T = par.taylor;
-          a.dslow = dslow + (size_t)z * d.NW * d.ny * d.nx;
-          a.bg = bg + (size_t)p0 * plane;
-          a.wstride = (long long)np * plane;
-          if (T >= 1) {
-            a.scale = sx;
-            { PassTimer t("born_rows_ic", 2 * pb + mb, st); hip_check(launch_rows_ic_fwd(a, st), "born ic"); }
-            a.nterm = T; a.scale = sy2;
-            { PassTimer t("born_cols_kfac", (1 + T) * pb, st); hip_check(launch_cols_fanout(2, a, st), "born scatter columns"); }
-          }
-          a.addin = du + (size_t)p0 * plane;
-          a.nterm = T + 1; a.scale = sx;
-          { PassTimer t("born_rows_scatter", (T + 3) * pb + 2 * mb, st); hip_check(launch_rows_scatter(true, a, st), "born scatter rows"); }
-          a.nterm = par.nref;
-          a.scale = sy2;
-          { PassTimer t("cols_fanout", pb + wb, st); hip_check(launch_cols_fanout(0, a, st), "born pspi columns"); }
-          a.out = du + (size_t)p0 * plane;
-          a.add = 0;
-          a.scale = sx;
-          { PassTimer t("rows_select", wb + pb + pb / 8, st); hip_check(launch_rows_select(a, st), "born pspi rows"); }
-        }
-      ++steps;
+      float2* nxt = keep ? du + n : du;
+      step_fwd(z, background(z, st), dslow + (size_t)z * MZ(), du, nxt, st);
+      du = nxt;
     }
   }
 
   // adjoint (ExtendedBorn.cpp:180-208) or, with image != null, the migration
-  // image I_z = sum_{w,s} conj(U_z) P_z^H lam_{z+1}
-  void adjoint(int add, float2* dslow, float2* image, const float2* data, hipStream_t st) {
+  // image I_z = sum_{w,s} conj(U_z) P_z^H lam_{z+1}.  With mu != null the
+  // down-reflected channel's adjoint source conj(r_z) mu_z joins Inj_z(data).
+  void adjoint(int add, float2* dslow, float2* image, const float2* data, hipStream_t st,
+               const float2* mu = nullptr) {
     if (!slow && !image) throw Error("background slowness not set (call fwix_born_set_slow)");
-    const size_t n = P(), plane = (size_t)d.NX * d.NY;
-    const size_t mz = (size_t)d.NW * d.ny * d.nx;
+    const size_t n = P();
     if (!add) {
-      if (dslow) hip_check(launch_zero(dslow, mz * d.nz, st), "zero model");
+      if (dslow) hip_check(launch_zero(dslow, MZ() * d.nz, st), "zero model");
       if (image) hip_check(launch_zero(image, (size_t)d.ny * d.nx * d.nz, st), "zero image");
     }
     float2* l = (float2*)lam.p;
     hip_check(launch_zero(l, n, st), "zero lambda");
-    const float sx = (float)(1.0 / std::sqrt((double)d.NX)), sy2 = (float)(1.0 / (double)d.NY);
-    const int cw = std::min(eng.chunk, d.NW);
     for (int z = d.nz - 1; z >= 0; --z) {
       if (z + 1 < d.nz) {  // lam_{nz} = 0: nothing to propagate or image at the bottom
         const float2* bg = background(z, st);
@@ -367,38 +416,125 @@ struct BornOp : fwix_op_s {
           eng.apply(true, z, 0, l, l, st);
           hip_check(launch_image(bg, l, image + (size_t)z * d.ny * d.nx, d.NX, d.NY, d.NW, d.NS, d.nx, d.ny, 1, st),
                     "image");
+          ++steps;
         } else {
-          for (int is = 0; is < d.NS; ++is)
-            for (int w0 = 0; w0 < d.NW; w0 += cw) {
-              const int p0 = is * d.NW + w0, np = std::min(cw, d.NW - w0);
-              float2* lc = l + (size_t)p0 * plane;
-              KArgs a = born_args(z, p0, np);
-              a.wstride = (long long)np * plane;
-              const double pb = 8.0 * np * plane, mb = 8.0 * np * d.nx * d.ny;
-              const double wb = 8.0 * d.NX * eng.present_rows(z, p0, np);
-              const int T = par.taylor;
-              // mu = P_z^H lam (PSPI adjoint, written in place) fused with S_z^H's first row pass
-              a.in = lc; a.scale = sx;
-              { PassTimer t("rows_fanout", pb + pb / 8 + wb, st); hip_check(launch_rows_fanout(a, st), "born adjoint pspi rows"); }
-              a.nterm = par.nref; a.scale = sy2;
-              { PassTimer t("cols_fanin", wb + pb, st); hip_check(launch_cols_fanin(1, a, st), "born adjoint pspi columns"); }
-              a.out = lc; a.nterm = T + 1; a.scale = sx;
-              { PassTimer t("born_rows_inv_scatter_adj", (2 + T) * pb + mb, st); hip_check(launch_rows_scatter_adj(true, a, st), "born adjoint inverse + scatter rows"); }
-              if (T >= 1) {
-                a.nterm = T; a.scale = sy2;
-                { PassTimer t("born_cols_kfac_adj", (1 + T) * pb, st); hip_check(launch_cols_fanin(2, a, st), "born adjoint scatter columns"); }
-              }
-              // ds_z += conj(U_z) S_z^H mu
-              a.in = lc; a.bg = bg + (size_t)p0 * plane;
-              a.aux = dslow + (size_t)z * mz;
-              a.add = 1; a.nterm = T + 1; a.scale = sx;
-              { PassTimer t("born_rows_adj_ic", 3 * pb + 3 * mb, st); hip_check(launch_rows_adj_out(true, a, st), "born adjoint imaging"); }
-            }
+          step_adj(z, l, bg, dslow + (size_t)z * MZ(), st);
         }
-        ++steps;
       }
       rec.forward(z, data, l, st);  // adjoint source (ExtendedBorn.cpp:195-196)
+      if (mu) {
+        ReflArgs r = refl_args(z, l, mu + (size_t)z * n, nullptr);
+        hip_check(launch_refl_src(2, r, st), "down-reflected adjoint source");
+      }
     }
+  }
+
+  // point-wise reflection arguments at depth z (slices z and z+1 of the models)
+  ReflArgs refl_args(int z, float2* out, const float2* U, const float2* dUz) const {
+    ReflArgs r{};
+    const size_t mz = MZ();
+    const int z1 = z + 1 < d.nz ? z + 1 : z;
+    r.out = out; r.U = U; r.dU = dUz;
+    r.s0 = slow + (size_t)z * mz; r.s1 = slow + (size_t)z1 * mz;
+    r.r0 = den + (size_t)z * mz; r.r1 = den + (size_t)z1 * mz;
+    r.NX = d.NX; r.NY = d.NY; r.NW = d.NW; r.NS = d.NS; r.nx = d.nx; r.ny = d.ny;
+    r.last = z + 1 >= d.nz;
+    return r;
+  }
+  void need_up() const {
+    if (!den) throw Error("density not set (call fwix_born_set_density)");
+    if (!slow) throw Error("background slowness not set (call fwix_born_set_slow)");
+  }
+  void alloc_stores(bool up) {
+    const size_t bytes = P() * sizeof(float2) * (size_t)d.nz;
+    if (up && vstore.n != bytes) vstore.alloc(bytes);
+    if (xstore.n != bytes) xstore.alloc(bytes);
+  }
+
+  // Up pass of the nonlinear modeling (Propagator.cpp:181-211): V_nz = 0;
+  // V_z = P_z V_{z+1} + r_z U_z with r from (slowness, density); every V_z is
+  // kept in HBM (the up-scattering channel's background); data += Rec_z^T V_z.
+  void model_up(float2* data, int add, hipStream_t st) {
+    need_up();
+    if (!have_bg) throw Error("background wavefield not modeled (call fwix_born_model)");
+    const size_t n = P();
+    const size_t bytes = n * sizeof(float2) * (size_t)d.nz;
+    if (vstore.n != bytes) vstore.alloc(bytes);
+    if (data && !add) hip_check(launch_zero(data, (size_t)nrec * d.NW, st), "zero data");
+    float2* V = (float2*)vstore.p;
+    for (int z = d.nz - 1; z >= 0; --z) {
+      float2* vz = V + (size_t)z * n;
+      if (z + 1 < d.nz) {
+        eng.apply(false, z, 0, vz + n, vz, st);
+        ++steps;
+      } else {
+        hip_check(launch_zero(vz, n, st), "zero V");
+      }
+      ReflArgs r = refl_args(z, vz, background(z, st), nullptr);
+      hip_check(launch_refl_src(0, r, st), "reflect");
+      if (data) rec.adjoint(z, data, vz, st);
+    }
+    have_up = true;
+  }
+
+  // The four channels of ExtendedBorn::forward (ExtendedBorn.cpp:80-118) with
+  // model (ds, drho): down scattering (channel 1, keeping every dU_z), then one
+  // up-going recursion carrying the other three together, which is possible
+  // because they are linear and share the same propagator and recording:
+  //   T_nz = 0;  T_z = P_z[ T_{z+1} + S_z(V_{z+1} ds_{z+1}) ] + r_z dU_z + U_z L_z(ds, drho)
+  // (up scattering :282-311 with its z-1 operators, down-reflected :210-238,
+  // back scattering :352-389), data += Rec_z^T T_z.
+  void ext_forward(int add, const float2* ds, const float2* dr, float2* data, hipStream_t st) {
+    need_up();
+    if (!have_up) throw Error("up-going background not modeled (call fwix_born_model_up)");
+    alloc_stores(false);
+    const size_t n = P(), mz = MZ();
+    forward(add, ds, data, st, true);
+    float2* T = (float2*)dU.p;
+    const float2* V = (const float2*)vstore.p;
+    const float2* dus = (const float2*)xstore.p;
+    hip_check(launch_zero(T, n, st), "zero T");
+    for (int z = d.nz - 1; z >= 0; --z) {
+      if (z + 1 < d.nz) step_fwd(z, V + (size_t)(z + 1) * n, ds + (size_t)(z + 1) * mz, T, T, st);
+      ReflArgs r = refl_args(z, T, background(z, st), dus + (size_t)z * n);
+      const int z1 = z + 1 < d.nz ? z + 1 : z;
+      r.ds0 = ds + (size_t)z * mz; r.ds1 = ds + (size_t)z1 * mz;
+      r.dr0 = dr + (size_t)z * mz; r.dr1 = dr + (size_t)z1 * mz;
+      hip_check(launch_refl_src(1, r, st), "reflection sources");
+      rec.adjoint(z, data, T, st);
+    }
+  }
+
+  // Adjoint (ExtendedBorn.cpp:122-150): the up-going residual
+  // mu_z = Inj_z(data) + P_{z-1}^H mu_{z-1} (kept for every depth) feeds the
+  // up-scattering adjoint (ds_z += conj(V_z) S_{z-1}^H P_{z-1}^H mu_{z-1}) and
+  // the back-scattering adjoint (L_z^H of sum_s conj(U_z) mu_z); then the
+  // down-scattering adjoint runs with the extra source conj(r_z) mu_z of the
+  // down-reflected channel.
+  void ext_adjoint(int add, float2* ds, float2* dr, const float2* data, hipStream_t st) {
+    need_up();
+    if (!have_up) throw Error("up-going background not modeled (call fwix_born_model_up)");
+    alloc_stores(false);
+    const size_t n = P(), mz = MZ();
+    if (!add) {
+      hip_check(launch_zero(ds, mz * d.nz, st), "zero model");
+      hip_check(launch_zero(dr, mz * d.nz, st), "zero model");
+    }
+    float2* mu = (float2*)lam.p;
+    float2* mus = (float2*)xstore.p;
+    const float2* V = (const float2*)vstore.p;
+    hip_check(launch_zero(mu, n, st), "zero mu");
+    for (int z = 0; z < d.nz; ++z) {
+      if (z > 0) step_adj(z - 1, mu, V + (size_t)z * n, ds + (size_t)z * mz, st);
+      rec.forward(z, data, mu, st);
+      hip_check(launch_axpy(mu, mus + (size_t)z * n, n, 1.f, 0, st), "keep mu");
+      ReflArgs r = refl_args(z, nullptr, background(z, st), mu);
+      const int z1 = z + 1 < d.nz ? z + 1 : z;
+      r.gs0 = ds + (size_t)z * mz; r.gs1 = ds + (size_t)z1 * mz;
+      r.gr0 = dr + (size_t)z * mz; r.gr1 = dr + (size_t)z1 * mz;
+      hip_check(launch_refl_adj(r, st), "back-scattering adjoint");
+    }
+    adjoint(1, ds, nullptr, data, st, mus);
   }
 
   // Up-going reflection recording of the background (Propagator.cpp:185-208):
@@ -642,12 +778,23 @@ int fwix_born_set_refs(fwix_op op, const fwix_c64* sref, const uint8_t* labels) 
     auto* p = as<BornOp>(op, KIND_BORN);
     p->eng.set_refs((const float2*)sref, labels);
     p->have_bg = false;
+    p->have_up = false;
   });
 }
 int fwix_born_set_slow(fwix_op op, const fwix_c64* slow_dev) {
   return guard([&] {
     if (!slow_dev) throw Error("null slowness");
-    as<BornOp>(op, KIND_BORN)->slow = (const float2*)slow_dev;
+    auto* p = as<BornOp>(op, KIND_BORN);
+    p->slow = (const float2*)slow_dev;
+    p->have_up = false;
+  });
+}
+int fwix_born_set_density(fwix_op op, const fwix_c64* den_dev) {
+  return guard([&] {
+    if (!den_dev) throw Error("null density");
+    auto* p = as<BornOp>(op, KIND_BORN);
+    p->den = (const float2*)den_dev;
+    p->have_up = false;
   });
 }
 int fwix_born_set_sources(fwix_op op, int n, const float* x, const float* y, const float* z, const int* ids) {
@@ -656,6 +803,7 @@ int fwix_born_set_sources(fwix_op op, int n, const float* x, const float* y, con
     p->src.build(p->d, p->d.nz, n, x, y, z, ids);
     p->nsrc = n;
     p->have_bg = false;
+    p->have_up = false;
   });
 }
 int fwix_born_set_receivers(fwix_op op, int n, const float* x, const float* y, const float* z, const int* ids) {
@@ -671,10 +819,32 @@ int fwix_born_set_wavelet(fwix_op op, const fwix_c64* wavelet_dev) {
     auto* p = as<BornOp>(op, KIND_BORN);
     p->wavelet = (const float2*)wavelet_dev;
     p->have_bg = false;
+    p->have_up = false;
   });
 }
 int fwix_born_model(fwix_op op, int add, fwix_c64* data, fwix_stream s) {
-  return guard([&] { as<BornOp>(op, KIND_BORN)->model((float2*)data, add, S(s)); });
+  return guard([&] {
+    auto* p = as<BornOp>(op, KIND_BORN);
+    p->model((float2*)data, add, S(s));
+    p->have_up = false;
+  });
+}
+int fwix_born_model_up(fwix_op op, int add, fwix_c64* data, fwix_stream s) {
+  return guard([&] { as<BornOp>(op, KIND_BORN)->model_up((float2*)data, add, S(s)); });
+}
+int fwix_born_extended_forward(fwix_op op, int add, const fwix_c64* dslow, const fwix_c64* dden, fwix_c64* data,
+                               fwix_stream s) {
+  return guard([&] {
+    if (!dslow || !dden || !data) throw Error("null model or data");
+    as<BornOp>(op, KIND_BORN)->ext_forward(add, (const float2*)dslow, (const float2*)dden, (float2*)data, S(s));
+  });
+}
+int fwix_born_extended_adjoint(fwix_op op, int add, fwix_c64* dslow, fwix_c64* dden, const fwix_c64* data,
+                               fwix_stream s) {
+  return guard([&] {
+    if (!dslow || !dden || !data) throw Error("null model or data");
+    as<BornOp>(op, KIND_BORN)->ext_adjoint(add, (float2*)dslow, (float2*)dden, (const float2*)data, S(s));
+  });
 }
 int fwix_born_forward(fwix_op op, int add, const fwix_c64* dslow, fwix_c64* data, fwix_stream s) {
   return guard([&] { as<BornOp>(op, KIND_BORN)->forward(add, (const float2*)dslow, (float2*)data, S(s)); });

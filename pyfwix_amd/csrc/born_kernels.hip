@@ -372,6 +372,78 @@ __global__ __launch_bounds__(kThreads) void k_reflect_coef(const float2* __restr
   }
 }
 
+// Reflection coefficient and linearized reflection at one model point, from
+// (s, rho) at depths z and z+1 (reflect.cu:42-45, lin_reflect.cu:45-90):
+// r = (rho1 q0 - rho0 q1) / D, D = rho1 q0 + rho0 q1, q = sqrt(s);
+// dr = a ds0 - b ds1 + c0 drho1 - c1 drho0 with A = rho0 rho1 / D^2,
+// B = 2 q0 q1 / D^2, a = A q1/q0, b = A q0/q1, c0 = rho0 B, c1 = rho1 B
+// (the exact derivative of r; the reference kernel's slowness terms use the
+// square root of q1/q0).
+struct ReflCoefs {
+  float2 r, a, b, c0, c1;
+  __device__ __forceinline__ ReflCoefs(float2 s0, float2 s1, float2 rho0, float2 rho1) {
+    const float2 q0 = csqrt_principal(s0), q1 = csqrt_principal(s1);
+    const float2 u = cmul(rho1, q0), v = cmul(rho0, q1);
+    const float2 D = cadd(u, v);
+    r = cdiv(csub(u, v), D);
+    const float2 iD = cdiv(make_float2(1.f, 0.f), D), iD2 = cmul(iD, iD);
+    const float2 A = cmul(cmul(rho0, rho1), iD2);
+    const float2 B = cscale(cmul(cmul(q0, q1), iD2), 2.f);
+    const float2 c = cdiv(q1, q0);
+    a = cmul(A, c);
+    b = cdiv(A, c);
+    c0 = cmul(rho0, B);
+    c1 = cmul(rho1, B);
+  }
+};
+// MODE 0: out += r U (background up pass); MODE 1: out += r dU + U L(ds, drho)
+// (down-reflected + back-scattering sources); MODE 2: out += conj(r) U
+// (the channel-2 adjoint source, U = mu_z).  One thread per model point,
+// looping over shots.
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_refl_src(ReflArgs a) {
+  const size_t n = (size_t)a.nx * a.ny * a.NW;
+  for (size_t t = (size_t)blockIdx.x * kThreads + threadIdx.x; t < n; t += (size_t)gridDim.x * kThreads) {
+    if (a.last) return;
+    const int ix = (int)(t % a.nx);
+    const int iy = (int)((t / a.nx) % a.ny);
+    const int iw = (int)(t / ((size_t)a.nx * a.ny));
+    const ReflCoefs c(a.s0[t], a.s1[t], a.r0[t], a.r1[t]);
+    float2 L = czero();
+    if constexpr (MODE == 1)
+      L = csub(cadd(cmul(c.a, a.ds0[t]), cmul(c.c0, a.dr1[t])), cadd(cmul(c.b, a.ds1[t]), cmul(c.c1, a.dr0[t])));
+    for (int is = 0; is < a.NS; ++is) {
+      const size_t idx = (((size_t)is * a.NW + iw) * a.NY + iy) * a.NX + ix;
+      float2 y = a.out[idx];
+      if constexpr (MODE == 0) y = cadd(y, cmul(c.r, a.U[idx]));
+      else if constexpr (MODE == 1) y = cadd(y, cadd(cmul(c.r, a.dU[idx]), cmul(a.U[idx], L)));
+      else y = cadd(y, cmulc(a.U[idx], c.r));
+      a.out[idx] = y;
+    }
+  }
+}
+// adjoint of the back-scattering source: g = sum_s conj(U) mu at a model point,
+// gs0 += conj(a) g, gs1 -= conj(b) g, gr1 += conj(c0) g, gr0 -= conj(c1) g
+__global__ __launch_bounds__(kThreads) void k_refl_adj(ReflArgs a) {
+  const size_t n = (size_t)a.nx * a.ny * a.NW;
+  for (size_t t = (size_t)blockIdx.x * kThreads + threadIdx.x; t < n; t += (size_t)gridDim.x * kThreads) {
+    if (a.last) return;
+    const int ix = (int)(t % a.nx);
+    const int iy = (int)((t / a.nx) % a.ny);
+    const int iw = (int)(t / ((size_t)a.nx * a.ny));
+    float2 g = czero();
+    for (int is = 0; is < a.NS; ++is) {
+      const size_t idx = (((size_t)is * a.NW + iw) * a.NY + iy) * a.NX + ix;
+      g = cadd(g, cmulc(a.dU[idx], a.U[idx]));  // dU carries mu here
+    }
+    const ReflCoefs c(a.s0[t], a.s1[t], a.r0[t], a.r1[t]);
+    a.gs0[t] = cadd(a.gs0[t], cmulc(g, c.a));
+    a.gs1[t] = csub(a.gs1[t], cmulc(g, c.b));
+    a.gr1[t] = cadd(a.gr1[t], cmulc(g, c.c0));
+    a.gr0[t] = csub(a.gr0[t], cmulc(g, c.c1));
+  }
+}
+
 // ------------------------------------------------------------------ point-wise Born helpers
 // img (+)= sum_{s, w} conj(U) * R over the model region: one thread per (y, x),
 // fixed summation order (the reduction of the migration image).
@@ -511,6 +583,17 @@ hipError_t launch_rows_adj_out(bool ic, const KArgs& a, hipStream_t st) {
     else hipLaunchKernelGGL((k_rows_adj_out<N, false>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
     return hipGetLastError();
   });
+}
+hipError_t launch_refl_src(int mode, const ReflArgs& a, hipStream_t st) {
+  const dim3 g = bflat((long long)a.nx * a.ny * a.NW);
+  if (mode == 0) hipLaunchKernelGGL(k_refl_src<0>, g, dim3(kThreads), 0, st, a);
+  else if (mode == 1) hipLaunchKernelGGL(k_refl_src<1>, g, dim3(kThreads), 0, st, a);
+  else hipLaunchKernelGGL(k_refl_src<2>, g, dim3(kThreads), 0, st, a);
+  return hipGetLastError();
+}
+hipError_t launch_refl_adj(const ReflArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_refl_adj, bflat((long long)a.nx * a.ny * a.NW), dim3(kThreads), 0, st, a);
+  return hipGetLastError();
 }
 hipError_t launch_reflect_coef(const float2* slow, const float2* den, float2* r, int nz, size_t slice, hipStream_t st) {
   hipLaunchKernelGGL(k_reflect_coef, bflat((long long)nz * (long long)slice), dim3(kThreads), 0, st, slow, den, r, nz,

@@ -48,6 +48,22 @@ hipError_t launch_rows_ic_fwd(const KArgs& a, hipStream_t st);
 hipError_t launch_rows_scatter(bool born, const KArgs& a, hipStream_t st);
 hipError_t launch_rows_scatter_adj(bool fused, const KArgs& a, hipStream_t st);
 hipError_t launch_rows_adj_out(bool ic, const KArgs& a, hipStream_t st);
+// Reflection sources of the up-going channels (point-wise on the model region)
+struct ReflArgs {
+  float2* out;              // wavefield (NS, NW, NY, NX): out (+)= source
+  const float2* U;          // background U_z
+  const float2* dU;         // channel-1 wavefield dU_z (MODE 1)
+  const float2* s0; const float2* s1;    // slowness slices z, z+1 (nw, ny, nx)
+  const float2* r0; const float2* r1;    // density slices z, z+1
+  const float2* ds0; const float2* ds1;  // slowness perturbation slices z, z+1
+  const float2* dr0; const float2* dr1;  // density perturbation slices z, z+1
+  float2* gs0; float2* gs1;              // adjoint: slowness gradient slices z, z+1
+  float2* gr0; float2* gr1;              // adjoint: density gradient slices z, z+1
+  int NX, NY, NW, NS, nx, ny;
+  int last;                 // z = nz-1: the slice is paired with itself (r = 0, L = 0)
+};
+hipError_t launch_refl_src(int mode, const ReflArgs& a, hipStream_t st);
+hipError_t launch_refl_adj(const ReflArgs& a, hipStream_t st);
 hipError_t launch_reflect_coef(const float2* slow, const float2* den, float2* r, int nz, size_t slice, hipStream_t st);
 hipError_t launch_image(const float2* U, const float2* R, float2* img, int NX, int NY, int NW, int NS, int nx,
                         int ny, int add, hipStream_t st);

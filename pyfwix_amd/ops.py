@@ -367,6 +367,36 @@ class Born:
         call("fwix_born_migrate", self._h.h, int(bool(add)), ptr(image), ptr(data), stream())
         return image
 
+    def set_density(self, den):
+        d = self.dims
+        _check(den, (d.nz, d.NW, d.ny, d.nx), name="density")
+        self._keep["den"] = den
+        call("fwix_born_set_density", self._h.h, ptr(den))
+
+    def model_up(self, data: torch.Tensor | None = None, add: bool = False):
+        """Up pass of the nonlinear modeling (``Propagator.cpp:181-211``); keeps every V_z."""
+        if data is not None:
+            _check(data, (self.nrec, self.dims.NW), name="data")
+        call("fwix_born_model_up", self._h.h, int(bool(add)), ptr(data), stream())
+        return data
+
+    def extended_forward(self, add, dslow, dden, data):
+        """All four ExtendedBorn channels (``ExtendedBorn.cpp:80-118``)."""
+        d = self.dims
+        for t, nm in ((dslow, "dslow"), (dden, "dden")):
+            _check(t, (d.nz, d.NW, d.ny, d.nx), name=nm)
+        _check(data, (self.nrec, d.NW), name="data")
+        call("fwix_born_extended_forward", self._h.h, int(bool(add)), ptr(dslow), ptr(dden), ptr(data), stream())
+        return data
+
+    def extended_adjoint(self, add, dslow, dden, data):
+        d = self.dims
+        for t, nm in ((dslow, "dslow"), (dden, "dden")):
+            _check(t, (d.nz, d.NW, d.ny, d.nx), name=nm)
+        _check(data, (self.nrec, d.NW), name="data")
+        call("fwix_born_extended_adjoint", self._h.h, int(bool(add)), ptr(dslow), ptr(dden), ptr(data), stream())
+        return dslow, dden
+
     def reflect_forward(self, add, refl, data):
         """data (+)= up-going recording of refl * U_z (``Propagator.cpp:185-208``)."""
         d = self.dims

@@ -279,3 +279,40 @@ def test_reflect_coef_matches_oracle(dev):
     rho = rng.uniform(1.0, 2.5, v.shape) * (1 + 0.01j * rng.standard_normal(v.shape))
     r = reflect_coef(dims_of(g), _t(s, dev), _t(rho, dev))
     assert rel(r.cpu().numpy(), O.reflect_coef(s.astype(np.complex64), rho.astype(np.complex64))) < 1e-5
+
+
+@pytest.mark.parametrize("ck", [1, 3])
+def test_extended_born_four_channels_match_oracle(dev, ck):
+    """Nonlinear up pass and ExtendedBorn with all four channels (ExtendedBorn.cpp:80-150)."""
+    import torch
+    g, born, slow, sref, labels, src, rec, wav, rng = _born_setup(dev, ckpt=ck, seed=23)
+    # surface receivers: the up-going channels are what they record
+    rec = (rec[0], rec[1], np.zeros_like(rec[2]), rec[3])
+    born.set_receivers(*rec)
+    s64, lab, sl = sref.astype(np.complex128), labels.astype(np.int64), slow.astype(np.complex64).astype(np.complex128)
+    den = (1.0 + 0.4 * rng.random(slow.shape)) * (1 + 0.01j * rng.standard_normal(slow.shape))
+    den = den.astype(np.complex64)
+    born.set_density(_t(den, dev))
+    born.model()
+    dup = _z((born.nrec, g.NW), dev)
+    born.model_up(dup)
+    _, bg = O.model_background(wav, src, rec, s64, lab, g, g.dz, 0.0, 4, 4)
+    d64 = den.astype(np.complex128)
+    ref_up, vbg = O.model_up(bg, sl, d64, rec, s64, lab, g, g.dz, 0.0, 4, 4)
+    assert rel(dup.cpu().numpy(), ref_up) < TOL_WFLD
+    ds = crand(rng, (g.nz, g.NW, g.ny, g.nx)) * 1e-8
+    dr = crand(rng, ds.shape) * 0.05
+    d = _z((born.nrec, g.NW), dev)
+    born.extended_forward(False, _t(ds, dev), _t(dr, dev), d)
+    ref_d = O.extended_born_forward(ds, dr, bg, vbg, sl, d64, rec, s64, lab, g, g.dz, 0.0, 4, 4, 0.04, 1)
+    assert rel(d.cpu().numpy(), ref_d) < TOL_WFLD
+    dd = crand(rng, (born.nrec, g.NW))
+    ms, mr = _z(ds.shape, dev), _z(ds.shape, dev)
+    born.extended_adjoint(False, ms, mr, _t(dd, dev))
+    ref_s, ref_r = O.extended_born_adjoint(dd, bg, vbg, sl, d64, rec, s64, lab, g, g.dz, 0.0, 4, 4, 0.04, 1)
+    assert rel(ms.cpu().numpy(), ref_s) < TOL_WFLD
+    assert rel(mr.cpu().numpy(), ref_r) < TOL_WFLD
+    lhs = np.vdot(d.cpu().numpy().astype(complex), dd)
+    rhs = np.vdot(ds, ms.cpu().numpy()) + np.vdot(dr, mr.cpu().numpy())
+    assert abs((lhs / rhs).real - 1) < TOL_DOT
+    torch.cuda.synchronize()

@@ -305,3 +305,52 @@ def test_reflect_channel_single_reflector_and_dot():
     assert dot_err(m, Am, dd, Atd) < 1e-12
     img = O.reflect_migrate(dd, bg, rec, sref, labels, g, g.dz, 0.0, 2, 2)
     assert rel(img, Atd.sum(axis=1)) < 1e-14 and shape == (g.NS, g.NW, g.NY, g.NX)
+
+
+def test_lin_reflect_is_the_derivative_of_reflect_coef():
+    rng = np.random.default_rng(11)
+    shape = (4, 2, 3, 5)
+    s = (1.0 / rng.uniform(1500, 4000, shape) ** 2) * (1 - 0.02j * rng.random(shape))
+    rho = rng.uniform(1.0, 2.5, shape) * (1 + 0.01j * rng.standard_normal(shape))
+    ds = crand(rng, shape) * 1e-8
+    dr = crand(rng, shape) * 0.1
+    h = 1e-4
+    fd = (O.reflect_coef(s + h * ds, rho + h * dr) - O.reflect_coef(s - h * ds, rho - h * dr)) / (2 * h)
+    L = O.lin_reflect(ds, dr, s, rho)
+    assert np.max(np.abs(L - fd)) < 1e-8 * np.max(np.abs(fd))
+    # the reference kernel's exponent (lin_reflect.cu:69-79) is not the derivative
+    L4 = O.lin_reflect(ds, dr, s, rho, ds_power=0.5)
+    assert np.max(np.abs(L4 - fd)) > 1e-3 * np.max(np.abs(fd))
+    # adjoint of the point-wise operator
+    gz = crand(rng, shape)
+    a_s, a_r = O.lin_reflect_adjoint(gz, s, rho)
+    lhs = np.vdot(L, gz)
+    rhs = np.vdot(ds, a_s) + np.vdot(dr, a_r)
+    assert abs(lhs / rhs - 1) < 1e-12
+
+
+def test_extended_born_four_channels_dot_and_up_pass():
+    rng = np.random.default_rng(12)
+    g = grid(16, 12, 2, 1, nx=12, ny=9, nz=4, ow=6.0, dw=3.0)
+    slow, sref, labels = _refs(g, rng, 2)
+    den = 1.0 + 0.3 * rng.random(slow.shape) + 0j
+    src = (np.array([55.0], np.float32), np.array([45.0], np.float32), np.array([2.0], np.float32), np.array([0]))
+    rx, ry = np.meshgrid(np.arange(1, 11) * 10.0, np.arange(1, 8) * 10.0)
+    rec = (rx.ravel().astype(np.float32), ry.ravel().astype(np.float32), np.zeros(rx.size, np.float32),
+           np.zeros(rx.size, int))
+    wav = crand(rng, (1, g.NW))
+    _, bg = O.model_background(wav, src, rec, sref, labels, g, g.dz, 0.0, 2, 2)
+    dup, vbg = O.model_up(bg, slow, den, rec, sref, labels, g, g.dz, 0.0, 2, 2)
+    r = O.reflect_coef(slow, den)
+    assert rel(dup, O.reflect_forward(r, bg, rec, sref, labels, g, g.dz, 0.0, 2, 2)) < 1e-14
+    ds = crand(rng, (g.nz, g.NW, g.ny, g.nx))
+    dr = crand(rng, ds.shape)
+    dd = crand(rng, (rx.size, g.NW))
+    d = O.extended_born_forward(ds, dr, bg, vbg, slow, den, rec, sref, labels, g, g.dz, 0.0, 2, 2)
+    ms, mr = O.extended_born_adjoint(dd, bg, vbg, slow, den, rec, sref, labels, g, g.dz, 0.0, 2, 2)
+    lhs = np.vdot(d, dd)
+    rhs = np.vdot(ds, ms) + np.vdot(dr, mr)
+    assert abs(lhs / rhs - 1) < 1e-12
+    # surface receivers: channel 1 alone records nothing, the full operator does
+    assert np.linalg.norm(O.born_forward(ds, bg, slow, rec, sref, labels, g, g.dz, 0.0, 2, 2)) == 0
+    assert np.linalg.norm(d) > 0
