@@ -372,8 +372,10 @@ class Upward : public OneWay {
 };
 
 // propagator/src/ExtendedBorn.h + Propagator.cpp: background modeling and the
-// linearized operator of the down-scattering channel, model = ds (nz, NW, ny, nx),
-// data = receiver traces (nrec, NW).  set_* borrow device arrays.
+// linearized operator.  cu_forward / cu_adjoint are the down-scattering
+// channel with model = ds (nz, NW, ny, nx); extended_forward / _adjoint carry
+// all four channels with model (ds, drho).  Data = receiver traces (nrec, NW).
+// set_* borrow device arrays.
 class ExtendedBorn : public CudaOperator {
  public:
   ExtendedBorn(const fwix_dims& d, const fwix_params& p, int nrec, int ckpt_every = 1, DeviceVector* model = nullptr,
@@ -401,6 +403,15 @@ class ExtendedBorn : public CudaOperator {
   void cu_adjoint(bool add, fwix_c64* m, const fwix_c64* d) override { check(fwix_born_adjoint(op_.h, add, m, d, stream_)); }
   void migrate(bool add, fwix_c64* image_dev, const fwix_c64* data_dev) {
     check(fwix_born_migrate(op_.h, add, image_dev, data_dev, stream_));
+  }
+  // density, the nonlinear up pass and the four-channel operator on (ds, drho)
+  void set_density(const fwix_c64* den_dev) { check(fwix_born_set_density(op_.h, den_dev)); }
+  void model_up(fwix_c64* data_dev = nullptr, bool add = false) { check(fwix_born_model_up(op_.h, add, data_dev, stream_)); }
+  void extended_forward(bool add, const fwix_c64* ds, const fwix_c64* drho, fwix_c64* data) {
+    check(fwix_born_extended_forward(op_.h, add, ds, drho, data, stream_));
+  }
+  void extended_adjoint(bool add, fwix_c64* ds, fwix_c64* drho, const fwix_c64* data) {
+    check(fwix_born_extended_adjoint(op_.h, add, ds, drho, data, stream_));
   }
 
  private:

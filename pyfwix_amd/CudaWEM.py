@@ -13,11 +13,9 @@ Differences from the reference classes (kept deliberately):
 * ``Downward.forward(data)`` / ``PSPI.forward(data)`` (one argument)
   propagate in place, ``u <- P u``; the reference's host form accumulates
   into the aliased buffer (``operator/src/CudaOperator.h:141-151``).
-* ``ExtendedBorn`` is the down-scattering channel
-  (``ExtendedBorn.cpp:152-208``); the reflected, up-scattering and
-  back-scattering channels are not part of this build, so the density part
-  of its model gets a zero gradient and does not contribute to the data.
-  ``Propagator.forward`` is the full nonlinear modeling (direct + reflected).
+* ``ExtendedBorn``'s linearized reflection uses the exact derivative of the
+  reflection coefficient (the reference kernel's slowness terms carry the
+  fourth root of s1/s0 instead of the square root, ``lin_reflect.cu:69-79``).
 * ``nbatches`` is accepted for signature compatibility: all shots and
   frequencies run in one launch sequence on one GPU.
 """
@@ -259,15 +257,25 @@ class Propagator(Operator):
         self._model_id = None
 
     def set_background_model(self, model):
+        """Slowness (and density, 1 when absent) of the background, with its
+        reference slownesses and labels (``Propagator.cpp:117-120``)."""
         import torch
         slow = model[0] if isinstance(model, (list, tuple)) else model
+        den = model[1] if isinstance(model, (list, tuple)) and len(model) > 1 else None
         arr = _slow_array(slow)
         sref, labels = _refs_on_device(arr, self.nref, self.dims.NX - self.dims.nx, self.dims.NY - self.dims.ny,
                                        self.device)
         self._slow_dev = torch.from_numpy(arr).to(self.device)
+        self._den_dev = (torch.from_numpy(_slow_array(den)).to(self.device) if den is not None
+                         else torch.ones_like(self._slow_dev))
         self.born.set_refs(sref, labels)
         self.born.set_slow(self._slow_dev)
-        self._model_id = id(slow)
+        self.born.set_density(self._den_dev)
+
+    def model_background(self, data_dev=None, add=False):
+        """Down pass then up pass; every U_z and V_z stays on the GPU."""
+        self.born.model(data_dev, add=add)
+        self.born.model_up(data_dev, add=True if data_dev is not None else False)
 
     def forward(self, add, model, data):
         """Nonlinear modeling (``Propagator.cpp:121-211``): the down-going wavefield
@@ -275,17 +283,11 @@ class Propagator(Operator):
         slowness/density contrasts (``reflect.cu:42-45``) recorded on its way up."""
         import torch
         self.set_background_model(model)
-        slow = model[0] if isinstance(model, (list, tuple)) else model
-        den = model[1] if isinstance(model, (list, tuple)) and len(model) > 1 else None
         out = data.arr if data.on_device else torch.zeros(data.getHyper().shape, dtype=torch.complex64,
                                                              device=self.device)
         if add and not data.on_device:
             out.copy_(torch.from_numpy(data.arr))
-        self.born.model(out, add=bool(add))
-        den_dev = (torch.from_numpy(_slow_array(den)).to(self.device) if den is not None
-                   else torch.ones_like(self._slow_dev))
-        refl = ops.reflect_coef(self.dims, self._slow_dev, den_dev)
-        self.born.reflect_forward(True, refl, out)
+        self.model_background(out, add=bool(add))
         if not data.on_device:
             data.arr[...] = out.cpu().numpy()
 
@@ -293,58 +295,65 @@ class Propagator(Operator):
 class ExtendedBorn(Operator):
     """``fwix/CudaWEM.py:30-50``: ExtendedBorn(model, data, slow_den, propagator).
 
-    Model [ds, drho] (or ds alone) with the slowness axes (extended: one
-    value per frequency); data the recorded traces.  Down-scattering channel
-    (``propagator/src/ExtendedBorn.cpp:152-208``).
+    Model [ds, drho] with the slowness axes (extended: one value per
+    frequency); data the recorded traces.  All four channels of
+    ``propagator/src/ExtendedBorn.cpp:80-150`` around the background of
+    ``slow_den`` = [slowness, density].
     """
 
     def __init__(self, model, data, slow_den, propagator: Propagator):
-        self.setDomainRange(model[0] if isinstance(model, (list, tuple)) else model, data)
+        parts = list(model) if isinstance(model, (list, tuple)) else [model]
+        self.domain = [p.cloneSpace() for p in parts]
+        self.range = data.cloneSpace()
         self.prop = propagator
         propagator.set_background_model(slow_den)
-        propagator.born.model()
+        propagator.model_background()
 
-    @staticmethod
-    def _parts(model):
-        return (model[0], model[1] if len(model) > 1 else None) if isinstance(model, (list, tuple)) else (model, None)
+    def getDomain(self):
+        return self.domain
 
     def _dev(self, v, load=True):
         import torch
-        if v.on_device:
+        if v is not None and v.on_device:
             return v.arr
-        t = torch.empty(v.getHyper().shape, dtype=torch.complex64, device=self.prop.device)
-        if load:
+        shape = self.domain[0].getHyper().shape if v is None else v.getHyper().shape
+        t = torch.zeros(shape, dtype=torch.complex64, device=self.prop.device)
+        if v is not None and load:
             t.copy_(torch.from_numpy(v.arr))
         return t
 
+    @staticmethod
+    def _parts(model):
+        if isinstance(model, (list, tuple)):
+            return model[0], (model[1] if len(model) > 1 else None)
+        return model, None
+
     def forward(self, add, model, data):
-        ds, _ = self._parts(model)
-        m = self._dev(ds)
+        ds, dr = self._parts(model)
         d = self._dev(data, load=add)
-        self.prop.born.forward(bool(add), m, d)
+        self.prop.born.extended_forward(bool(add), self._dev(ds), self._dev(dr), d)
         if not data.on_device:
             data.arr[...] = d.cpu().numpy()
 
     def adjoint(self, add, model, data):
-        ds, drho = self._parts(model)
-        m = self._dev(ds, load=add)
-        d = self._dev(data)
-        self.prop.born.adjoint(bool(add), m, d)
-        if not ds.on_device:
-            ds.arr[...] = m.cpu().numpy()
-        if drho is not None and not add:
-            drho.zero()
+        ds, dr = self._parts(model)
+        ms, mr = self._dev(ds, load=add), self._dev(dr, load=add)
+        self.prop.born.extended_adjoint(bool(add), ms, mr, self._dev(data))
+        for v, t in ((ds, ms), (dr, mr)):
+            if v is not None and not v.on_device:
+                v.arr[...] = t.cpu().numpy()
 
     def dotTest(self, verbose=False, rng=None):
         rng = rng if rng is not None else np.random.default_rng()
-        m = self.domain.cloneSpace().random(rng)
+        m = [p.cloneSpace().random(rng) for p in self.domain]
         d = self.range.cloneSpace().random(rng)
-        d1, m1 = self.range.cloneSpace(), self.domain.cloneSpace()
+        d1, m1 = self.range.cloneSpace(), [p.cloneSpace() for p in self.domain]
         errs = []
         for add in (False, True):
             self.forward(add, m, d1)
             self.adjoint(add, m1, d)
-            errs.append(abs((d.dot(d1).conjugate() / m.dot(m1)).real - 1.0))
+            rhs = sum(a.dot(b) for a, b in zip(m, m1))
+            errs.append(abs((d.dot(d1).conjugate() / rhs).real - 1.0))
             if verbose:
                 print(f"add={add}: error {errs[-1]:.3e}")
         return tuple(errs)

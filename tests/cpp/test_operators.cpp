@@ -159,6 +159,30 @@ int main(int argc, char** argv) {
       op.set_wavelet(wav.data());
       op.model();
       report("ExtendedBorn", op.dotTest(false, 12));
+      // all four channels on (ds, drho): dot test on the device
+      std::vector<cfloat> den(slow.size(), cfloat(1.f, 0.f));
+      for (size_t i = 0; i < den.size(); ++i) den[i] = cfloat(1.f + 0.3f * (float)((i * 40503u) % 97) / 97.f, 0.f);
+      DeviceVector den_d(den.size());
+      den_d.upload(den, s);
+      op.set_density(den_d.data());
+      op.model_up();
+      const size_t nm = model_size(d), nd = rx.size() * (size_t)d.NW;
+      std::mt19937 g(13);
+      std::uniform_real_distribution<float> u(-1.f, 1.f);
+      hostvec ms(nm), mr(nm), dd(nd);
+      for (auto& v : ms) v = cfloat(u(g), u(g));
+      for (auto& v : mr) v = cfloat(u(g), u(g));
+      for (auto& v : dd) v = cfloat(u(g), u(g));
+      DeviceVector ms_d(nm), mr_d(nm), dd_d(nd), am_d(nd), as_d(nm), ar_d(nm);
+      ms_d.upload(ms, s); mr_d.upload(mr, s); dd_d.upload(dd, s);
+      op.extended_forward(false, ms_d.data(), mr_d.data(), am_d.data());
+      op.extended_adjoint(false, as_d.data(), ar_d.data(), dd_d.data());
+      hostvec am, as, ar;
+      am_d.download(am, s); as_d.download(as, s); ar_d.download(ar, s);
+      const std::complex<double> lhs = dot(am, dd), rhs = dot(ms, as) + dot(mr, ar);
+      const double e = std::abs((lhs / rhs).real() - 1.0);
+      std::printf("%-18s dot error %.2e %s\n", "ExtendedBorn x4", e, e < 1e-5 ? "ok" : "FAIL");
+      if (!(e < 1e-5)) ++failures;
     }
     // errors surface as exceptions, not exit()
     try {

@@ -135,11 +135,15 @@ def test_propagator_and_extended_born(dev):
     ds = complexVector(sl)
     ds.arr[...] = (1e-8 * (rng.standard_normal(ds.arr.shape) + 1j * rng.standard_normal(ds.arr.shape)))
     drho = complexVector(sl)
+    drho.arr[...] = 0.05 * rng.standard_normal(drho.arr.shape)
     born = CudaWEM.ExtendedBorn([ds, drho], data, [slow, den], prop)
     dd = data.cloneSpace()
     born.forward(False, [ds, drho], dd)
-    ref = O.born_forward(ds.arr.astype(np.complex128), bg, slow.arr.astype(np.complex128), rec,
-                         sref.astype(np.complex128), labels.astype(np.int64), g, g.dz, 0.0, 4, 3, 0.04, 1)
+    s64, lab = sref.astype(np.complex128), labels.astype(np.int64)
+    sl64, den64 = slow.arr.astype(np.complex128), den.arr.astype(np.complex128)
+    _, vbg = O.model_up(bg, sl64, den64, rec, s64, lab, g, g.dz, 0.0, 4, 3)
+    ref = O.extended_born_forward(ds.arr.astype(np.complex128), drho.arr.astype(np.complex128), bg, vbg, sl64,
+                                  den64, rec, s64, lab, g, g.dz, 0.0, 4, 3, 0.04, 1)
     assert rel(dd.arr, ref) < TOL_WFLD
     e1, e2 = born.dotTest(rng=rng)
     assert e1 < TOL_DOT and e2 < TOL_DOT
