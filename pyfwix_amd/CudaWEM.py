@@ -16,8 +16,9 @@ Differences from the reference classes (kept deliberately):
 * ``ExtendedBorn``'s linearized reflection uses the exact derivative of the
   reflection coefficient (the reference kernel's slowness terms carry the
   fourth root of s1/s0 instead of the square root, ``lin_reflect.cu:69-79``).
-* ``nbatches`` is accepted for signature compatibility: all shots and
-  frequencies run in one launch sequence on one GPU.
+* ``nbatches`` splits shots and frequency bands as the reference's
+  streaming layer does; the batches run one after another on one GPU.
+* Windows and padding are rounded up to the next supported FFT size.
 """
 from __future__ import annotations
 
@@ -225,71 +226,42 @@ class Propagator(Operator):
     (nx, ny, nw, nz); ``data`` holds the recorded traces (nw, ntrace);
     ``wavelet`` the source traces (nw, nsrc).  ``par`` needs ``padx``/``pady``
     >= 0 (``Propagator.cpp:61-70``), ``nref``, and optionally
-    ``taperx/tapery``, ``eps``, ``eps_scat``, ``taylor``, ``ckpt_every``.
+    ``taperx/tapery``, ``eps``, ``eps_scat``, ``taylor``, ``ginsu_x/ginsu_y``,
+    ``ckpt_every``.  As the reference's front end, it runs through the
+    streaming layer (``pyFWIX.StreamingPropagator``): source batches x
+    frequency bands on ginsu windows of the model.
     """
 
     def __init__(self, model, data, wavelet, par, geometry, nbatches=(1, 1), device=None):
-        import torch
-        self.setDomainRange(model[0] if isinstance(model, (list, tuple)) else model, data)
+        from .streaming import StreamingPropagator
+        slow = model[0] if isinstance(model, (list, tuple)) else model
+        self.setDomainRange(slow, data)
         par = par if isinstance(par, paramObj) else paramObj(par)
-        self.device = torch.device(device or "cuda:0")
-        slow = model[0] if isinstance(model, (list, tuple)) else model
         sh = _hyp(slow)
-        padx, pady = par.getInt("padx", -1), par.getInt("pady", -1)
-        if padx < 0 or pady < 0:
-            raise RuntimeError("Error: negative padx/pady values")
-        s_ids, r_ids, nshot = dense_shot_ids(geometry["s_ids"], geometry["r_ids"])
-        ax = sh.getAxes()
-        self.wfld_hyper = hypercube([type(ax[0])(ax[0].n + padx, ax[0].o, ax[0].d),
-                                     type(ax[1])(ax[1].n + pady, ax[1].o, ax[1].d), ax[2],
-                                     type(ax[2])(nshot, 0.0, 1.0)])
-        self.dims = _dims(self.wfld_hyper, sh)
-        self.nref = par.getInt("nref", 1)
-        self.born = ops.Born(self.dims, nref=self.nref, taperx=par.getInt("taperx", 0),
-                             tapery=par.getInt("tapery", 0), eps=par.getFloat("eps", 0.0),
-                             eps_scat=par.getFloat("eps_scat", 0.04), taylor=par.getInt("taylor", 1),
-                             ckpt_every=par.getInt("ckpt_every", 1))
-        self.born.set_sources(geometry["sx"], geometry["sy"], geometry["sz"], s_ids)
-        self.born.set_receivers(geometry["rx"], geometry["ry"], geometry["rz"], r_ids)
+        axes = tuple((sh.getAxis(i).n, sh.getAxis(i).o, sh.getAxis(i).d) for i in (1, 2, 3, 4))
+        keys = {"padx": -1, "pady": -1, "nref": 1, "taperx": 0, "tapery": 0, "taylor": 1, "ckpt_every": 1}
+        params = {k: par.getInt(k, v) for k, v in keys.items()}
+        params.update({k: par.getFloat(k, v) for k, v in (("eps", 0.0), ("eps_scat", 0.04), ("ginsu_x", 0.0),
+                                                            ("ginsu_y", 0.0))})
         w = wavelet.numpy() if hasattr(wavelet, "numpy") else np.asarray(wavelet)
-        self.born.set_wavelet(torch.from_numpy(np.ascontiguousarray(w, np.complex64)).to(self.device))
+        self.cppMode = StreamingPropagator(axes, w, geometry, params, tuple(nbatches), device)
         self.nbatches = tuple(nbatches)
-        self._model_id = None
 
-    def set_background_model(self, model):
-        """Slowness (and density, 1 when absent) of the background, with its
-        reference slownesses and labels (``Propagator.cpp:117-120``)."""
-        import torch
-        slow = model[0] if isinstance(model, (list, tuple)) else model
-        den = model[1] if isinstance(model, (list, tuple)) and len(model) > 1 else None
-        arr = _slow_array(slow)
-        sref, labels = _refs_on_device(arr, self.nref, self.dims.NX - self.dims.nx, self.dims.NY - self.dims.ny,
-                                       self.device)
-        self._slow_dev = torch.from_numpy(arr).to(self.device)
-        self._den_dev = (torch.from_numpy(_slow_array(den)).to(self.device) if den is not None
-                         else torch.ones_like(self._slow_dev))
-        self.born.set_refs(sref, labels)
-        self.born.set_slow(self._slow_dev)
-        self.born.set_density(self._den_dev)
-
-    def model_background(self, data_dev=None, add=False):
-        """Down pass then up pass; every U_z and V_z stays on the GPU."""
-        self.born.model(data_dev, add=add)
-        self.born.model_up(data_dev, add=True if data_dev is not None else False)
+    @staticmethod
+    def _parts(model):
+        if isinstance(model, (list, tuple)):
+            return _slow_array(model[0]), (_slow_array(model[1]) if len(model) > 1 else None)
+        return _slow_array(model), None
 
     def forward(self, add, model, data):
         """Nonlinear modeling (``Propagator.cpp:121-211``): the down-going wavefield
         recorded at every depth, then the up-going wavefield reflected at the
         slowness/density contrasts (``reflect.cu:42-45``) recorded on its way up."""
-        import torch
-        self.set_background_model(model)
-        out = data.arr if data.on_device else torch.zeros(data.getHyper().shape, dtype=torch.complex64,
-                                                             device=self.device)
-        if add and not data.on_device:
-            out.copy_(torch.from_numpy(data.arr))
-        self.model_background(out, add=bool(add))
-        if not data.on_device:
-            data.arr[...] = out.cpu().numpy()
+        slow, den = self._parts(model)
+        out = data.numpy().copy() if data.on_device else data.arr
+        self.cppMode.forward(add, slow, den, out)
+        if data.on_device:
+            data.arr.copy_(__import__("torch").from_numpy(out))
 
 
 class ExtendedBorn(Operator):
@@ -298,50 +270,45 @@ class ExtendedBorn(Operator):
     Model [ds, drho] with the slowness axes (extended: one value per
     frequency); data the recorded traces.  All four channels of
     ``propagator/src/ExtendedBorn.cpp:80-150`` around the background of
-    ``slow_den`` = [slowness, density].
+    ``slow_den`` = [slowness, density], per batch of the propagator
+    (``pyFWIX.StreamingExtendedBorn``).
     """
 
     def __init__(self, model, data, slow_den, propagator: Propagator):
+        from .streaming import StreamingExtendedBorn
         parts = list(model) if isinstance(model, (list, tuple)) else [model]
         self.domain = [p.cloneSpace() for p in parts]
         self.range = data.cloneSpace()
-        self.prop = propagator
-        propagator.set_background_model(slow_den)
-        propagator.model_background()
+        slow, den = Propagator._parts(slow_den)
+        self.cppMode = StreamingExtendedBorn(propagator.cppMode, slow, den)
 
     def getDomain(self):
         return self.domain
 
-    def _dev(self, v, load=True):
-        import torch
-        if v is not None and v.on_device:
-            return v.arr
-        shape = self.domain[0].getHyper().shape if v is None else v.getHyper().shape
-        t = torch.zeros(shape, dtype=torch.complex64, device=self.prop.device)
-        if v is not None and load:
-            t.copy_(torch.from_numpy(v.arr))
-        return t
-
     @staticmethod
-    def _parts(model):
-        if isinstance(model, (list, tuple)):
-            return model[0], (model[1] if len(model) > 1 else None)
-        return model, None
+    def _host(v, like):
+        if v is None:
+            return np.zeros(like.getHyper().shape, np.complex64)
+        return v.numpy() if v.on_device else v.arr
 
     def forward(self, add, model, data):
-        ds, dr = self._parts(model)
-        d = self._dev(data, load=add)
-        self.prop.born.extended_forward(bool(add), self._dev(ds), self._dev(dr), d)
-        if not data.on_device:
-            data.arr[...] = d.cpu().numpy()
+        parts = list(model) if isinstance(model, (list, tuple)) else [model]
+        ds = self._host(parts[0], self.domain[0])
+        dr = self._host(parts[1] if len(parts) > 1 else None, self.domain[0])
+        out = data.numpy().copy() if data.on_device else data.arr
+        self.cppMode.forward(add, ds, dr, out)
+        if data.on_device:
+            data.arr.copy_(__import__("torch").from_numpy(out))
 
     def adjoint(self, add, model, data):
-        ds, dr = self._parts(model)
-        ms, mr = self._dev(ds, load=add), self._dev(dr, load=add)
-        self.prop.born.extended_adjoint(bool(add), ms, mr, self._dev(data))
-        for v, t in ((ds, ms), (dr, mr)):
-            if v is not None and not v.on_device:
-                v.arr[...] = t.cpu().numpy()
+        parts = list(model) if isinstance(model, (list, tuple)) else [model]
+        ds = parts[0].numpy().copy() if parts[0].on_device else parts[0].arr
+        dr = (parts[1].numpy().copy() if parts[1].on_device else parts[1].arr) if len(parts) > 1 \
+            else np.zeros_like(ds)
+        self.cppMode.adjoint(add, ds, dr, self._host(data, self.range))
+        for v, arr in zip(parts, (ds, dr)):
+            if v.on_device:
+                v.arr.copy_(__import__("torch").from_numpy(arr))
 
     def dotTest(self, verbose=False, rng=None):
         rng = rng if rng is not None else np.random.default_rng()

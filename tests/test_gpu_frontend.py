@@ -29,7 +29,8 @@ def _setup(seed=3, NS=2):
     g = grid(48, 32, 3, NS, nx=40, ny=24, nz=4)
     wf, sl = _hypers(g)
     slow = complexVector(sl, smooth_slowness(rng, g).astype(np.complex64))
-    par = paramObj(nref=3, taperx=4, tapery=3, padx=g.NX - g.nx, pady=g.NY - g.ny)
+    # ginsu margins wider than the model: every batch window is the whole model
+    par = paramObj(nref=3, taperx=4, tapery=3, padx=g.NX - g.nx, pady=g.NY - g.ny, ginsu_x=1e4, ginsu_y=1e4)
     return g, wf, sl, slow, par, rng
 
 
