@@ -201,6 +201,17 @@ int fwix_born_model(fwix_op op, int add, fwix_c64* data, fwix_stream s);
 int fwix_born_forward(fwix_op op, int add, const fwix_c64* dslow, fwix_c64* data, fwix_stream s);
 int fwix_born_adjoint(fwix_op op, int add, fwix_c64* dslow, const fwix_c64* data, fwix_stream s);
 int fwix_born_migrate(fwix_op op, int add, fwix_c64* image, const fwix_c64* data, fwix_stream s);
+/* Checkpoint codec of the background wavefields (the reference stores them
+ * ZFP-compressed at a fixed rate, WavefieldPool.cpp:13-284): 0 = lossless
+ * complex64 (default), 1 = 16-bit block floating point (64 complex samples
+ * share one float scale; 4.06 bytes per sample; |error| <= scale / 65534).
+ * Forward and adjoint read the same decoded fields, so they stay adjoint.
+ * Set before fwix_born_model. */
+int fwix_born_set_codec(fwix_op op, int codec);
+/* The codec on any device array of n complex samples: q holds n short2,
+ * scales (n + 63) / 64 floats. */
+int fwix_bfp16_pack(const fwix_c64* in, void* q, float* scales, size_t n, fwix_stream s);
+int fwix_bfp16_unpack(const void* q, const float* scales, fwix_c64* out, size_t n, fwix_stream s);
 /* Background density (nz, NW, ny, nx), borrowed; needed by the up pass and
  * the extended operator (Propagator.cpp:117-120 set_background_model). */
 int fwix_born_set_density(fwix_op op, const fwix_c64* den_dev);

@@ -34,6 +34,7 @@ __all__ = [
     "model_background", "born_forward", "born_adjoint", "migrate",
     "reflect_coef", "reflect_forward", "reflect_adjoint", "reflect_migrate",
     "lin_reflect", "lin_reflect_adjoint", "model_up", "extended_born_forward", "extended_born_adjoint",
+    "bfp16_pack", "bfp16_unpack",
     "SCATTER_COEF", "set_fft_workers",
 ]
 
@@ -714,3 +715,29 @@ def extended_born_adjoint(data, bg, vbg, slow, den, rec, sref, labels, g: Grid, 
     gz = np.stack([ic_adjoint(mus[z], bg[z], g.nx, g.ny) for z in range(nz)])
     d4s, d4r = lin_reflect_adjoint(gz, slow, den, ds_power)
     return ds + d4s, d4r
+
+
+def bfp16_pack(x):
+    """Fixed-rate checkpoint code, the restatement of the library's codec (its
+    counterpart in the reference is ZFP at a fixed rate, ``WavefieldPool.cpp:51-54``,
+    whose bits are not reproducible without zfp): blocks of 64 complex samples,
+    scale m = max(|re|, |im|) per block, components round(v * (32767 / m)) in int16,
+    all in float32 as the device computes it.  Returns (q (n, 2) int16, scales)."""
+    x = np.asarray(x, np.complex64).ravel()
+    n = x.size
+    nb = (n + 63) // 64
+    pad = np.zeros(nb * 64, np.complex64)
+    pad[:n] = x
+    re = pad.real.reshape(nb, 64)
+    im = pad.imag.reshape(nb, 64)
+    m = np.maximum(np.abs(re), np.abs(im)).max(axis=1).astype(np.float32)
+    inv = np.where(m > 0, np.float32(32767.0) / np.where(m > 0, m, np.float32(1)), np.float32(0)).astype(np.float32)
+    q = np.stack([np.rint(re * inv[:, None]), np.rint(im * inv[:, None])], axis=-1).reshape(nb * 64, 2)[:n]
+    return q.astype(np.int16), m
+
+
+def bfp16_unpack(q, scales):
+    q = np.asarray(q)
+    s = (np.asarray(scales, np.float32) * np.float32(1.0 / 32767.0)).astype(np.float32)
+    sc = np.repeat(s, 64)[:q.shape[0]]
+    return (q[:, 0].astype(np.float32) * sc + 1j * (q[:, 1].astype(np.float32) * sc)).astype(np.complex64)

@@ -98,6 +98,9 @@ _SIGS = {
     "fwix_born_adjoint": [_P, _I, _P, _P, _P],
     "fwix_born_migrate": [_P, _I, _P, _P, _P],
     "fwix_born_set_density": [_P, _P],
+    "fwix_born_set_codec": [_P, _I],
+    "fwix_bfp16_pack": [_P, _P, _P, C.c_size_t, _P],
+    "fwix_bfp16_unpack": [_P, _P, _P, C.c_size_t, _P],
     "fwix_born_model_up": [_P, _I, _P, _P],
     "fwix_born_extended_forward": [_P, _I, _P, _P, _P, _P],
     "fwix_born_extended_adjoint": [_P, _I, _P, _P, _P, _P],

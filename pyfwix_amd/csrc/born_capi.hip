@@ -240,6 +240,9 @@ struct BornOp : fwix_op_s {
   int nsrc = 0, nrec = 0;
   int ckpt = 1;
   DevBuf store, seg, U, dU, lam;
+  int codec = 0;   // checkpoints: 0 lossless complex64, 1 16-bit block floating point
+  DevBuf scales;  // codec 1: one float per 64 samples of every checkpoint
+  DevBuf ubuf;    // codec 1, every depth kept: the decoded U_z
   DevBuf vstore;  // up-going background V_z of every depth (model_up)
   DevBuf xstore;  // dU_z (extended forward) or mu_z (extended adjoint) of every depth
   const float2* den = nullptr;  // (nz, NW, ny, nx) background density
@@ -251,17 +254,50 @@ struct BornOp : fwix_op_s {
   size_t P() const { return wsize(d); }
   float2* bufU() { return (float2*)U.p; }
 
+  size_t nblk() const { return (P() + 63) / 64; }
+  size_t nck() const { return (size_t)((d.nz + ckpt - 1) / ckpt); }
+  // checkpoint k into dst (decoded when the codec is on)
+  void restore(int k, float2* dst, hipStream_t st) {
+    const size_t n = P();
+    if (codec)
+      hip_check(launch_bfp16_unpack((const char*)store.p + (size_t)k * n * 4, (const float*)scales.p + (size_t)k * nblk(),
+                                    dst, n, st), "restore");
+    else
+      hip_check(launch_axpy((const float2*)store.p + (size_t)k * n, dst, n, 1.f, 0, st), "restore");
+  }
+  // (re)allocate the checkpoint store for the codec
+  void set_codec(int c) {
+    if (c != 0 && c != 1) throw Error("codec must be 0 (lossless) or 1 (16-bit block floating point)");
+    codec = c;
+    const size_t n = P();
+    store.alloc(n * (c ? 4 : 8) * nck());
+    if (c) {
+      scales.alloc(nblk() * nck() * sizeof(float));
+      if (ckpt == 1) ubuf.alloc(n * sizeof(float2));
+    } else {
+      scales.release();
+      ubuf.release();
+    }
+    have_bg = false;
+    have_up = false;
+    seg_first = -1;
+  }
+
   // wavefield U_z of the background down-going recursion (Propagator.cpp:155-180)
   const float2* background(int z, hipStream_t st) {
     if (!have_bg) throw Error("background wavefield not modeled (call fwix_born_model)");
     const size_t n = P();
-    if (ckpt == 1) return (const float2*)store.p + (size_t)z * n;
+    if (ckpt == 1) {
+      if (!codec) return (const float2*)store.p + (size_t)z * n;
+      restore(z, (float2*)ubuf.p, st);
+      return (const float2*)ubuf.p;
+    }
     const int s0 = (z / ckpt) * ckpt;
     if (seg_first != s0) {
       // recompute U_{s0..s1-1} from the checkpoint at s0
       const int s1 = std::min(s0 + ckpt, d.nz);
       float2* sg = (float2*)seg.p;
-      hip_check(launch_axpy((const float2*)store.p + (size_t)(s0 / ckpt) * n, sg, n, 1.f, 0, st), "restore");
+      restore(s0 / ckpt, sg, st);
       for (int zz = s0 + 1; zz < s1; ++zz) {
         float2* prev = sg + (size_t)(zz - 1 - s0) * n;
         float2* cur = sg + (size_t)(zz - s0) * n;
@@ -285,8 +321,12 @@ struct BornOp : fwix_op_s {
       src.forward(z, wavelet, u, st);
       if (data) rec.adjoint(z, data, u, st);
       if (z % ckpt == 0) {
-        PassTimer t("checkpoint_store", 16.0 * n, st);
-        hip_check(launch_axpy(u, (float2*)store.p + (size_t)(z / ckpt) * n, n, 1.f, 0, st), "store");
+        PassTimer t("checkpoint_store", (codec ? 12.0 + 4.0 / 64 : 16.0) * n, st);
+        const size_t k = (size_t)(z / ckpt);
+        if (codec)
+          hip_check(launch_bfp16_pack(u, (char*)store.p + k * n * 4, (float*)scales.p + k * nblk(), n, st), "store");
+        else
+          hip_check(launch_axpy(u, (float2*)store.p + k * n, n, 1.f, 0, st), "store");
       }
       if (z + 1 < d.nz) {
         eng.apply(false, z, 0, u, u, st);
@@ -787,6 +827,21 @@ int fwix_born_set_slow(fwix_op op, const fwix_c64* slow_dev) {
     auto* p = as<BornOp>(op, KIND_BORN);
     p->slow = (const float2*)slow_dev;
     p->have_up = false;
+  });
+}
+int fwix_born_set_codec(fwix_op op, int codec) {
+  return guard([&] { as<BornOp>(op, KIND_BORN)->set_codec(codec); });
+}
+int fwix_bfp16_pack(const fwix_c64* in, void* q, float* scales, size_t n, fwix_stream s) {
+  return guard([&] {
+    if ((!in || !q || !scales) && n) throw Error("null buffer");
+    hip_check(launch_bfp16_pack((const float2*)in, q, scales, n, S(s)), "bfp16 pack");
+  });
+}
+int fwix_bfp16_unpack(const void* q, const float* scales, fwix_c64* out, size_t n, fwix_stream s) {
+  return guard([&] {
+    if ((!q || !scales || !out) && n) throw Error("null buffer");
+    hip_check(launch_bfp16_unpack(q, scales, (float2*)out, n, S(s)), "bfp16 unpack");
   });
 }
 int fwix_born_set_density(fwix_op op, const fwix_c64* den_dev) {

@@ -42,6 +42,9 @@ hipError_t launch_select(const float2* m, float2* d, const PwArgs& a, hipStream_
 hipError_t launch_taper(const float2* m, float2* d, const PwArgs& a, hipStream_t st);
 hipError_t launch_axpy(const float2* m, float2* d, size_t n, float alpha, int add, hipStream_t st);
 hipError_t launch_zero(float2* d, size_t n, hipStream_t st);
+// 16-bit block floating point checkpoint codec (64 complex samples per block)
+hipError_t launch_bfp16_pack(const float2* in, void* q, float* scale, size_t n, hipStream_t st);
+hipError_t launch_bfp16_unpack(const void* q, const float* scale, float2* out, size_t n, hipStream_t st);
 
 // Born / imaging / injection kernels (born_kernels.hip)
 hipError_t launch_rows_ic_fwd(const KArgs& a, hipStream_t st);

@@ -17,7 +17,7 @@ from . import _lib
 from ._lib import FWIX_NSPS, FWIX_PSPI, FWIX_PSPI_PLAIN, Handle, call, create, default_params, make_dims
 
 __all__ = ["FFT2d", "PhaseShift", "Selector", "Taper", "OneStep", "Injection", "Scatter",
-           "ImagingCondition", "OneWay", "Born", "reflect_coef", "ptr", "stream",
+           "ImagingCondition", "OneWay", "Born", "reflect_coef", "bfp16_pack", "bfp16_unpack", "ptr", "stream",
            "make_dims", "default_params", "FWIX_PSPI", "FWIX_NSPS", "FWIX_PSPI_PLAIN"]
 
 
@@ -367,6 +367,10 @@ class Born:
         call("fwix_born_migrate", self._h.h, int(bool(add)), ptr(image), ptr(data), stream())
         return image
 
+    def set_codec(self, codec: int) -> None:
+        """Checkpoint codec: 0 lossless, 1 16-bit block floating point (set before model())."""
+        call("fwix_born_set_codec", self._h.h, int(codec))
+
     def set_density(self, den):
         d = self.dims
         _check(den, (d.nz, d.NW, d.ny, d.nx), name="density")
@@ -423,6 +427,21 @@ class Born:
     @property
     def steps(self) -> int:
         return int(_lib.lib.fwix_born_steps(self._h.h))
+
+
+def bfp16_pack(x: torch.Tensor):
+    """16-bit block floating point code of a complex64 device array: (q int16 pairs, scales)."""
+    n = x.numel()
+    q = torch.empty((n, 2), dtype=torch.int16, device=x.device)
+    sc = torch.empty(((n + 63) // 64,), dtype=torch.float32, device=x.device)
+    call("fwix_bfp16_pack", ptr(x.contiguous()), ptr(q), ptr(sc), C.c_size_t(n), stream())
+    return q, sc
+
+
+def bfp16_unpack(q: torch.Tensor, sc: torch.Tensor, shape) -> torch.Tensor:
+    out = torch.empty(shape, dtype=torch.complex64, device=q.device)
+    call("fwix_bfp16_unpack", ptr(q), ptr(sc), ptr(out), C.c_size_t(out.numel()), stream())
+    return out
 
 
 def reflect_coef(dims, slow: torch.Tensor, den: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:

@@ -316,3 +316,39 @@ def test_extended_born_four_channels_match_oracle(dev, ck):
     rhs = np.vdot(ds, ms.cpu().numpy()) + np.vdot(dr, mr.cpu().numpy())
     assert abs((lhs / rhs).real - 1) < TOL_DOT
     torch.cuda.synchronize()
+
+
+def test_bfp16_codec_matches_oracle(dev):
+    import torch
+    from pyfwix_amd.ops import bfp16_pack, bfp16_unpack
+    rng = np.random.default_rng(14)
+    x = (crand(rng, 5000) * np.repeat(10.0 ** rng.uniform(-6, 1, 79), 64)[:5000]).astype(np.complex64)
+    q, sc = bfp16_pack(_t(x, dev))
+    rq, rsc = O.bfp16_pack(x)
+    assert np.array_equal(sc.cpu().numpy(), rsc)
+    assert np.abs(q.cpu().numpy().astype(int) - rq.astype(int)).max() <= 1   # round-half ties aside
+    y = bfp16_unpack(q, sc, x.shape)
+    assert rel(y.cpu().numpy(), O.bfp16_unpack(rq, rsc)) < 1e-6
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("ck", [1, 3])
+def test_born_with_checkpoint_codec(dev, ck):
+    """16-bit checkpoints: Born forward/adjoint within 1e-4 of the lossless ones, still adjoint."""
+    outs = []
+    for codec in (0, 1):
+        g, born, slow, sref, labels, src, rec, wav, rng = _born_setup(dev, ckpt=ck, seed=27)
+        born.set_codec(codec)
+        born.model()
+        rng = np.random.default_rng(3)
+        ds = crand(rng, (g.nz, g.NW, g.ny, g.nx)) * 1e-7
+        dd = crand(rng, (born.nrec, g.NW))
+        d = _z((born.nrec, g.NW), dev)
+        born.forward(False, _t(ds, dev), d)
+        m = _z(ds.shape, dev)
+        born.adjoint(False, m, _t(dd, dev))
+        outs.append((d.cpu().numpy(), m.cpu().numpy()))
+        if codec:
+            assert dot_err(ds, outs[-1][0], dd, outs[-1][1]) < TOL_DOT
+    assert rel(outs[1][0], outs[0][0]) < 1e-4
+    assert rel(outs[1][1], outs[0][1]) < 1e-4

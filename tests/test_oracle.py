@@ -354,3 +354,22 @@ def test_extended_born_four_channels_dot_and_up_pass():
     # surface receivers: channel 1 alone records nothing, the full operator does
     assert np.linalg.norm(O.born_forward(ds, bg, slow, rec, sref, labels, g, g.dz, 0.0, 2, 2)) == 0
     assert np.linalg.norm(d) > 0
+
+
+def test_bfp16_codec_bounds():
+    """The checkpoint codec's error bound, exactness at block maxima, idempotence."""
+    rng = np.random.default_rng(13)
+    x = (crand(rng, 1000) * np.repeat(10.0 ** rng.uniform(-8, 2, 16), 64)[:1000]).astype(np.complex64)
+    q, m = O.bfp16_pack(x)
+    y = O.bfp16_unpack(q, m)
+    assert q.dtype == np.int16 and m.shape == (16,) and np.abs(q).max() <= 32767
+    scale = np.repeat(m, 64)[:1000]
+    err = np.maximum(np.abs(y.real - x.real), np.abs(y.imag - x.imag))
+    assert np.all(err <= scale * (0.5 / 32767 + 2e-7))
+    # the block maximum is reproduced, and coding a decoded field changes nothing
+    blk = np.maximum(np.abs(x.real), np.abs(x.imag))[:960].reshape(15, 64)
+    assert np.allclose(blk.max(axis=1), m[:15], rtol=0, atol=0)
+    q2, m2 = O.bfp16_pack(y)
+    assert np.array_equal(q2, q) and np.allclose(m2, m, rtol=1e-6)
+    q0, m0 = O.bfp16_pack(np.zeros(70, np.complex64))
+    assert np.all(O.bfp16_unpack(q0, m0) == 0)
