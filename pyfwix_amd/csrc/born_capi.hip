@@ -315,11 +315,22 @@ struct BornOp : fwix_op_s {
     if (!eng.sref) throw Error("reference slownesses / labels not set");
     const size_t n = P();
     if (data && !add) hip_check(launch_zero(data, (size_t)nrec * d.NW, st), "zero data");
-    float2* u = bufU();
+    // Lossless checkpoints of every depth: the recursion runs in the store
+    // itself, each step writing U_{z+1} into its slot (no copies).
+    const bool direct = ckpt == 1 && !codec;
+    float2* u = direct ? (float2*)store.p : bufU();
     hip_check(launch_zero(u, n, st), "zero wavefield");
     for (int z = 0; z < d.nz; ++z) {
       src.forward(z, wavelet, u, st);
       if (data) rec.adjoint(z, data, u, st);
+      if (direct) {
+        if (z + 1 < d.nz) {
+          eng.apply(false, z, 0, u, u + n, st);
+          ++steps;
+          u += n;
+        }
+        continue;
+      }
       if (z % ckpt == 0) {
         PassTimer t("checkpoint_store", (codec ? 12.0 + 4.0 / 64 : 16.0) * n, st);
         const size_t k = (size_t)(z / ckpt);
