@@ -73,31 +73,15 @@ __device__ __forceinline__ int next_bit(uint32_t m, int r) {
   return rest ? __builtin_ctz(rest) : 32;
 }
 
-// sin and cos of x for |x| up to ~1e4: Cody-Waite reduction by pi/2 with a
-// three-part constant (1.5703125 + 4.837512969970703e-4 + 7.54979012640433e-8,
-// the first two short enough that n*P is exact), then Taylor polynomials on
-// [-pi/4, pi/4] (truncation below 3e-8).  Much smaller than the libm
-// sincosf, whose large-argument path costs registers in the unrolled
-// per-sample loops of the fused kernels.
+// sin and cos of x (radians) from the transcendental unit: v_sin_f32 /
+// v_cos_f32 take revolutions, so x is scaled by 1/(2 pi) and reduced to
+// [0, 1) with v_fract_f32.  Measured on gfx950 over [-2, 2) revolutions:
+// |error| <= 1.3e-7 against double-precision sin / cos, float rounding level
+// for the phases of a depth step (|kz dz| well below one revolution).
 __device__ __forceinline__ void fast_sincos(float x, float* s, float* c) {
-  const float n = rintf(x * 0.636619772367581343f);
-  float r = fmaf(-n, 1.5703125f, x);
-  r = fmaf(-n, 4.837512969970703e-4f, r);
-  r = fmaf(-n, 7.54979012640433e-8f, r);
-  const float r2 = r * r;
-  float ps = fmaf(r2, 2.75573192e-6f, -1.98412698e-4f);   // 1/9!, -1/7!
-  ps = fmaf(r2, ps, 8.33333333e-3f);                      // 1/5!
-  ps = fmaf(r2, ps, -1.66666667e-1f);                     // -1/3!
-  const float sr = fmaf(r * r2, ps, r);
-  float pc = fmaf(r2, 2.48015873e-5f, -1.38888889e-3f);   // 1/8!, -1/6!
-  pc = fmaf(r2, pc, 4.16666667e-2f);                      // 1/4!
-  pc = fmaf(r2, pc, -0.5f);
-  const float cr = fmaf(r2, pc, 1.f);
-  const int q = (int)n & 3;
-  const float ss = (q & 1) ? cr : sr;
-  const float cc = (q & 1) ? sr : cr;
-  *s = (q & 2) ? -ss : ss;
-  *c = ((q + 1) & 2) ? -cc : cc;
+  const float r = __builtin_amdgcn_fractf(x * 0.159154943091895336f);
+  *s = __builtin_amdgcn_sinf(r);
+  *c = __builtin_amdgcn_cosf(r);
 }
 
 // Phase-shift multiplier exp(-i kz dz) for one reference slowness.
