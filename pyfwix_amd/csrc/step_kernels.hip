@@ -44,10 +44,14 @@ struct ColCfg {
   static constexpr int TILE = N * CWP;
   static constexpr int PRIV = NB * padded(N);
   static constexpr int SMEM = TILE > WAVES * PRIV ? TILE : WAVES * PRIV;
-  // waves per SIMD the register budget is sized for: 4 (two workgroups per
-  // CU, 128 VGPRs) where the column transform and its terms fit in 128
-  // registers, else 2 (256 VGPRs, no spills)
-  static constexpr int WPE = (!size_has3(N) && N <= 512) ? 4 : 2;
+  // Waves per SIMD the register budget is sized for, per size, the largest
+  // that compiles without spills (measured): 6 (three workgroups per CU,
+  // 80 VGPRs) for the forward / fan-out transforms of 2^a <= 128 and 512,
+  // 4 (two workgroups, 128 VGPRs) for 256 and for every fan-in (which
+  // spills at 80), 2 (256 VGPRs) for the 3 * 2^a sizes and 1024.
+  static constexpr bool POW2 = !size_has3(N);
+  static constexpr int WPE = (POW2 && (N <= 128 || N == 512)) ? 6 : (POW2 && N <= 256) ? 4 : 2;
+  static constexpr int WPE_IN = (POW2 && N <= 512) ? 4 : 2;
   static_assert(NB >= 1 && V >= 4, "column tile too narrow");
 };
 
@@ -507,7 +511,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
 
 // cols_fanin: K = F_y^-1[sum_r M_r * F_y(W_r)] / NY (M_r: term_mult<MODE>).
 template <int N, int MODE>
-__global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(ColCfg<N>::WPE))) void k_cols_fanin(KArgs a) {
+__global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(ColCfg<N>::WPE_IN))) void k_cols_fanin(KArgs a) {
   using C = ColCfg<N>;
   using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
   using Fi = WaveFft<N, C::NB, false, true, PadLds<N>>;
