@@ -192,6 +192,191 @@ def plain_vs_fused(w, slow_dims, sref, labels, u, reps=5):
             "speedup": round(out["fused"] / out["plain"], 3)}
 
 
+def bench_shot_batch(args):
+    """Batched multi-shot modeling + migration (64 shots of uneven aperture and
+    frequency count): every launch of the plan runs background modeling + Born
+    forward + Born adjoint for its shots; the same shots one per operator; and
+    a PSPI forward + adjoint step sample, batched fused vs per-shot plain."""
+    import concurrent.futures as cf
+
+    import torch
+    from pyfwix_amd import _lib
+    from pyfwix_amd.batching import ShotBatch, min_ckpt, padded_size, plan_launches, shot_refs
+    from pyfwix_amd.ops import FWIX_PSPI, FWIX_PSPI_PLAIN, OneStep, make_dims
+    from pyfwix_amd.workloads import SHOT_BATCH_CONFIGS, shot_models, uneven_shots
+
+    cfg = SHOT_BATCH_CONFIGS[args.config]
+    dev = torch.device("cuda:0")
+    nz, d, nref, taper = cfg["nz"], cfg["d"], cfg["nref"], cfg["taper"]
+    params = {"nref": nref, "taper": taper, "eps": 0.0, "eps_scat": 0.04, "taylor": 1}
+    shots = uneven_shots(**cfg)
+    if args.shots:
+        shots = shots[:args.shots]
+    budget = int(args.budget_gb * (1 << 30))
+    plan = plan_launches(shots, nz, budget)
+    t_prep = time.perf_counter()
+
+    def prep(k):
+        s = shots[k]
+        s3, ds3 = shot_models(s, nz, d)
+        return s3, ds3, shot_refs(s3, nref, padded_size(s.nx), padded_size(s.ny))
+
+    with cf.ThreadPoolExecutor(8) as ex:
+        models = list(ex.map(prep, range(len(shots))))
+    t_prep = time.perf_counter() - t_prep
+    print(f"[c4] {len(shots)} shots, {len(plan)} launches, host models {t_prep:.1f} s", file=sys.stderr, flush=True)
+    nominal = 3 * (nz - 1)                     # background + forward + adjoint steps per shot slice
+    pts = sum(s.nw * padded_size(s.nx) * padded_size(s.ny) for s in shots)
+
+    def run(launches, timed_passes=False):
+        total, steps = 0.0, 0
+        if timed_passes:
+            _lib.timing_reset()
+            _lib.timing_select(None, max(1, args.pass_sample))
+            _lib.timing_enable(1 << 15)
+        for idx, NX, NY, k in launches:
+            b = ShotBatch([shots[i] for i in idx], nz, d, d, d, params, ckpt_every=k, device=dev, NX=NX, NY=NY)
+            b.set_slowness([models[i][0] for i in idx], [models[i][2] for i in idx])
+            b.set_perturbation([models[i][1] for i in idx])
+            torch.cuda.synchronize()
+            s0 = b.op.steps
+            t0 = time.perf_counter()
+            b.model()
+            data = b.forward()
+            b.adjoint(data)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            total += dt
+            steps += b.op.steps - s0
+            print(f"[c4] launch {NX}x{NY} shots={len(idx)} planes={b.planes} ckpt={k}: {dt:.3f} s",
+                  file=sys.stderr, flush=True)
+            del b, data
+            torch.cuda.empty_cache()
+        passes = {}
+        if timed_passes:
+            _lib.timing_disable()
+            passes = _lib.timing_collect()
+            _lib.timing_select(None, 1)
+        return total, steps, passes
+
+    # warm-up: one small shot, a few depths
+    w0 = ShotBatch([shots[0]], 4, d, d, d, params, device=dev)
+    w0.set_slowness([models[0][0][:4]])
+    w0.set_perturbation([models[0][1][:4]])
+    w0.model()
+    w0.adjoint(w0.forward())
+    torch.cuda.synchronize()
+    del w0
+    t_b, steps_b, passes = run(plan, timed_passes=not args.no_pass_timing)
+    print(f"[c4] batched {t_b:.2f} s", file=sys.stderr, flush=True)
+    alone = [([i], padded_size(shots[i].nx), padded_size(shots[i].ny),
+              min_ckpt(shots[i].nw, 1, padded_size(shots[i].nx), padded_size(shots[i].ny), shots[i].nx,
+                       shots[i].ny, nz, budget)) for i in range(len(shots))]
+    t_s, steps_s, _ = (None, None, None) if args.no_per_shot else run(alone)
+    if t_s is not None:
+        print(f"[c4] per shot {t_s:.2f} s", file=sys.stderr, flush=True)
+
+    # PSPI forward + adjoint step sample (depths 0 and 1): per-shot plain vs batched fused
+    def step_rate(launches, method, batched):
+        total, npts = 0.0, 0
+        for idx, NX, NY, _k in launches:
+            sh = [shots[i] for i in idx]
+            P = sum(s.nw for s in sh)
+            dims = make_dims(NX, NY, P, 1, max(s.nx for s in sh), max(s.ny for s in sh), 0.0, d, 0.0, d,
+                             1.0, 0.0, 0.0, d, 2)
+            op = OneStep(method, dims, nref=nref, taperx=taper, tapery=taper)
+            if batched:
+                op.set_shot_planes([s.nw for s in sh])
+                sref = np.stack([models[i][2][0][:2] for i in idx], 2)
+                lab = np.stack([models[i][2][1][:2] for i in idx], 1)
+            else:
+                sref = np.repeat(models[idx[0]][2][0][:2, :, None], P, 2)
+                lab = np.repeat(models[idx[0]][2][1][:2, None], P, 1)
+            op.set_frequencies(np.concatenate([s.freqs for s in sh]))
+            refs = (torch.from_numpy(np.ascontiguousarray(sref)).to(dev),
+                    torch.from_numpy(np.ascontiguousarray(lab)).to(dev))
+            op.set_refs(*refs)
+            u = torch.randn((1, P, NY, NX), dtype=torch.complex64, device=dev)
+
+            def once():
+                for z in (0, 1):
+                    op.set_depth(z)
+                    op.forward(False, u, u)
+                    op.adjoint(False, u, u)
+            once()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            once()
+            torch.cuda.synchronize()
+            total += time.perf_counter() - t0
+            npts += 4 * P * NX * NY
+            del op, u, refs
+        torch.cuda.empty_cache()
+        return npts / total / 1e9
+
+    sample = None
+    if not args.no_plain:
+        g_plain = step_rate(alone, FWIX_PSPI_PLAIN, False)
+        g_batch = step_rate(plan, FWIX_PSPI, True)
+        sample = {"plain_per_shot_gpts": round(g_plain, 3), "batched_fused_gpts": round(g_batch, 3),
+                  "speedup": round(g_batch / g_plain, 3),
+                  "sample": "PSPI forward + adjoint at depths 0 and 1 of every shot (4 steps per shot)"}
+    roof = None
+    if passes:
+        est = {k: v["ms"] / v["launches"] * v["total_launches"] for k, v in passes.items()}
+        dom_name = max(est, key=est.get)
+        dom = passes[dom_name]
+        avg_us = dom["ms"] * 1e3 / dom["launches"]
+        achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "alg_bytes_per_launch": round(dom["bytes"] / dom["launches"]), "avg_us": round(avg_us, 2),
+                "launches_timed": dom["launches"], "launches": dom["total_launches"],
+                "note": "all padded grid sizes of the plan together"}
+    classes = {}
+    for idx, NX, NY, k in plan:
+        c = classes.setdefault(f"{NX}x{NY}", {"shots": 0, "launches": 0, "planes": 0, "ckpt_every": set()})
+        c["shots"] += len(idx)
+        c["launches"] += 1
+        c["planes"] += sum(shots[i].nw for i in idx)
+        c["ckpt_every"].add(k)
+    for c in classes.values():
+        c["ckpt_every"] = sorted(c["ckpt_every"])
+    value = nominal * pts / t_b / 1e9
+    res = {
+        "metric": "wavefield Gpts*freq/s per depth step (fwd+adj)",
+        "value": round(value, 4),
+        "unit": "Gpts*freq/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": round(t_b * 1e3, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex64",
+        "data": "synthetic (seeded apertures / frequency counts, smooth velocity, three flat reflectors)",
+        "config": {"workload": f"{args.config}: batched multi-shot Born modeling + migration (background "
+                               f"modeling included), {len(shots)} shots",
+                   "nz": nz, "aperture_nodes": [cfg["nmin"], cfg["nmax"]], "freqs_per_shot": [cfg["wmin"], cfg["wmax"]],
+                   "band_hz": [cfg["fmin"], cfg["fmax"]], "nref": nref, "taper": taper,
+                   "receivers": f"every {cfg['rec_step']}th node of the aperture on the deepest plane",
+                   "hbm_budget_gb": args.budget_gb, "launches": len(plan), "size_classes": classes,
+                   "points_per_depth": pts, "depth_steps_counted": f"{nominal} per shot slice (background, "
+                   f"forward, adjoint); recomputed background steps not counted",
+                   "depth_steps_executed_batched": steps_b},
+        "roofline": roof,
+        "per_shot": None if t_s is None else {
+            "value": round(nominal * pts / t_s / 1e9, 4), "seconds": round(t_s, 3), "steps_executed": steps_s,
+            "form": "the same fused kernels, one shot per operator and launch"},
+        "batching_speedup": None if t_s is None else round(t_s / t_b, 3),
+        "vs_plain_per_shot": sample,
+        "host_model_prep_s": round(t_prep, 1),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,6 +392,9 @@ def main():
                     help="planes per launch of the fused passes (0: the library's choice)")
     ap.add_argument("--no-pass-timing", action="store_true",
                     help="no event markers at all (counter-collection runs)")
+    ap.add_argument("--shots", type=int, default=0, help="shot-batch configs: first n shots only (0: all)")
+    ap.add_argument("--budget-gb", type=float, default=160.0, help="shot-batch configs: HBM per launch")
+    ap.add_argument("--no-per-shot", action="store_true", help="shot-batch configs: skip the one-shot-per-launch leg")
     ap.add_argument("--summarize-stats", nargs=2, metavar=("CSV", "OUT"))
     ap.add_argument("--summarize-traffic", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT"))
     ap.add_argument("--command", action="append", default=[], help="profiling command recorded in a summary")
@@ -223,6 +411,11 @@ def main():
               "frequency bands would shard across GPUs with one image all-reduce, which is not built here",
               file=sys.stderr)
         sys.exit(2)
+
+    from pyfwix_amd.workloads import SHOT_BATCH_CONFIGS
+    if args.config in SHOT_BATCH_CONFIGS:
+        bench_shot_batch(args)
+        return
 
     import torch
     from pyfwix_amd import _lib

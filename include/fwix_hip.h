@@ -126,6 +126,17 @@ int fwix_taper_adjoint(fwix_op op, int add, fwix_c64* model, const fwix_c64* dat
 int fwix_onestep_create(int method, const fwix_dims* d, const fwix_params* p, fwix_op* out);  /* any FWIX_* method */
 int fwix_onestep_set_refs(fwix_op op, const fwix_c64* sref_dev, const uint8_t* labels_dev);
 int fwix_onestep_set_depth(fwix_op op, int iz);
+/* Per-plane frequencies: NW values in Hz (host) replacing ow + iw * dw, so
+ * that the planes of one launch may belong to shots with different frequency
+ * sets (shots of one padded size batched along the plane axis).  No
+ * counterpart in the reference, whose operators take a regular axis.  The
+ * oneway and born variants below do the same for those handles. */
+int fwix_onestep_set_frequencies(fwix_op op, const float* f_hz);
+/* Shot batch on the fused path (NS == 1): shot k owns the next nplanes[k]
+ * planes (sum = NW); references (nz, nref, nshot) and labels
+ * (nz, nshot, NY, NX) are then per shot and must be set again.  No
+ * counterpart in the reference (one shot geometry per operator). */
+int fwix_onestep_set_shot_planes(fwix_op op, int nshot, const int* nplanes);
 int fwix_onestep_forward(fwix_op op, int add, const fwix_c64* model, fwix_c64* data, fwix_stream s);
 int fwix_onestep_adjoint(fwix_op op, int add, fwix_c64* model, const fwix_c64* data, fwix_stream s);
 
@@ -176,6 +187,12 @@ int fwix_ic_adjoint(fwix_op op, int add, fwix_c64* img, const fwix_c64* wfld, fw
  * (OneWay::compress_slice, OneWay.h:52-58). */
 int fwix_oneway_create(int direction, int method, const fwix_dims* d, const fwix_params* p, fwix_op* out);
 int fwix_oneway_set_refs(fwix_op op, const fwix_c64* sref_dev, const uint8_t* labels_dev);
+int fwix_oneway_set_frequencies(fwix_op op, const float* f_hz);
+/* Shot batch (NS == 1): shot k owns the next nplanes[k] planes (the counts
+ * add up to NW); references and labels are then per shot, (nz, nref, nshot)
+ * and (nz, nshot, NY, NX), and must be set again.  No counterpart in the
+ * reference, which runs one shot geometry per operator. */
+int fwix_oneway_set_shot_planes(fwix_op op, int nshot, const int* nplanes);
 int fwix_oneway_forward(fwix_op op, int add, const fwix_c64* model, fwix_c64* data, fwix_c64* ckpt, fwix_stream s);
 int fwix_oneway_adjoint(fwix_op op, int add, fwix_c64* model, const fwix_c64* data, fwix_stream s);
 
@@ -193,6 +210,7 @@ int fwix_oneway_adjoint(fwix_op op, int add, fwix_c64* model, const fwix_c64* da
  * before forward / adjoint / migrate; it optionally records the direct data. */
 int fwix_born_create(const fwix_dims* d, const fwix_params* p, int ckpt_every, fwix_op* out);
 int fwix_born_set_refs(fwix_op op, const fwix_c64* sref_dev, const uint8_t* labels_dev);
+int fwix_born_set_frequencies(fwix_op op, const float* f_hz);
 int fwix_born_set_slow(fwix_op op, const fwix_c64* slow_dev);
 int fwix_born_set_sources(fwix_op op, int n, const float* x, const float* y, const float* z, const int* ids);
 int fwix_born_set_receivers(fwix_op op, int n, const float* x, const float* y, const float* z, const int* ids);
@@ -201,6 +219,22 @@ int fwix_born_model(fwix_op op, int add, fwix_c64* data, fwix_stream s);
 int fwix_born_forward(fwix_op op, int add, const fwix_c64* dslow, fwix_c64* data, fwix_stream s);
 int fwix_born_adjoint(fwix_op op, int add, fwix_c64* dslow, const fwix_c64* data, fwix_stream s);
 int fwix_born_migrate(fwix_op op, int add, fwix_c64* image, const fwix_c64* data, fwix_stream s);
+/* Shot batch: many shots of one padded grid share every launch (no
+ * counterpart in the reference, which runs one shot geometry per operator;
+ * StreamingPropagator.cpp:68-153 runs its batches one after another).  With
+ * NS == 1, shot k owns the next nplanes[k] planes (sum = NW) and each plane
+ * has its own frequency (fwix_born_set_frequencies).  Afterwards:
+ *   slowness and model ds (nz, nshot, ny, nx), one slice per shot (shared by
+ *   its planes); references (nz, nref, nshot); labels (nz, nshot, NY, NX);
+ *   the adjoint returns ds summed over the shot's planes;
+ *   trace ids are shots; trace t holds one sample per plane of its shot, the
+ *   traces stored one after another (fwix_born_trace_samples gives the total).
+ * Slowness, references, sources, receivers and wavelet must be set again; the
+ * migration image and the reflection / extended channels are refused. */
+int fwix_born_set_shot_planes(fwix_op op, int nshot, const int* nplanes);
+/* samples of all receiver (receivers != 0) or source traces: nrec * NW
+ * without a shot batch; -1 for an invalid handle */
+long long fwix_born_trace_samples(fwix_op op, int receivers);
 /* Checkpoint codec of the background wavefields (the reference stores them
  * ZFP-compressed at a fixed rate, WavefieldPool.cpp:13-284): 0 = lossless
  * complex64 (default), 1 = 16-bit block floating point (64 complex samples

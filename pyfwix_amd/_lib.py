@@ -67,6 +67,8 @@ _SIGS = {
     "fwix_onestep_create": [_I, _D, _PP, _OPP],
     "fwix_onestep_set_refs": [_P, _P, _P],
     "fwix_onestep_set_depth": [_P, _I],
+    "fwix_onestep_set_frequencies": [_P, _P],
+    "fwix_onestep_set_shot_planes": [_P, _I, _P],
     "fwix_onestep_forward": [_P, _I, _P, _P, _P],
     "fwix_onestep_adjoint": [_P, _I, _P, _P, _P],
     "fwix_injection_create": [_D, _I, _OPP],
@@ -85,10 +87,14 @@ _SIGS = {
     "fwix_ic_adjoint": [_P, _I, _P, _P, _P],
     "fwix_oneway_create": [_I, _I, _D, _PP, _OPP],
     "fwix_oneway_set_refs": [_P, _P, _P],
+    "fwix_oneway_set_frequencies": [_P, _P],
+    "fwix_oneway_set_shot_planes": [_P, _I, _P],
     "fwix_oneway_forward": [_P, _I, _P, _P, _P, _P],
     "fwix_oneway_adjoint": [_P, _I, _P, _P, _P],
     "fwix_born_create": [_D, _PP, _I, _OPP],
     "fwix_born_set_refs": [_P, _P, _P],
+    "fwix_born_set_frequencies": [_P, _P],
+    "fwix_born_set_shot_planes": [_P, _I, _P],
     "fwix_born_set_slow": [_P, _P],
     "fwix_born_set_sources": [_P, _I, _P, _P, _P, _P],
     "fwix_born_set_receivers": [_P, _I, _P, _P, _P, _P],
@@ -117,7 +123,7 @@ _SIGS = {
 }
 
 EXPORTS = ["fwix_version", "fwix_last_error", "fwix_params_default", "fwix_fft_size_supported",
-           "fwix_born_steps", *_SIGS.keys()]
+           "fwix_born_steps", "fwix_born_trace_samples", *_SIGS.keys()]
 
 for _name, _args in _SIGS.items():
     _fn = getattr(lib, _name, None)
@@ -133,6 +139,8 @@ lib.fwix_fft_size_supported.argtypes = [_I]
 lib.fwix_fft_size_supported.restype = C.c_int
 lib.fwix_born_steps.argtypes = [_P]
 lib.fwix_born_steps.restype = C.c_longlong
+lib.fwix_born_trace_samples.argtypes = [_P, _I]
+lib.fwix_born_trace_samples.restype = C.c_longlong
 
 
 def check(rc: int) -> None:

@@ -24,7 +24,7 @@ static void upload_vec(DevBuf& b, const std::vector<T>& v) {
 
 // ------------------------------------------------------------------ injection plan
 void InjPlan::build(const fwix_dims& d, int nlev_, int ntrace_, const float* cx, const float* cy, const float* cz,
-                    const int* ids) {
+                    const int* ids, const std::vector<int>* first, const std::vector<int>* count) {
   if (ntrace_ < 0) throw Error("ntrace must be >= 0");
   if (ntrace_ > 0 && (!cx || !cy || !cz || !ids)) throw Error("null coordinates");
   ntrace = ntrace_;
@@ -32,13 +32,19 @@ void InjPlan::build(const fwix_dims& d, int nlev_, int ntrace_, const float* cx,
   NW = d.NW;
   NX = d.NX;
   plane = (long long)d.NX * d.NY;
-  const long long vol = plane * d.NW;  // one shot
+  const int nshot = first ? (int)first->size() : d.NS;
+  auto p_first = [&](int s) { return first ? (long long)(*first)[s] : (long long)s * d.NW; };
+  auto p_count = [&](int s) { return count ? (*count)[s] : d.NW; };
   struct Ent { int lev; long long node; int trace; float w; };
   struct Act { int lev; int trace; long long base; float4 w; };
   std::vector<Ent> ents;
   std::vector<Act> acts;
+  std::vector<long long> to(ntrace + 1, 0);
+  wmax = 0;
   for (int t = 0; t < ntrace; ++t) {
-    if (ids[t] < 0 || ids[t] >= d.NS) throw Error("trace shot id out of range [0, NS)");
+    if (ids[t] < 0 || ids[t] >= nshot) throw Error("trace shot id out of range [0, NS)");
+    to[t + 1] = to[t] + p_count(ids[t]);
+    wmax = std::max(wmax, p_count(ids[t]));
     // propagator/src/injection.cu:38-48, float arithmetic as in the kernel
     const int iy = (int)floorf((cy[t] - d.oy) / d.dy);
     const float y = d.oy + iy * d.dy;
@@ -59,7 +65,8 @@ void InjPlan::build(const fwix_dims& d, int nlev_, int ntrace_, const float* cx,
       w.y = f * (1 - lx) * ly;
       w.z = f * lx * (1 - ly);
       w.w = f * (1 - lx) * (1 - ly);
-      const long long base = (long long)ids[t] * vol + (long long)iy * d.NX + ix;
+      // first plane of the trace's shot: its nodes never meet another shot's
+      const long long base = p_first(ids[t]) * plane + (long long)iy * d.NX + ix;
       acts.push_back({lev, t, base, w});
       ents.push_back({lev, base, t, w.x});
       ents.push_back({lev, base + 1, t, w.y});
@@ -67,6 +74,7 @@ void InjPlan::build(const fwix_dims& d, int nlev_, int ntrace_, const float* cx,
       ents.push_back({lev, base + d.NX + 1, t, w.w});
     }
   }
+  nsamp = to[ntrace];
   std::stable_sort(acts.begin(), acts.end(), [](const Act& a, const Act& b) { return a.lev < b.lev; });
   std::stable_sort(ents.begin(), ents.end(), [](const Ent& a, const Ent& b) {
     if (a.lev != b.lev) return a.lev < b.lev;
@@ -75,22 +83,24 @@ void InjPlan::build(const fwix_dims& d, int nlev_, int ntrace_, const float* cx,
   });
   act_lvl.assign(nlev + 1, 0);
   seg_lvl.assign(nlev + 1, 0);
-  std::vector<int> at(acts.size());
+  std::vector<int> at(acts.size()), an(acts.size());
   std::vector<long long> ab(acts.size());
   std::vector<float4> aw(acts.size());
   for (size_t k = 0; k < acts.size(); ++k) {
     at[k] = acts[k].trace;
     ab[k] = acts[k].base;
     aw[k] = acts[k].w;
+    an[k] = p_count(ids[acts[k].trace]);
     act_lvl[acts[k].lev + 1]++;
   }
   std::vector<long long> sb;
-  std::vector<int> so, et;
+  std::vector<int> so, sn, et;
   std::vector<float> ew;
   for (size_t k = 0; k < ents.size(); ++k) {
     if (k == 0 || ents[k].lev != ents[k - 1].lev || ents[k].node != ents[k - 1].node) {
       sb.push_back(ents[k].node);
       so.push_back((int)et.size());
+      sn.push_back(p_count(ids[ents[k].trace]));
       seg_lvl[ents[k].lev + 1]++;
     }
     et.push_back(ents[k].trace);
@@ -104,24 +114,28 @@ void InjPlan::build(const fwix_dims& d, int nlev_, int ntrace_, const float* cx,
   upload_vec(act_trace, at);
   upload_vec(act_base, ab);
   upload_vec(act_w, aw);
+  upload_vec(act_np, an);
   upload_vec(seg_base, sb);
   upload_vec(seg_off, so);
+  upload_vec(seg_np, sn);
   upload_vec(ent_trace, et);
   upload_vec(ent_w, ew);
+  upload_vec(toff, to);
 }
 
 void InjPlan::forward(int iz, const float2* traces, float2* wfld, hipStream_t st) const {
   if (iz < 0 || iz >= nlev) return;
-  hip_check(launch_inj_fwd(traces, wfld, (const long long*)seg_base.p, (const int*)seg_off.p,
-                           (const int*)ent_trace.p, (const float*)ent_w.p, seg_lvl[iz],
-                           seg_lvl[iz + 1] - seg_lvl[iz], NW, plane, st),
+  hip_check(launch_inj_fwd(traces, wfld, (const long long*)seg_base.p, (const int*)seg_off.p, (const int*)seg_np.p,
+                           (const int*)ent_trace.p, (const float*)ent_w.p, (const long long*)toff.p, seg_lvl[iz],
+                           seg_lvl[iz + 1] - seg_lvl[iz], wmax, plane, st),
             "injection forward");
 }
 
 void InjPlan::adjoint(int iz, float2* traces, const float2* wfld, hipStream_t st) const {
   if (iz < 0 || iz >= nlev) return;
   hip_check(launch_inj_adj(traces, wfld, (const int*)act_trace.p, (const long long*)act_base.p,
-                           (const float4*)act_w.p, act_lvl[iz], act_lvl[iz + 1] - act_lvl[iz], NW, NX, plane, st),
+                           (const float4*)act_w.p, (const int*)act_np.p, (const long long*)toff.p, act_lvl[iz],
+                           act_lvl[iz + 1] - act_lvl[iz], wmax, NX, plane, st),
             "injection adjoint");
 }
 
@@ -249,7 +263,34 @@ struct BornOp : fwix_op_s {
   int seg_first = -1;  // first depth of the recomputed segment held in seg
   bool have_bg = false, have_up = false;
   long long steps = 0;  // depth steps executed (statistics)
+  // shot batch: shot k owns planes [sfirst[k], sfirst[k] + scount[k]); the
+  // slowness, references, labels and model are per shot, traces ragged
+  std::vector<int> sfirst, scount;
+  DevBuf gscr, sfirst_dev, scount_dev;  // per-plane gradient of one depth; plane ranges
   BornOp() { kind = KIND_BORN; }
+  bool batched() const { return !sfirst.empty(); }
+  void no_batch(const char* what) const {
+    if (batched()) throw Error(std::string(what) + " is not available for a shot batch");
+  }
+  void set_shot_planes(int nshot, const int* nplanes) {
+    eng.set_model_map(nshot, nplanes);
+    sfirst.assign(nshot, 0);
+    scount.assign(nplanes, nplanes + nshot);
+    for (int k = 1; k < nshot; ++k) sfirst[k] = sfirst[k - 1] + scount[k - 1];
+    upload_vec(sfirst_dev, sfirst);
+    upload_vec(scount_dev, scount);
+    gscr.alloc((size_t)d.NW * d.ny * d.nx * sizeof(float2));
+    slow = nullptr;  // model-sized inputs change shape: set them again
+    wavelet = nullptr;
+    nsrc = nrec = 0;
+    build_plan(src, 0, nullptr, nullptr, nullptr, nullptr);
+    build_plan(rec, 0, nullptr, nullptr, nullptr, nullptr);
+    have_bg = have_up = false;
+  }
+  void build_plan(InjPlan& pl, int n, const float* x, const float* y, const float* z, const int* ids) {
+    if (batched()) pl.build(d, d.nz, n, x, y, z, ids, &sfirst, &scount);
+    else pl.build(d, d.nz, n, x, y, z, ids);
+  }
 
   size_t P() const { return wsize(d); }
   float2* bufU() { return (float2*)U.p; }
@@ -314,7 +355,7 @@ struct BornOp : fwix_op_s {
     if (!wavelet) throw Error("wavelet not set (call fwix_born_set_wavelet)");
     if (!eng.sref) throw Error("reference slownesses / labels not set");
     const size_t n = P();
-    if (data && !add) hip_check(launch_zero(data, (size_t)nrec * d.NW, st), "zero data");
+    if (data && !add) hip_check(launch_zero(data, (size_t)rec.nsamp, st), "zero data");
     // Lossless checkpoints of every depth: the recursion runs in the store
     // itself, each step writing U_{z+1} into its slot (no copies).
     const bool direct = ckpt == 1 && !codec;
@@ -351,12 +392,12 @@ struct BornOp : fwix_op_s {
   KArgs born_args(int z, int p0, int np) const {
     KArgs a = eng.base(z, p0, np);
     a.nterm = par.taylor + 1;
-    a.slow = slow + (size_t)z * d.NW * d.ny * d.nx;
+    a.slow = slow + (size_t)z * MZ();
     a.eps_s = par.eps_scat;
     return a;
   }
 
-  size_t MZ() const { return (size_t)d.NW * d.ny * d.nx; }
+  size_t MZ() const { return (size_t)eng.nmodel * d.ny * d.nx; }
 
   // out = P_z[ addin + S_z(bg * ds_z) ] over every plane (Scattering.cpp:43-52,
   // fused): addin and out may be the same wavefield.  The scattering operator
@@ -421,12 +462,17 @@ struct BornOp : fwix_op_s {
           a.nterm = T; a.scale = sy2;
           { PassTimer t("born_cols_kfac_adj", (1 + T) * pb, st); hip_check(launch_cols_fanin(2, a, st), "born adjoint scatter columns"); }
         }
-        // gz += conj(bg) S_z^H mu
+        // gz += conj(bg) S_z^H mu (a shot batch: per plane, summed over each shot below)
         a.in = lc; a.bg = bg + (size_t)p0 * plane;
-        a.aux = gz;
-        a.add = 1; a.nterm = T + 1; a.scale = sx;
+        a.aux = batched() ? (float2*)gscr.p : gz;
+        a.add = batched() ? 0 : 1; a.nterm = T + 1; a.scale = sx;
         { PassTimer t("born_rows_adj_ic", 3 * pb + 3 * mb, st); hip_check(launch_rows_adj_out(true, a, st), "born adjoint imaging"); }
       }
+    if (batched()) {
+      PassTimer t("born_shot_sum", 8.0 * d.NW * d.ny * d.nx + 16.0 * MZ(), st);
+      hip_check(launch_plane_sum((const float2*)gscr.p, gz, (const int*)sfirst_dev.p, (const int*)scount_dev.p,
+                                 eng.nmodel, (long long)d.ny * d.nx, st), "shot gradient sum");
+    }
     ++steps;
   }
 
@@ -435,7 +481,7 @@ struct BornOp : fwix_op_s {
   void forward(int add, const float2* dslow, float2* data, hipStream_t st, bool keep = false) {
     if (!slow) throw Error("background slowness not set (call fwix_born_set_slow)");
     const size_t n = P();
-    if (!add) hip_check(launch_zero(data, (size_t)nrec * d.NW, st), "zero data");
+    if (!add) hip_check(launch_zero(data, (size_t)rec.nsamp, st), "zero data");
     float2* du = keep ? (float2*)xstore.p : (float2*)dU.p;
     hip_check(launch_zero(du, n, st), "zero dU");
     for (int z = 0; z < d.nz; ++z) {
@@ -453,6 +499,7 @@ struct BornOp : fwix_op_s {
   void adjoint(int add, float2* dslow, float2* image, const float2* data, hipStream_t st,
                const float2* mu = nullptr) {
     if (!slow && !image) throw Error("background slowness not set (call fwix_born_set_slow)");
+    if (image || mu) no_batch("the migration image");
     const size_t n = P();
     if (!add) {
       if (dslow) hip_check(launch_zero(dslow, MZ() * d.nz, st), "zero model");
@@ -493,6 +540,7 @@ struct BornOp : fwix_op_s {
     return r;
   }
   void need_up() const {
+    no_batch("the up-going (reflection) channel");
     if (!den) throw Error("density not set (call fwix_born_set_density)");
     if (!slow) throw Error("background slowness not set (call fwix_born_set_slow)");
   }
@@ -511,7 +559,7 @@ struct BornOp : fwix_op_s {
     const size_t n = P();
     const size_t bytes = n * sizeof(float2) * (size_t)d.nz;
     if (vstore.n != bytes) vstore.alloc(bytes);
-    if (data && !add) hip_check(launch_zero(data, (size_t)nrec * d.NW, st), "zero data");
+    if (data && !add) hip_check(launch_zero(data, (size_t)rec.nsamp, st), "zero data");
     float2* V = (float2*)vstore.p;
     for (int z = d.nz - 1; z >= 0; --z) {
       float2* vz = V + (size_t)z * n;
@@ -592,8 +640,9 @@ struct BornOp : fwix_op_s {
   // V = 0; for z = nz-1..0: V <- P_z V; V += refl_z * U_z (model region);
   // data += Rec_z^T V.  Linear in refl (nz, NW, ny, nx).
   void reflect_forward(int add, const float2* refl, float2* data, hipStream_t st) {
+    no_batch("the reflection channel");
     const size_t n = P(), mz = (size_t)d.NW * d.ny * d.nx;
-    if (!add) hip_check(launch_zero(data, (size_t)nrec * d.NW, st), "zero data");
+    if (!add) hip_check(launch_zero(data, (size_t)rec.nsamp, st), "zero data");
     float2* v = (float2*)dU.p;
     hip_check(launch_zero(v, n, st), "zero V");
     for (int z = d.nz - 1; z >= 0; --z) {
@@ -615,6 +664,7 @@ struct BornOp : fwix_op_s {
   // With image != null: image_z (+)= sum_{w,s} conj(U_z) mu_z, the migration
   // image of surface data (source wavefield U, receiver wavefield mu).
   void reflect_adjoint(int add, float2* refl, float2* image, const float2* data, hipStream_t st) {
+    no_batch("the reflection channel");
     const size_t n = P(), mz = (size_t)d.NW * d.ny * d.nx;
     if (!add) {
       if (refl) hip_check(launch_zero(refl, mz * d.nz, st), "zero model");
@@ -685,7 +735,7 @@ int fwix_injection_adjoint(fwix_op op, int add, fwix_c64* traces, const fwix_c64
   return guard([&] {
     auto* p = as<InjectionOp>(op, KIND_INJECTION);
     if (!p->has_coords) throw Error("injection coordinates not set");
-    if (!add) hip_check(launch_zero((float2*)traces, (size_t)p->ntrace * p->d.NW, S(s)), "zero");
+    if (!add) hip_check(launch_zero((float2*)traces, (size_t)p->plan.nsamp, S(s)), "zero");
     p->plan.adjoint(p->iz, (float2*)traces, (const float2*)wfld, S(s));
   });
 }
@@ -780,6 +830,12 @@ int fwix_oneway_create(int direction, int method, const fwix_dims* d, const fwix
     *out = op;
   });
 }
+int fwix_oneway_set_shot_planes(fwix_op op, int nshot, const int* nplanes) {
+  return guard([&] { as<OneWayOp>(op, KIND_ONEWAY)->eng.set_model_map(nshot, nplanes); });
+}
+int fwix_oneway_set_frequencies(fwix_op op, const float* f_hz) {
+  return guard([&] { as<OneWayOp>(op, KIND_ONEWAY)->eng.tables.set_frequencies(f_hz, false); });
+}
 int fwix_oneway_set_refs(fwix_op op, const fwix_c64* sref, const uint8_t* labels) {
   return guard([&] {
     auto* p = as<OneWayOp>(op, KIND_ONEWAY);
@@ -824,6 +880,14 @@ int fwix_born_create(const fwix_dims* d, const fwix_params* par, int ckpt_every,
     *out = op;
   });
 }
+int fwix_born_set_frequencies(fwix_op op, const float* f_hz) {
+  return guard([&] {
+    auto* p = as<BornOp>(op, KIND_BORN);
+    p->eng.tables.set_frequencies(f_hz, true);
+    p->have_bg = false;
+    p->have_up = false;
+  });
+}
 int fwix_born_set_refs(fwix_op op, const fwix_c64* sref, const uint8_t* labels) {
   return guard([&] {
     auto* p = as<BornOp>(op, KIND_BORN);
@@ -863,10 +927,18 @@ int fwix_born_set_density(fwix_op op, const fwix_c64* den_dev) {
     p->have_up = false;
   });
 }
+int fwix_born_set_shot_planes(fwix_op op, int nshot, const int* nplanes) {
+  return guard([&] { as<BornOp>(op, KIND_BORN)->set_shot_planes(nshot, nplanes); });
+}
+long long fwix_born_trace_samples(fwix_op op, int receivers) {
+  if (!op || op->kind != KIND_BORN) return -1;
+  auto* p = static_cast<BornOp*>(op);
+  return receivers ? p->rec.nsamp : p->src.nsamp;
+}
 int fwix_born_set_sources(fwix_op op, int n, const float* x, const float* y, const float* z, const int* ids) {
   return guard([&] {
     auto* p = as<BornOp>(op, KIND_BORN);
-    p->src.build(p->d, p->d.nz, n, x, y, z, ids);
+    p->build_plan(p->src, n, x, y, z, ids);
     p->nsrc = n;
     p->have_bg = false;
     p->have_up = false;
@@ -875,7 +947,7 @@ int fwix_born_set_sources(fwix_op op, int n, const float* x, const float* y, con
 int fwix_born_set_receivers(fwix_op op, int n, const float* x, const float* y, const float* z, const int* ids) {
   return guard([&] {
     auto* p = as<BornOp>(op, KIND_BORN);
-    p->rec.build(p->d, p->d.nz, n, x, y, z, ids);
+    p->build_plan(p->rec, n, x, y, z, ids);
     p->nrec = n;
   });
 }

@@ -36,10 +36,10 @@ struct BRowCfg {
 // Taylor coefficients Scatter.h:73).  s^(-1/2) is the reciprocal of the
 // principal square root, identical to |s|^(-1/2) e^{-i arg(s)/2}
 // (scatter.cu:6-21); higher terms multiply by 1/s.
-__device__ __forceinline__ void scatter_factors(const KArgs& a, int iw, int iy, int ix, float2& rsq, float2& inv) {
+__device__ __forceinline__ void scatter_factors(const KArgs& a, int im, int iy, int ix, float2& rsq, float2& inv) {
   const int yy = iy < a.ny ? iy : a.ny - 1;
   const int xx = ix < a.nx ? ix : a.nx - 1;
-  const float2 s = a.slow[(unsigned)((iw * a.ny + yy) * a.nx + xx)];
+  const float2 s = a.slow[(unsigned)((im * a.ny + yy) * a.nx + xx)];
   const float sr = s.x, si = s.y - a.eps_s * s.x;
   // principal sqrt of (sr, si)
   const float r = sqrtf(fmaf(sr, sr, si * si));
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_ic_fwd(KArgs a) {
       const int p = lr / a.NY, iy = lr - p * a.NY;
       if (iy < a.ny && i < a.nx) {
         const float2 u = a.bg[(unsigned)(lr * N + i)];
-        const float2 m = a.dslow[(unsigned)((plane_iw(a, p) * a.ny + iy) * a.nx + i)];
+        const float2 m = a.dslow[(unsigned)((model_iw(a, plane_iw(a, p)) * a.ny + iy) * a.nx + i)];
         v[e] = cmul(u, m);
       }
     }
@@ -125,12 +125,12 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
     inv[e] = czero();
     if (lr < nrows) {
       const int p = lr / a.NY, iy = lr - p * a.NY;
-      scatter_factors(a, plane_iw(a, p), iy, i, pw[e], inv[e]);
+      scatter_factors(a, model_iw(a, plane_iw(a, p)), iy, i, pw[e], inv[e]);
       float2 v;
       if constexpr (BORN) {
         v = czero();
         if (iy < a.ny && i < a.nx)
-          v = cmul(a.bg[(unsigned)(lr * N + i)], a.dslow[(unsigned)((plane_iw(a, p) * a.ny + iy) * a.nx + i)]);
+          v = cmul(a.bg[(unsigned)(lr * N + i)], a.dslow[(unsigned)((model_iw(a, plane_iw(a, p)) * a.ny + iy) * a.nx + i)]);
       } else {
         v = a.in[(unsigned)(lr * N + i)];
       }
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
       const int p = lr / a.NY;
       const float h = 0.5f * a.w[plane_iw(a, p)] * a.dz;   // scatter.cu:197-238
       u[e] = make_float2(-h * u[e].y, h * u[e].x);
-      scatter_factors(a, plane_iw(a, p), lr - p * a.NY, i, pw[e], inv[e]);
+      scatter_factors(a, model_iw(a, plane_iw(a, p)), lr - p * a.NY, i, pw[e], inv[e]);
       pw[e] = cmul(pw[e], inv[e]);                          // start at order 1
     }
   }
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj_out(KArgs a) {
       const int iw = plane_iw(a, p);
       const unsigned idx = lr * N + i;
       float2 rsq, inv;
-      scatter_factors(a, iw, iy, i, rsq, inv);
+      scatter_factors(a, model_iw(a, iw), iy, i, rsq, inv);
       const float h = 0.5f * a.w[iw] * a.dz;
       const float2 d = a.in[idx];
       const float2 u = make_float2(-h * d.y, h * d.x);
@@ -499,14 +499,17 @@ __global__ __launch_bounds__(kThreads) void k_ic_adj(float2* img, const float2* 
 // scatter is race-free and deterministic (the reference's kernel does a
 // non-atomic read-modify-write, injection.cu:89-92).
 __global__ __launch_bounds__(kThreads) void k_inj_fwd(const float2* __restrict__ tr, float2* wf, const long long* __restrict__ seg_base,
-                                                      const int* __restrict__ seg_off, const int* __restrict__ ent_trace,
-                                                      const float* __restrict__ ent_w, int s0, int nseg, int NW, long long plane) {
-  const long long n = (long long)nseg * NW;
+                                                      const int* __restrict__ seg_off, const int* __restrict__ seg_np,
+                                                      const int* __restrict__ ent_trace, const float* __restrict__ ent_w,
+                                                      const long long* __restrict__ toff, int s0, int nseg, int wmax,
+                                                      long long plane) {
+  const long long n = (long long)nseg * wmax;
   for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < n; t += (long long)gridDim.x * kThreads) {
-    const int sg = s0 + (int)(t / NW), iw = (int)(t % NW);
+    const int sg = s0 + (int)(t / wmax), iw = (int)(t % wmax);
+    if (iw >= seg_np[sg]) continue;
     float2 acc = czero();
     for (int k = seg_off[sg]; k < seg_off[sg + 1]; ++k)
-      acc = cadd(acc, cscale(tr[(size_t)ent_trace[k] * NW + iw], ent_w[k]));
+      acc = cadd(acc, cscale(tr[toff[ent_trace[k]] + iw], ent_w[k]));
     const long long idx = seg_base[sg] + (long long)iw * plane;
     wf[idx] = cadd(wf[idx], acc);
   }
@@ -515,18 +518,37 @@ __global__ __launch_bounds__(kThreads) void k_inj_fwd(const float2* __restrict__
 // its four corners.
 __global__ __launch_bounds__(kThreads) void k_inj_adj(float2* tr, const float2* __restrict__ wf, const int* __restrict__ act_trace,
                                                       const long long* __restrict__ act_base, const float4* __restrict__ act_w,
-                                                      int a0, int nact, int NW, int NX, long long plane) {
-  const long long n = (long long)nact * NW;
+                                                      const int* __restrict__ act_np, const long long* __restrict__ toff,
+                                                      int a0, int nact, int wmax, int NX, long long plane) {
+  const long long n = (long long)nact * wmax;
   for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < n; t += (long long)gridDim.x * kThreads) {
-    const int k = a0 + (int)(t / NW), iw = (int)(t % NW);
+    const int k = a0 + (int)(t / wmax), iw = (int)(t % wmax);
+    if (iw >= act_np[k]) continue;
     const long long b = act_base[k] + (long long)iw * plane;
     const float4 w = act_w[k];
     float2 v = cscale(wf[b], w.x);
     v = cadd(v, cscale(wf[b + 1], w.y));
     v = cadd(v, cscale(wf[b + NX], w.z));
     v = cadd(v, cscale(wf[b + NX + 1], w.w));
-    const size_t o = (size_t)act_trace[k] * NW + iw;
+    const long long o = toff[act_trace[k]] + iw;
     tr[o] = cadd(tr[o], v);
+  }
+}
+
+// Shot sums of a per-plane gradient: out[k, j] += sum over the planes p of
+// shot k, in plane order, of g[p, j] (shot k owns planes first[k] ..
+// first[k] + count[k] - 1).
+__global__ __launch_bounds__(kThreads) void k_plane_sum(const float2* __restrict__ g, float2* out,
+                                                        const int* __restrict__ first, const int* __restrict__ count,
+                                                        int nshot, long long m) {
+  const long long n = (long long)nshot * m;
+  for (long long t = (long long)blockIdx.x * kThreads + threadIdx.x; t < n; t += (long long)gridDim.x * kThreads) {
+    const int k = (int)(t / m);
+    const long long j = t - (long long)k * m;
+    const float2* src = g + (size_t)first[k] * m + j;
+    float2 acc = out[t];
+    for (int p = 0; p < count[k]; ++p) acc = cadd(acc, src[(size_t)p * m]);
+    out[t] = acc;
   }
 }
 
@@ -618,19 +640,26 @@ hipError_t launch_ic_adj(float2* img, const float2* D, const float2* U, int NX, 
                      ny, add);
   return hipGetLastError();
 }
+hipError_t launch_plane_sum(const float2* g, float2* out, const int* first, const int* count, int nshot, long long m,
+                            hipStream_t st) {
+  if (nshot <= 0 || m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_plane_sum, bflat((long long)nshot * m), dim3(kThreads), 0, st, g, out, first, count, nshot, m);
+  return hipGetLastError();
+}
 hipError_t launch_inj_fwd(const float2* tr, float2* wf, const long long* seg_base, const int* seg_off,
-                          const int* ent_trace, const float* ent_w, int s0, int nseg, int NW, long long plane,
-                          hipStream_t st) {
-  if (nseg <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_inj_fwd, bflat((long long)nseg * NW), dim3(kThreads), 0, st, tr, wf, seg_base, seg_off,
-                     ent_trace, ent_w, s0, nseg, NW, plane);
+                          const int* seg_np, const int* ent_trace, const float* ent_w, const long long* toff, int s0,
+                          int nseg, int wmax, long long plane, hipStream_t st) {
+  if (nseg <= 0 || wmax <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_inj_fwd, bflat((long long)nseg * wmax), dim3(kThreads), 0, st, tr, wf, seg_base, seg_off,
+                     seg_np, ent_trace, ent_w, toff, s0, nseg, wmax, plane);
   return hipGetLastError();
 }
 hipError_t launch_inj_adj(float2* tr, const float2* wf, const int* act_trace, const long long* act_base,
-                          const float4* act_w, int a0, int nact, int NW, int NX, long long plane, hipStream_t st) {
-  if (nact <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_inj_adj, bflat((long long)nact * NW), dim3(kThreads), 0, st, tr, wf, act_trace, act_base, act_w,
-                     a0, nact, NW, NX, plane);
+                          const float4* act_w, const int* act_np, const long long* toff, int a0, int nact, int wmax,
+                          int NX, long long plane, hipStream_t st) {
+  if (nact <= 0 || wmax <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_inj_adj, bflat((long long)nact * wmax), dim3(kThreads), 0, st, tr, wf, act_trace, act_base, act_w,
+                     act_np, toff, a0, nact, wmax, NX, plane);
   return hipGetLastError();
 }
 

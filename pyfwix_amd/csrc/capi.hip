@@ -162,6 +162,20 @@ void Tables::init(const fwix_dims& d, int tx, int ty) {
   dims = d;
 }
 
+void Tables::set_frequencies(const float* f_hz, bool nonzero) {
+  if (!f_hz) throw Error("null frequency list");
+  std::vector<float> w2(dims.NW), w(dims.NW);
+  for (int i = 0; i < dims.NW; ++i) {
+    if (!std::isfinite(f_hz[i])) throw Error("frequencies must be finite");
+    if (nonzero && f_hz[i] == 0.f) throw Error("Frequency is zero in the scattering operator!");
+    const double f = 2.0 * M_PI * (double)f_hz[i];
+    w2[i] = (float)(f * f);
+    w[i] = (float)f;
+  }
+  upload_bytes(b_w2, w2.data(), w2.size() * 4);
+  upload_bytes(b_w, w.data(), w.size() * 4);
+}
+
 KArgs Tables::kargs() const {
   KArgs a;
   std::memset(&a, 0, sizeof(a));
@@ -217,6 +231,8 @@ void StepEngine::init(int method_, const fwix_dims& d, const fwix_params& p, int
   method = plain ? FWIX_PSPI : method_;
   params = p;
   tables.init(d, method == FWIX_PSPI ? p.taperx : 0, method == FWIX_PSPI ? p.tapery : 0);
+  nmodel = d.NW;
+  mplane.clear();
   const size_t plane = (size_t)d.NX * d.NY;
   if (plain) {  // two whole-shot scratch wavefields
     chunk = d.NW;
@@ -230,14 +246,45 @@ void StepEngine::init(int method_, const fwix_dims& d, const fwix_params& p, int
   W.alloc(plane * chunk * wterms * sizeof(float2));
 }
 
+void StepEngine::set_model_map(int nshot, const int* nplanes) {
+  const fwix_dims& d = tables.dims;
+  if (plain) throw Error("shot batches run on the fused path only");
+  if (d.NS != 1) throw Error("a shot batch needs NS == 1 (the shots are laid along the plane axis)");
+  if (nshot < 1 || !nplanes) throw Error("a shot batch needs at least one shot");
+  std::vector<int> map;
+  map.reserve(d.NW);
+  for (int k = 0; k < nshot; ++k) {
+    if (nplanes[k] < 1) throw Error("every shot of a batch needs at least one plane");
+    map.insert(map.end(), (size_t)nplanes[k], k);
+    if ((long long)map.size() > d.NW) break;
+  }
+  if ((long long)map.size() != d.NW) throw Error("the shots' plane counts must add up to NW");
+  mplane = map;
+  nmodel = nshot;
+  upload_bytes(mplane_dev, mplane.data(), mplane.size() * sizeof(int));
+  sref = nullptr;  // references and labels are per shot now: set them again
+  labels = nullptr;
+  masks = false;
+}
+
 void StepEngine::set_refs(const float2* sref_, const uint8_t* labels_) {
   if (!sref_ || !labels_) throw Error("null reference slownesses or labels");
+  const fwix_dims& d = tables.dims;
   sref = sref_;
+  if (!mplane.empty()) {  // (nz, nref, nshot) per shot -> (nz, nref, NW) per plane
+    const size_t nin = (size_t)d.nz * params.nref * nmodel, nout = (size_t)d.nz * params.nref * d.NW;
+    std::vector<float2> hin(nin), hout(nout);
+    hip_check(hipDeviceSynchronize(), "synchronize before reference expansion");
+    hip_check(hipMemcpy(hin.data(), sref_, nin * sizeof(float2), hipMemcpyDeviceToHost), "references D2H");
+    for (size_t zr = 0; zr < (size_t)d.nz * params.nref; ++zr)
+      for (int w = 0; w < d.NW; ++w) hout[zr * d.NW + w] = hin[zr * nmodel + mplane[w]];
+    upload_bytes(sref_exp, hout.data(), nout * sizeof(float2));
+    sref = (const float2*)sref_exp.p;
+  }
   labels = labels_;
   masks = false;
   if (params.nref > 32) return;  // presence skipping covers references 0..31
-  const fwix_dims& d = tables.dims;
-  const long long nplanes = (long long)d.nz * d.NW;
+  const long long nplanes = (long long)d.nz * nmodel;
   rowmask.alloc((size_t)nplanes * d.NY * sizeof(uint32_t));
   planemask.alloc((size_t)nplanes * sizeof(uint32_t));
   hip_check(hipDeviceSynchronize(), "synchronize before presence masks");
@@ -256,7 +303,7 @@ double StepEngine::present_rows(int iz, int p0, int np) const {
   const fwix_dims& d = tables.dims;
   if (!masks) return (double)params.nref * np * d.NY;
   double c = 0;
-  for (int p = 0; p < np; ++p) c += present[(size_t)iz * d.NW + (p0 + p) % d.NW];
+  for (int p = 0; p < np; ++p) c += present[(size_t)iz * nmodel + model_of((p0 + p) % d.NW)];
   return c;
 }
 
@@ -270,11 +317,12 @@ KArgs StepEngine::base(int iz, int c0, int np) const {
   KArgs a = tables.kargs();
   a.K = (float2*)K.p;
   a.W = (float2*)W.p;
-  a.labels = labels + (size_t)iz * d.NW * d.NY * d.NX;
+  a.labels = labels + (size_t)iz * nmodel * d.NY * d.NX;
   a.sref = sref + (size_t)iz * params.nref * d.NW;
+  a.mplane = mplane.empty() ? nullptr : (const int*)mplane_dev.p;
   if (masks) {
-    a.rowmask = (const uint32_t*)rowmask.p + (size_t)iz * d.NW * d.NY;
-    a.planemask = (const uint32_t*)planemask.p + (size_t)iz * d.NW;
+    a.rowmask = (const uint32_t*)rowmask.p + (size_t)iz * nmodel * d.NY;
+    a.planemask = (const uint32_t*)planemask.p + (size_t)iz * nmodel;
   }
   a.nref = params.nref;
   a.nterm = params.nref;
@@ -609,6 +657,12 @@ int fwix_onestep_set_refs(fwix_op op, const fwix_c64* sref, const uint8_t* label
     auto* p = as<OneStepOp>(op, KIND_ONESTEP);
     p->eng.set_refs((const float2*)sref, labels);
   });
+}
+int fwix_onestep_set_shot_planes(fwix_op op, int nshot, const int* nplanes) {
+  return guard([&] { as<OneStepOp>(op, KIND_ONESTEP)->eng.set_model_map(nshot, nplanes); });
+}
+int fwix_onestep_set_frequencies(fwix_op op, const float* f_hz) {
+  return guard([&] { as<OneStepOp>(op, KIND_ONESTEP)->eng.tables.set_frequencies(f_hz, false); });
 }
 int fwix_onestep_set_depth(fwix_op op, int iz) {
   return guard([&] {

@@ -69,6 +69,8 @@ struct Tables {
   fwix_dims dims{};
   DevBuf b_kx, b_ky, b_w2, b_w, b_tapx, b_tapy, b_twx, b_twy;
   void init(const fwix_dims& d, int tx, int ty);
+  // per-plane frequencies in Hz (NW of them) in place of ow + iw * dw
+  void set_frequencies(const float* f_hz, bool nonzero);
   KArgs kargs() const;
   PwArgs pwargs() const;
 };
@@ -95,12 +97,19 @@ struct StepEngine {
   DevBuf T;            // plain path: second whole-shot scratch
   fwix_params params{};
   Tables tables;
-  const float2* sref = nullptr;     // borrowed (nz, nref, NW)
-  const uint8_t* labels = nullptr;  // borrowed (nz, NW, NY, NX)
+  const float2* sref = nullptr;     // borrowed (nz, nref, NW), or the expansion below
+  const uint8_t* labels = nullptr;  // borrowed (nz, nmodel, NY, NX)
   DevBuf K, W;
-  DevBuf rowmask, planemask;  // reference presence per (depth, w, y) row and per (depth, w) plane
+  DevBuf rowmask, planemask;  // reference presence per (depth, model plane, y) row and per (depth, model plane)
   bool masks = false;
-  std::vector<int> present;   // (nz, NW): present (row, reference) pairs per plane
+  std::vector<int> present;   // (nz, nmodel): present (row, reference) pairs per model plane
+  // Shot batch: plane iw takes its labels / slowness / model from model plane
+  // mplane[iw] (the shot it belongs to); nmodel == NW and no map otherwise.
+  int nmodel = 0;
+  std::vector<int> mplane;
+  DevBuf mplane_dev, sref_exp;  // device map; references expanded to (nz, nref, NW)
+  void set_model_map(int nshot, const int* nplanes);
+  int model_of(int iw) const { return mplane.empty() ? iw : mplane[iw]; }
   // rows of the per-reference scratch a chunk actually reads or writes
   double present_rows(int iz, int p0, int np) const;
   int chunk = 1;
@@ -121,13 +130,20 @@ struct StepEngine {
 // gathered per node) and the active traces with their four corners
 // (adjoint).  Built on the host from float32 coordinates with the kernel's
 // own float arithmetic.
+//
+// Traces are ragged when the shot ids map to plane ranges (a shot batch):
+// trace t holds count[id_t] samples, one per plane of its shot, stored one
+// trace after another; with no ranges shot id s owns planes [s NW, (s+1) NW)
+// and every trace holds NW samples.
 struct InjPlan {
   int ntrace = 0, nlev = 0, NW = 0, NX = 0;
+  int wmax = 0;           // largest number of planes of one trace
   long long plane = 0;
+  long long nsamp = 0;    // samples of all traces
   std::vector<int> seg_lvl, act_lvl;  // (nlev + 1) offsets
-  DevBuf seg_base, seg_off, ent_trace, ent_w, act_trace, act_base, act_w;
+  DevBuf seg_base, seg_off, seg_np, ent_trace, ent_w, act_trace, act_base, act_w, act_np, toff;
   void build(const fwix_dims& d, int nlev, int ntrace, const float* cx, const float* cy, const float* cz,
-             const int* ids);
+             const int* ids, const std::vector<int>* first = nullptr, const std::vector<int>* count = nullptr);
   void forward(int iz, const float2* traces, float2* wfld, hipStream_t st) const;  // wfld += Inj_iz(traces)
   void adjoint(int iz, float2* traces, const float2* wfld, hipStream_t st) const;  // traces += Rec_iz^T wfld
 };

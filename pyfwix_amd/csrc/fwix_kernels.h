@@ -43,6 +43,7 @@ struct KArgs {
   const uint32_t* rowmask; // (NW, NY) bit r set: reference r labels a node of that row (null: all)
   const uint32_t* planemask; // (NW) OR of rowmask over the plane's rows (null: all)
   const float2* sref;      // (nref, NW) reference slownesses of the current depth
+  const int* mplane;       // (NW) model plane of every plane (null: the plane itself)
   const float* kx;         // (NX) wavenumbers
   const float* ky;         // (NY)
   const float* w2;         // (NW) (2 pi f)^2
@@ -66,6 +67,10 @@ struct KArgs {
 };
 
 __device__ __forceinline__ int plane_iw(const KArgs& a, int p_local) { return (a.p0 + p_local) % a.NW; }
+// Index into the model-sized arrays (labels, presence masks, slowness, model
+// perturbation): plane iw itself, or the shot it belongs to when planes of
+// several shots share one launch.
+__device__ __forceinline__ int model_iw(const KArgs& a, int iw) { return a.mplane ? a.mplane[iw] : iw; }
 
 // lowest set bit of m above position r (32 when none); r = -1 gives the lowest
 __device__ __forceinline__ int next_bit(uint32_t m, int r) {

@@ -74,10 +74,15 @@ hipError_t launch_ic_fwd(const float2* img, float2* D, const float2* U, int NX, 
                          int add, hipStream_t st);
 hipError_t launch_ic_adj(float2* img, const float2* D, const float2* U, int NX, int NY, int NW, int NS, int nx, int ny,
                          int add, hipStream_t st);
+// out[k] += sum of g over the planes of shot k (first[k], count[k]); m per plane
+hipError_t launch_plane_sum(const float2* g, float2* out, const int* first, const int* count, int nshot, long long m,
+                            hipStream_t st);
+// traces: trace t's sample of its plane iw at toff[t] + iw (iw < its plane count)
 hipError_t launch_inj_fwd(const float2* tr, float2* wf, const long long* seg_base, const int* seg_off,
-                          const int* ent_trace, const float* ent_w, int s0, int nseg, int NW, long long plane,
-                          hipStream_t st);
+                          const int* seg_np, const int* ent_trace, const float* ent_w, const long long* toff, int s0,
+                          int nseg, int wmax, long long plane, hipStream_t st);
 hipError_t launch_inj_adj(float2* tr, const float2* wf, const int* act_trace, const long long* act_base,
-                          const float4* act_w, int a0, int nact, int NW, int NX, long long plane, hipStream_t st);
+                          const float4* act_w, const int* act_np, const long long* toff, int a0, int nact, int wmax,
+                          int NX, long long plane, hipStream_t st);
 
 }  // namespace fwix

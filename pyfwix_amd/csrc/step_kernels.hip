@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
     if (lr < nrows) {
       u[e] = a.in[(unsigned)(lr * N + i)];
       const int p = lr / a.NY, iy = lr - p * a.NY;
-      lab[e] = a.labels[(unsigned)((plane_iw(a, p) * a.NY + iy) * N + i)];
+      lab[e] = a.labels[(unsigned)((model_iw(a, plane_iw(a, p)) * a.NY + iy) * N + i)];
     }
   }
   const float s = a.scale;
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
       const int lr = rb + b;
       if (lr < nrows) {
         const int p = lr / a.NY;
-        wm |= a.rowmask[plane_iw(a, p) * a.NY + (lr - p * a.NY)];
+        wm |= a.rowmask[model_iw(a, plane_iw(a, p)) * a.NY + (lr - p * a.NY)];
       }
     }
     wm = __builtin_amdgcn_readfirstlane(wm);
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
     lab[e] = 255;
     if (lr < nrows) {
       const int p = lr / a.NY, iy = lr - p * a.NY;
-      lab[e] = a.labels[(unsigned)((plane_iw(a, p) * a.NY + iy) * N + i)];
+      lab[e] = a.labels[(unsigned)((model_iw(a, plane_iw(a, p)) * a.NY + iy) * N + i)];
     }
   }
   // references present in this wave's rows (wave-uniform)
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
       const int lr = rb + b;
       if (lr < nrows) {
         const int p = lr / a.NY;
-        wm |= a.rowmask[plane_iw(a, p) * a.NY + (lr - p * a.NY)];
+        wm |= a.rowmask[model_iw(a, plane_iw(a, p)) * a.NY + (lr - p * a.NY)];
       }
     }
     wm = __builtin_amdgcn_readfirstlane(wm);
@@ -482,8 +482,9 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   const float w2 = a.w2[iw];
   const float s2 = a.scale;  // 1/NY: forward and inverse column normalisation together
   const int colw = c0 + wave * C::NB;
-  const uint32_t* rm = (MODE != 2 && a.rowmask) ? a.rowmask + iw * a.NY : nullptr;
-  const uint32_t pm = (MODE != 2 && a.planemask) ? a.planemask[iw] : 0xffffffffu;
+  const int im = model_iw(a, iw);
+  const uint32_t* rm = (MODE != 2 && a.rowmask) ? a.rowmask + im * a.NY : nullptr;
+  const uint32_t pm = (MODE != 2 && a.planemask) ? a.planemask[im] : 0xffffffffu;
   __shared__ uint32_t rmk[C::NROW];
   tile_row_masks<C>(rmk, rm);
   float k2v[C::V];
@@ -529,8 +530,9 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   float2 acc[C::V];
 #pragma unroll
   for (int e = 0; e < C::V; ++e) acc[e] = czero();
-  const uint32_t* rm = (MODE != 2 && a.rowmask) ? a.rowmask + iw * a.NY : nullptr;
-  const uint32_t pm = (MODE != 2 && a.planemask) ? a.planemask[iw] : 0xffffffffu;
+  const int im = model_iw(a, iw);
+  const uint32_t* rm = (MODE != 2 && a.rowmask) ? a.rowmask + im * a.NY : nullptr;
+  const uint32_t pm = (MODE != 2 && a.planemask) ? a.planemask[im] : 0xffffffffu;
   const uint32_t live = a.nterm >= 32 ? pm : (pm & ((1u << a.nterm) - 1u));
   const bool masked = rm != nullptr;
   const int rend = masked ? 32 : a.nterm;

@@ -141,16 +141,29 @@ class OneStep(_Op):
         self.params = params if params is not None else default_params(**kw)
         self._h = Handle(create("fwix_onestep_create", int(method), C.byref(dims), C.byref(self.params)))
         self._refs = None
+        self.nmodel = dims.NW
+
+    def set_shot_planes(self, counts) -> None:
+        """Shot batch (NS == 1): shot k owns the next counts[k] planes; references
+        (nz, nref, nshot) and labels (nz, nshot, NY, NX) are then per shot."""
+        c = _i32(counts, None, "plane counts")
+        call("fwix_onestep_set_shot_planes", self._h.h, len(c), _vp(c))
+        self.nmodel = len(c)
+        self._refs = None
 
     def set_refs(self, sref: torch.Tensor, labels: torch.Tensor) -> None:
         d, p = self.dims, self.params
-        _check(sref, (d.nz, p.nref, d.NW), name="sref")
-        _check(labels, (d.nz, d.NW, d.NY, d.NX), torch.uint8, name="labels")
+        _check(sref, (d.nz, p.nref, self.nmodel), name="sref")
+        _check(labels, (d.nz, self.nmodel, d.NY, d.NX), torch.uint8, name="labels")
         self._refs = (sref, labels)
         call("fwix_onestep_set_refs", self._h.h, ptr(sref), ptr(labels))
 
     def set_depth(self, iz: int) -> None:
         call("fwix_onestep_set_depth", self._h.h, int(iz))
+
+    def set_frequencies(self, f_hz) -> None:
+        """Per-plane frequencies (NW values, Hz) in place of the regular axis."""
+        call("fwix_onestep_set_frequencies", self._h.h, _vp(_f32(f_hz, self.dims.NW, "frequencies")))
 
 
 def _f32(a, n=None, name="array") -> np.ndarray:
@@ -268,13 +281,25 @@ class OneWay:
         self._h = Handle(create("fwix_oneway_create", int(direction), int(method), C.byref(dims),
                                 C.byref(self.params)))
         self._refs = None
+        self.nmodel = dims.NW
+
+    def set_shot_planes(self, counts) -> None:
+        """Shot batch (NS == 1): shot k owns the next counts[k] planes; references
+        (nz, nref, nshot) and labels (nz, nshot, NY, NX) are then per shot."""
+        c = _i32(counts, None, "plane counts")
+        call("fwix_oneway_set_shot_planes", self._h.h, len(c), _vp(c))
+        self.nmodel = len(c)
+        self._refs = None
 
     def set_refs(self, sref: torch.Tensor, labels: torch.Tensor) -> None:
         d, p = self.dims, self.params
-        _check(sref, (d.nz, p.nref, d.NW), name="sref")
-        _check(labels, (d.nz, d.NW, d.NY, d.NX), torch.uint8, name="labels")
+        _check(sref, (d.nz, p.nref, self.nmodel), name="sref")
+        _check(labels, (d.nz, self.nmodel, d.NY, d.NX), torch.uint8, name="labels")
         self._refs = (sref, labels)
         call("fwix_oneway_set_refs", self._h.h, ptr(sref), ptr(labels))
+
+    def set_frequencies(self, f_hz) -> None:
+        call("fwix_oneway_set_frequencies", self._h.h, _vp(_f32(f_hz, self.dims.NW, "frequencies")))
 
     def forward(self, add, model, data, checkpoints: torch.Tensor | None = None):
         s = _wshape(self.dims)
@@ -301,6 +326,12 @@ class Born:
     Model ds (nz, NW, ny, nx); data (nrec, NW); image (nz, ny, nx).
     ``ckpt_every`` > 1 keeps every ckpt_every-th background wavefield and
     recomputes the others in the adjoint.
+
+    After ``set_shot_planes(counts)`` (NS == 1) the planes are a batch of shots:
+    shot k owns the next counts[k] planes; slowness and model become
+    (nz, nshot, ny, nx), references (nz, nref, nshot), labels
+    (nz, nshot, NY, NX), and wavelet / data flat ragged traces (each trace
+    one sample per plane of its shot, traces one after another).
     """
 
     def __init__(self, dims, params=None, ckpt_every: int = 1, **kw):
@@ -309,19 +340,42 @@ class Born:
         self._h = Handle(create("fwix_born_create", C.byref(dims), C.byref(self.params), int(ckpt_every)))
         self._keep = {}
         self.nsrc = self.nrec = 0
+        self.nmodel = dims.NW        # model planes: NW, or the shots of a batch
+        self.shot_planes = None
+
+    def _model_shape(self):
+        d = self.dims
+        return (d.nz, self.nmodel, d.ny, d.nx)
+
+    def _trace_shape(self, receivers: bool):
+        n = self.nrec if receivers else self.nsrc
+        if self.shot_planes is None:
+            return (n, self.dims.NW)
+        return (int(_lib.lib.fwix_born_trace_samples(self._h.h, int(receivers))),)
+
+    def set_shot_planes(self, counts) -> None:
+        """Lay a batch of shots along the plane axis (shot k: the next counts[k] planes)."""
+        c = _i32(counts, None, "plane counts")
+        call("fwix_born_set_shot_planes", self._h.h, len(c), _vp(c))
+        self.nmodel = len(c)
+        self.shot_planes = c.copy()
+        self.nsrc = self.nrec = 0
+        self._keep.clear()
 
     def set_refs(self, sref, labels):
         d, p = self.dims, self.params
-        _check(sref, (d.nz, p.nref, d.NW), name="sref")
-        _check(labels, (d.nz, d.NW, d.NY, d.NX), torch.uint8, name="labels")
+        _check(sref, (d.nz, p.nref, self.nmodel), name="sref")
+        _check(labels, (d.nz, self.nmodel, d.NY, d.NX), torch.uint8, name="labels")
         self._keep["refs"] = (sref, labels)
         call("fwix_born_set_refs", self._h.h, ptr(sref), ptr(labels))
 
     def set_slow(self, slow):
-        d = self.dims
-        _check(slow, (d.nz, d.NW, d.ny, d.nx), name="slowness")
+        _check(slow, self._model_shape(), name="slowness")
         self._keep["slow"] = slow
         call("fwix_born_set_slow", self._h.h, ptr(slow))
+
+    def set_frequencies(self, f_hz) -> None:
+        call("fwix_born_set_frequencies", self._h.h, _vp(_f32(f_hz, self.dims.NW, "frequencies")))
 
     def set_sources(self, x, y, z, ids):
         n = len(np.ravel(x))
@@ -336,34 +390,32 @@ class Born:
         self.nrec = n
 
     def set_wavelet(self, wavelet):
-        _check(wavelet, (self.nsrc, self.dims.NW), name="wavelet")
+        _check(wavelet, self._trace_shape(False), name="wavelet")
         self._keep["wavelet"] = wavelet
         call("fwix_born_set_wavelet", self._h.h, ptr(wavelet))
 
     def model(self, data: torch.Tensor | None = None, add: bool = False):
         if data is not None:
-            _check(data, (self.nrec, self.dims.NW), name="data")
+            _check(data, self._trace_shape(True), name="data")
         call("fwix_born_model", self._h.h, int(bool(add)), ptr(data), stream())
         return data
 
     def forward(self, add, dslow, data):
-        d = self.dims
-        _check(dslow, (d.nz, d.NW, d.ny, d.nx), name="model")
-        _check(data, (self.nrec, d.NW), name="data")
+        _check(dslow, self._model_shape(), name="model")
+        _check(data, self._trace_shape(True), name="data")
         call("fwix_born_forward", self._h.h, int(bool(add)), ptr(dslow), ptr(data), stream())
         return data
 
     def adjoint(self, add, dslow, data):
-        d = self.dims
-        _check(dslow, (d.nz, d.NW, d.ny, d.nx), name="model")
-        _check(data, (self.nrec, d.NW), name="data")
+        _check(dslow, self._model_shape(), name="model")
+        _check(data, self._trace_shape(True), name="data")
         call("fwix_born_adjoint", self._h.h, int(bool(add)), ptr(dslow), ptr(data), stream())
         return dslow
 
     def migrate(self, add, image, data):
         d = self.dims
         _check(image, (d.nz, d.ny, d.nx), name="image")
-        _check(data, (self.nrec, d.NW), name="data")
+        _check(data, self._trace_shape(True), name="data")
         call("fwix_born_migrate", self._h.h, int(bool(add)), ptr(image), ptr(data), stream())
         return image
 

@@ -14,7 +14,8 @@ import numpy as np
 
 from .refsampler import select_refs_depth
 
-__all__ = ["Workload", "born_workload", "ricker_spectrum", "CONFIGS"]
+__all__ = ["Workload", "born_workload", "ricker_spectrum", "CONFIGS", "SHOT_BATCH_CONFIGS", "uneven_shots",
+           "shot_models"]
 
 
 @dataclass
@@ -170,6 +171,45 @@ CONFIGS = {
 NSPS_CONFIGS = {
     "c3": dict(nx=384, nz=300, nw=100, fmin=2.0, fmax=50.0, nref=8, taper=20, seed=3),
 }
+
+# Batched multi-shot modeling + migration: 64 shots, square apertures U{96..512}
+# nodes padded to the next supported size >= 2n, nz 250, U{50..200} frequencies
+# per shot over 3-40 Hz, one Ricker 15 Hz source at the aperture centre,
+# receivers every 4th node on the deepest plane
+SHOT_BATCH_CONFIGS = {
+    "c4": dict(nshots=64, nmin=96, nmax=512, wmin=50, wmax=200, fmin=3.0, fmax=40.0, nz=250, d=10.0,
+               rec_step=4, nref=4, taper=20, seed=4),
+}
+
+
+def uneven_shots(nshots, nmin, nmax, wmin, wmax, fmin, fmax, nz, d, rec_step, seed, **_):
+    """Shots of uneven aperture and frequency count (window-local coordinates)."""
+    from .batching import Shot
+    rng = np.random.default_rng(seed)
+    out = []
+    for _k in range(nshots):
+        n = int(rng.integers(nmin, nmax + 1))
+        nw = int(rng.integers(wmin, wmax + 1))
+        f = (fmin + (fmax - fmin) * np.arange(nw) / max(nw - 1, 1)).astype(np.float32)
+        src = np.array([[n // 2 * d, n // 2 * d, 0.0]], np.float32)
+        gx, gy = np.meshgrid(np.arange(0, n, rec_step) * d, np.arange(0, n, rec_step) * d)
+        rec = np.stack([gx.ravel(), gy.ravel(), np.full(gx.size, (nz - 1) * d)], 1).astype(np.float32)
+        wav = ricker_spectrum(f.astype(np.float64))[None].astype(np.complex64)
+        out.append(Shot(n, n, f, src, wav, rec))
+    return out
+
+
+def shot_models(shot, nz, d):
+    """Background slowness (the smooth family of the other workloads on the
+    shot's window) and a model perturbation of three flat +-5 % reflectors at
+    1/4, 1/2, 3/4 of the depth; both (nz, ny, nx) float32."""
+    v = _velocity(shot.nx, shot.ny, nz, d, d, d)
+    s = (1.0 / v ** 2).astype(np.float32)
+    del v
+    ds = np.zeros_like(s)
+    for k, sign in zip((nz // 4, nz // 2, 3 * nz // 4), (1.0, -1.0, 1.0)):
+        ds[k] = sign * 0.05 * s[k]
+    return s, ds
 
 
 def workload_dims(w: Workload):

@@ -59,3 +59,32 @@ def test_streaming_split_rules():
     assert ginsu_window([55.0], [40.0], 0.0, 0.0, 30, 20, 0.0, 10.0, 0.0, 10.0) == (5, 6, 0, 19)  # y on a node
     assert ginsu_window([-100.0, 900.0], [0.0, 10.0], 0.0, 0.0, 30, 20, 0.0, 10.0, 0.0, 10.0)[:2] == (0, 29)
     assert (next_fft_size(41), next_fft_size(97), next_fft_size(8), next_fft_size(1000)) == (48, 128, 8, 1024)
+
+
+def test_shot_batch_planning_and_references():
+    """Shot-batch host logic: per-shot references equal the per-depth selector,
+    launches group one padded grid, fit the budget and never raise a shot's
+    checkpoint interval above what it needs alone."""
+    from pyfwix_amd.batching import Shot, launch_bytes, min_ckpt, padded_size, plan_launches, shot_refs
+    from pyfwix_amd.refsampler import select_refs_depth
+    rng = np.random.default_rng(3)
+    s3 = (1.0 / (1500.0 + 1000.0 * rng.random((5, 9, 11))) ** 2)
+    sref, lab = shot_refs(s3, 3, 24, 16)
+    for z in range(5):
+        r1, l1 = select_refs_depth(s3[z][None].astype(np.complex64), 3, 24 - 11, 16 - 9)
+        assert np.array_equal(sref[z], r1[:, 0]) and np.array_equal(lab[z], l1[0])
+    assert (padded_size(96), padded_size(97), padded_size(300), padded_size(512)) == (192, 256, 768, 1024)
+    shots = [Shot(n, n, np.arange(1, nw + 1, dtype=np.float32), np.zeros((1, 3)), np.zeros((1, nw)),
+                  np.zeros((4, 3))) for n, nw in ((100, 50), (120, 200), (300, 80), (260, 120), (128, 60))]
+    nz, budget = 250, 48 << 30
+    plan = plan_launches(shots, nz, budget)
+    assert sorted(i for idx, *_ in plan for i in idx) == list(range(len(shots)))
+    for idx, NX, NY, k in plan:
+        assert {padded_size(shots[i].nx) for i in idx} == {NX}
+        P = sum(shots[i].nw for i in idx)
+        nx = max(shots[i].nx for i in idx)
+        assert launch_bytes(P, len(idx), NX, NY, nx, nx, nz, k) <= budget
+        assert k <= max(min_ckpt(shots[i].nw, 1, NX, NY, shots[i].nx, shots[i].ny, nz, budget) for i in idx)
+    # the three 256-padded shots share a launch; the 768-padded ones are their own group
+    assert any(sorted(idx) == [0, 1, 4] for idx, *_ in plan)
+    assert launch_bytes(10, 1, 64, 64, 32, 32, 100, 1) > launch_bytes(10, 1, 64, 64, 32, 32, 100, 4)
