@@ -41,15 +41,20 @@ COPY_PROBE_TAG = "MulFunctor"       # the probe is y = 1.0 * x over float32
 
 
 def pass_kernel(name, w):
-    """Kernel (demangled name prefix) that a timed pass launches, for the c2-shaped PSPI/Born path."""
+    """Kernel (demangled name) that a timed pass launches, for the c2-shaped PSPI/Born path
+    (column kernels carry their register budget, row kernels the shot-map flag)."""
+    from pyfwix_amd import autotune
     nx, ny = w.NX, w.NY
-    return {"cols_fanout": f"fwix::k_cols_fanout<{ny}, 0>", "cols_fanin": f"fwix::k_cols_fanin<{ny}, 1>",
-            "born_cols_kfac": f"fwix::k_cols_fanout<{ny}, 2>", "born_cols_kfac_adj": f"fwix::k_cols_fanin<{ny}, 2>",
-            "rows_select": f"fwix::k_rows_select<{nx}>", "rows_fanout": f"fwix::k_rows_fanout<{nx}>",
-            "born_rows_scatter": f"fwix::k_rows_scatter<{nx}, true>",
-            "born_rows_inv_scatter_adj": f"fwix::k_rows_scatter_adj<{nx}, true>",
-            "born_rows_adj_ic": f"fwix::k_rows_adj_out<{nx}, true>",
-            "born_rows_ic": f"fwix::k_rows_ic_fwd<{nx}>"}.get(name)
+    b = lambda kind, mode: autotune.get(kind, mode, ny)  # noqa: E731
+    return {"cols_fanout": f"fwix::k_cols_fanout<{ny}, 0, {b(0, 0)}>",
+            "cols_fanin": f"fwix::k_cols_fanin<{ny}, 1, {b(1, 1)}>",
+            "born_cols_kfac": f"fwix::k_cols_fanout<{ny}, 2, {b(0, 2)}>",
+            "born_cols_kfac_adj": f"fwix::k_cols_fanin<{ny}, 2, {b(1, 2)}>",
+            "rows_select": f"fwix::k_rows_select<{nx}, false>", "rows_fanout": f"fwix::k_rows_fanout<{nx}, false>",
+            "born_rows_scatter": f"fwix::k_rows_scatter<{nx}, true, false>",
+            "born_rows_inv_scatter_adj": f"fwix::k_rows_scatter_adj<{nx}, true, false>",
+            "born_rows_adj_ic": f"fwix::k_rows_adj_out<{nx}, true, false>",
+            "born_rows_ic": f"fwix::k_rows_ic_fwd<{nx}, false>"}.get(name)
 
 
 def _csv_rows(path):
@@ -192,6 +197,20 @@ def plain_vs_fused(w, slow_dims, sref, labels, u, reps=5):
             "speedup": round(out["fused"] / out["plain"], 3)}
 
 
+def autotune_sizes(sizes):
+    """Tune the column kernels' register budget at every padded size (outside
+    the timed region); -> {size: {"chosen": ..., "us": {...}}, "seconds": t}."""
+    from pyfwix_amd import autotune
+    t0 = time.perf_counter()
+    out = {}
+    for n in sizes:
+        r = autotune.tune_size(int(n))
+        out[str(n)] = {"chosen": r["chosen"], "us": {k: v for k, v in r.items() if k != "chosen"}}
+        print(f"[tune] {n}: {r['chosen']}", file=sys.stderr, flush=True)
+    out["seconds"] = round(time.perf_counter() - t0, 2)
+    return out
+
+
 def bench_shot_batch(args):
     """Batched multi-shot modeling + migration (64 shots of uneven aperture and
     frequency count): every launch of the plan runs background modeling + Born
@@ -259,6 +278,7 @@ def bench_shot_batch(args):
             _lib.timing_select(None, 1)
         return total, steps, passes
 
+    tuned = None if args.no_autotune else autotune_sizes(sorted({NX for _i, NX, _NY, _k in plan}))
     # warm-up: one small shot, a few depths
     w0 = ShotBatch([shots[0]], 4, d, d, d, params, device=dev)
     w0.set_slowness([models[0][0][:4]])
@@ -371,6 +391,7 @@ def bench_shot_batch(args):
             "form": "the same fused kernels, one shot per operator and launch"},
         "batching_speedup": None if t_s is None else round(t_s / t_b, 3),
         "vs_plain_per_shot": sample,
+        "column_budgets": tuned,
         "host_model_prep_s": round(t_prep, 1),
         "cpu_baseline": None,
     }
@@ -392,6 +413,8 @@ def main():
                     help="planes per launch of the fused passes (0: the library's choice)")
     ap.add_argument("--no-pass-timing", action="store_true",
                     help="no event markers at all (counter-collection runs)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default column-kernel register budgets (no tuning before the timed region)")
     ap.add_argument("--shots", type=int, default=0, help="shot-batch configs: first n shots only (0: all)")
     ap.add_argument("--budget-gb", type=float, default=160.0, help="shot-batch configs: HBM per launch")
     ap.add_argument("--no-per-shot", action="store_true", help="shot-batch configs: skip the one-shot-per-launch leg")
@@ -432,6 +455,7 @@ def main():
         del x, y
     cfg = CONFIGS[args.config]
     w = born_workload(args.config, **cfg)
+    tuned = None if args.no_autotune else autotune_sizes([w.NY])
     slow, ds, sref, labels, wav = device_inputs(w, dev)
     dims = workload_dims(w)
     born = Born(dims, nref=w.nref, taperx=w.taper, tapery=w.taper, taylor=1, eps_scat=0.04, ckpt_every=1,
@@ -513,6 +537,7 @@ def main():
                    "receiver_depth_m": float(w.rec[2][0]), "depth_steps_per_step": per,
                    "planes_per_chunk": args.chunk or "auto", "checkpointing": "all depths in HBM"},
         "roofline": roof,
+        "column_budgets": tuned,
         "step_roofline": {"achieved": round(alg_total / wall / 1e9, 1), "unit": "GB/s",
                           "frac": round(alg_total / wall / 1e9 / HBM_PEAK_GBS, 4),
                           "bytes": "16P+labels per background step (+8P checkpoint), 24P+8m+labels per "

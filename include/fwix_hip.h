@@ -306,6 +306,18 @@ int fwix_timing_collect(void);
 int fwix_timing_reset(void);
 int fwix_timing_report(char* buf, int len);
 
+/* ---------------------------------------------------------------- column kernel tuning
+ * No counterpart in the reference (cuFFT picks its own kernels).  Every
+ * fused column kernel exists with three register budgets: 2, 4 or 6 waves
+ * per SIMD.  kind 0 = fan-out transforms (PSPI forward, NSPS adjoint,
+ * scattering terms), 1 = fan-in transforms; mode 0 / 1 = forward / adjoint
+ * phase shift, 2 = scattering terms; n = column length (NY).
+ * fwix_tune_columns_set returns 0, or -1 for an invalid argument;
+ * fwix_tune_columns_get returns the budget in use or -1.  The table is
+ * process-wide; pyfwix_amd/autotune.py sets it from timed launches. */
+int fwix_tune_columns_set(int kind, int mode, int n, int waves_per_simd);
+int fwix_tune_columns_get(int kind, int mode, int n);
+
 #ifdef __cplusplus
 }
 #endif

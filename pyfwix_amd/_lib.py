@@ -116,6 +116,8 @@ _SIGS = {
     "fwix_reflect_coef": [_D, _P, _P, _P, _P],
     "fwix_timing_enable": [_I],
     "fwix_timing_select": [C.c_char_p, _I],
+    "fwix_tune_columns_set": [_I, _I, _I, _I],
+    "fwix_tune_columns_get": [_I, _I, _I],
     "fwix_timing_overhead": [C.c_void_p, _I, C.POINTER(C.c_float)],
     "fwix_timing_collect": [],
     "fwix_timing_reset": [],

@@ -189,11 +189,13 @@ struct ScatterOp : fwix_op_s {
       const float2* cin = in + (size_t)c0 * plane;
       float2* cout = out + (size_t)c0 * plane;
       const int T = eng.params.taylor;
+      const double pb = 8.0 * np * plane;
       if (!adjoint) {  // Scatter.cpp:90-102
         if (T >= 1) {
           a.in = cin; a.out = a.K; a.add = 0; a.scale = sx;
           hip_check(launch_rows(0, a, st), "scatter rows");
           a.nterm = T; a.scale = sy2;
+          PassTimer t("scatter_cols", (1 + T) * pb, st);
           hip_check(launch_cols_fanout(2, a, st), "scatter columns");
         }
         a.in = cin; a.out = cout; a.add = add; a.nterm = T + 1; a.scale = sx;
@@ -203,6 +205,7 @@ struct ScatterOp : fwix_op_s {
         if (T >= 1) {
           hip_check(launch_rows_scatter_adj(false, a, st), "scatter adjoint rows");
           a.nterm = T; a.scale = sy2;
+          PassTimer t("scatter_cols_adj", (1 + T) * pb, st);
           hip_check(launch_cols_fanin(2, a, st), "scatter adjoint columns");
         }
         a.in = cin; a.out = cout; a.add = add; a.nterm = T + 1; a.scale = sx;

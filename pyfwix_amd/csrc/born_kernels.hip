@@ -62,7 +62,7 @@ __device__ __forceinline__ float taylor_coef(int it) {  // Scatter.h:73
 }
 
 // K = scale * F_x( U * ds ) on the model region (x < nx, y < ny), zero elsewhere
-template <int N>
+template <int N, bool MAP>
 __global__ __launch_bounds__(kThreads) void k_rows_ic_fwd(KArgs a) {
   using C = BRowCfg<N>;
   using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_ic_fwd(KArgs a) {
       const int p = lr / a.NY, iy = lr - p * a.NY;
       if (iy < a.ny && i < a.nx) {
         const float2 u = a.bg[(unsigned)(lr * N + i)];
-        const float2 m = a.dslow[(unsigned)((model_iw(a, plane_iw(a, p)) * a.ny + iy) * a.nx + i)];
+        const float2 m = a.dslow[(unsigned)((model_iw<MAP>(a, plane_iw(a, p)) * a.ny + iy) * a.nx + i)];
         v[e] = cmul(u, m);
       }
     }
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_ic_fwd(KArgs a) {
 // (BORN: U_z * ds_z on the model region, else the input rows).
 // BORN = true : K = scale * F_x( taper * (y + addin) )   (feeds the PSPI column pass)
 // BORN = false: out = (add ? out : 0) + y                (standalone Scatter forward)
-template <int N, bool BORN>
+template <int N, bool BORN, bool MAP>
 __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
   using C = BRowCfg<N>;
   using Fi = WaveFft<N, C::NB, false, true, PadLds<N>>;
@@ -125,12 +125,12 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
     inv[e] = czero();
     if (lr < nrows) {
       const int p = lr / a.NY, iy = lr - p * a.NY;
-      scatter_factors(a, model_iw(a, plane_iw(a, p)), iy, i, pw[e], inv[e]);
+      scatter_factors(a, model_iw<MAP>(a, plane_iw(a, p)), iy, i, pw[e], inv[e]);
       float2 v;
       if constexpr (BORN) {
         v = czero();
         if (iy < a.ny && i < a.nx)
-          v = cmul(a.bg[(unsigned)(lr * N + i)], a.dslow[(unsigned)((model_iw(a, plane_iw(a, p)) * a.ny + iy) * a.nx + i)]);
+          v = cmul(a.bg[(unsigned)(lr * N + i)], a.dslow[(unsigned)((model_iw<MAP>(a, plane_iw(a, p)) * a.ny + iy) * a.nx + i)]);
       } else {
         v = a.in[(unsigned)(lr * N + i)];
       }
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
 // FUSED = true : mu = taper * scale * F_x^-1(K) is the end of the PSPI adjoint
 //                step; it is written to out (the adjoint wavefield) as well.
 // FUSED = false: mu = in (standalone Scatter adjoint).
-template <int N, bool FUSED>
+template <int N, bool FUSED, bool MAP>
 __global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
   using C = BRowCfg<N>;
   using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
       const int p = lr / a.NY;
       const float h = 0.5f * a.w[plane_iw(a, p)] * a.dz;   // scatter.cu:197-238
       u[e] = make_float2(-h * u[e].y, h * u[e].x);
-      scatter_factors(a, model_iw(a, plane_iw(a, p)), lr - p * a.NY, i, pw[e], inv[e]);
+      scatter_factors(a, model_iw<MAP>(a, plane_iw(a, p)), lr - p * a.NY, i, pw[e], inv[e]);
       pw[e] = cmul(pw[e], inv[e]);                          // start at order 1
     }
   }
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
 //             is owned by exactly one lane: no atomics, deterministic (the
 //             reference's ic_adj uses float atomicAdd, imaging_condition.cu:103-104).
 // IC = false: out = (add ? out : 0) + x  (standalone Scatter adjoint).
-template <int N, bool IC>
+template <int N, bool IC, bool MAP>
 __global__ __launch_bounds__(kThreads) void k_rows_adj_out(KArgs a) {
   using C = BRowCfg<N>;
   using Fi = WaveFft<N, C::NB, false, true, PadLds<N>>;
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj_out(KArgs a) {
       const int iw = plane_iw(a, p);
       const unsigned idx = lr * N + i;
       float2 rsq, inv;
-      scatter_factors(a, model_iw(a, iw), iy, i, rsq, inv);
+      scatter_factors(a, model_iw<MAP>(a, iw), iy, i, rsq, inv);
       const float h = 0.5f * a.w[iw] * a.dz;
       const float2 d = a.in[idx];
       const float2 u = make_float2(-h * d.y, h * d.x);
@@ -575,36 +575,31 @@ static inline dim3 bflat(long long n) {
   return dim3((unsigned)g);
 }
 
-hipError_t launch_rows_ic_fwd(const KArgs& a, hipStream_t st) {
+// Born / scattering row passes; the shot-batch flavour (a.mplane set) is a
+// separate instantiation (MAP).  ScatterOp and the unfused forms never map.
+template <template <int, bool, bool> class K, bool FLAG>
+static hipError_t launch_mapped(const KArgs& a, hipStream_t st) {
   return bdispatch(a.NX, [&](auto nc) -> hipError_t {
     constexpr int N = decltype(nc)::value;
-    hipLaunchKernelGGL((k_rows_ic_fwd<N>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    if (a.mplane) hipLaunchKernelGGL((K<N, FLAG, true>::fn), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    else hipLaunchKernelGGL((K<N, FLAG, false>::fn), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
     return hipGetLastError();
   });
 }
+template <int N, bool F, bool M> struct KIcFwd { static constexpr auto fn = k_rows_ic_fwd<N, M>; };
+template <int N, bool F, bool M> struct KScatter { static constexpr auto fn = k_rows_scatter<N, F, M>; };
+template <int N, bool F, bool M> struct KScatterAdj { static constexpr auto fn = k_rows_scatter_adj<N, F, M>; };
+template <int N, bool F, bool M> struct KAdjOut { static constexpr auto fn = k_rows_adj_out<N, F, M>; };
+
+hipError_t launch_rows_ic_fwd(const KArgs& a, hipStream_t st) { return launch_mapped<KIcFwd, true>(a, st); }
 hipError_t launch_rows_scatter(bool born, const KArgs& a, hipStream_t st) {
-  return bdispatch(a.NX, [&](auto nc) -> hipError_t {
-    constexpr int N = decltype(nc)::value;
-    if (born) hipLaunchKernelGGL((k_rows_scatter<N, true>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
-    else hipLaunchKernelGGL((k_rows_scatter<N, false>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
-    return hipGetLastError();
-  });
+  return born ? launch_mapped<KScatter, true>(a, st) : launch_mapped<KScatter, false>(a, st);
 }
 hipError_t launch_rows_scatter_adj(bool fused, const KArgs& a, hipStream_t st) {
-  return bdispatch(a.NX, [&](auto nc) -> hipError_t {
-    constexpr int N = decltype(nc)::value;
-    if (fused) hipLaunchKernelGGL((k_rows_scatter_adj<N, true>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
-    else hipLaunchKernelGGL((k_rows_scatter_adj<N, false>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
-    return hipGetLastError();
-  });
+  return fused ? launch_mapped<KScatterAdj, true>(a, st) : launch_mapped<KScatterAdj, false>(a, st);
 }
 hipError_t launch_rows_adj_out(bool ic, const KArgs& a, hipStream_t st) {
-  return bdispatch(a.NX, [&](auto nc) -> hipError_t {
-    constexpr int N = decltype(nc)::value;
-    if (ic) hipLaunchKernelGGL((k_rows_adj_out<N, true>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
-    else hipLaunchKernelGGL((k_rows_adj_out<N, false>), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
-    return hipGetLastError();
-  });
+  return ic ? launch_mapped<KAdjOut, true>(a, st) : launch_mapped<KAdjOut, false>(a, st);
 }
 hipError_t launch_refl_src(int mode, const ReflArgs& a, hipStream_t st) {
   const dim3 g = bflat((long long)a.nx * a.ny * a.NW);

@@ -697,6 +697,13 @@ int fwix_timing_enable(int on) {
     g_timing = on > 0;
   });
 }
+int fwix_tune_columns_set(int kind, int mode, int n, int waves_per_simd) {
+  return guard([&] {
+    if (col_wpe_set(kind, mode, n, waves_per_simd) != 0)
+      throw Error("tune_columns: kind must be 0/1, mode 0..2, n a supported size, waves per SIMD 2, 4 or 6");
+  });
+}
+int fwix_tune_columns_get(int kind, int mode, int n) { return col_wpe_get(kind, mode, n); }
 int fwix_timing_select(const char* pass, int every) {
   return guard([&] {
     if (every < 1) throw Error("every must be >= 1");

@@ -16,6 +16,16 @@ constexpr int row_nb(int N) {
 // Column passes: 512-thread workgroups, 8 waves x CW/8 columns, with each
 // lane holding V = CW*N/512 values: 8 (2^a, N <= 512), 16 (1024) or 12 (3*2^a).
 // Two workgroups fit a CU, so one tile's loads overlap another's transforms.
+// Default register budget (waves per SIMD) of the column kernels, from timed
+// launches of every budget on MI355X: kind 0 the forward / fan-out
+// transforms, kind 1 the fan-in transforms; mode 2 the scattering terms.
+// (Every mode of a kind has the same winners so far; mode stays in the key.)
+constexpr int col_wpe_default(int kind, int mode, int N) {
+  const bool pow2 = !size_has3(N);
+  (void)mode;
+  if (kind == 0) return (pow2 && N <= 512) ? 6 : (N >= 384) ? 4 : 2;
+  return (pow2 && N <= 512) || N == 384 || N == 768 ? 4 : 2;
+}
 constexpr int col_cw(int N) {
   const int cw = (size_has3(N) ? 6144 : 4096) / N;
   return cw < 8 ? 8 : cw;
@@ -70,7 +80,16 @@ __device__ __forceinline__ int plane_iw(const KArgs& a, int p_local) { return (a
 // Index into the model-sized arrays (labels, presence masks, slowness, model
 // perturbation): plane iw itself, or the shot it belongs to when planes of
 // several shots share one launch.
-__device__ __forceinline__ int model_iw(const KArgs& a, int iw) { return a.mplane ? a.mplane[iw] : iw; }
+// MAP is a template flag so that the per-element index of the unbatched
+// kernels carries no branch (a per-element test of a.mplane split the
+// unrolled loads into separate blocks and cost the row passes 7-17 %).
+template <bool MAP>
+__device__ __forceinline__ int model_iw(const KArgs& a, int iw) {
+  if constexpr (MAP) return a.mplane[iw];
+  else return iw;
+}
+// once-per-workgroup form (column kernels)
+__device__ __forceinline__ int model_iw_any(const KArgs& a, int iw) { return a.mplane ? a.mplane[iw] : iw; }
 
 // lowest set bit of m above position r (32 when none); r = -1 gives the lowest
 __device__ __forceinline__ int next_bit(uint32_t m, int r) {

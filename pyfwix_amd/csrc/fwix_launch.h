@@ -74,6 +74,10 @@ hipError_t launch_ic_fwd(const float2* img, float2* D, const float2* U, int NX, 
                          int add, hipStream_t st);
 hipError_t launch_ic_adj(float2* img, const float2* D, const float2* U, int NX, int NY, int NW, int NS, int nx, int ny,
                          int add, hipStream_t st);
+// Register budget (2, 4 or 6 waves per SIMD) of the column kernels, kind 0
+// fan-out / 1 fan-in, per mode (0, 1, 2) and column length; -1 if invalid.
+int col_wpe_get(int kind, int mode, int n);
+int col_wpe_set(int kind, int mode, int n, int wpe);
 // out[k] += sum of g over the planes of shot k (first[k], count[k]); m per plane
 hipError_t launch_plane_sum(const float2* g, float2* out, const int* first, const int* count, int nshot, long long m,
                             hipStream_t st);

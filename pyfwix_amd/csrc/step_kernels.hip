@@ -44,14 +44,6 @@ struct ColCfg {
   static constexpr int TILE = N * CWP;
   static constexpr int PRIV = NB * padded(N);
   static constexpr int SMEM = TILE > WAVES * PRIV ? TILE : WAVES * PRIV;
-  // Waves per SIMD the register budget is sized for, per size, the largest
-  // that compiles without spills (measured): 6 (three workgroups per CU,
-  // 80 VGPRs) for the forward / fan-out transforms of 2^a <= 128 and 512,
-  // 4 (two workgroups, 128 VGPRs) for 256 and for every fan-in (which
-  // spills at 80), 2 (256 VGPRs) for the 3 * 2^a sizes and 1024.
-  static constexpr bool POW2 = !size_has3(N);
-  static constexpr int WPE = (POW2 && (N <= 128 || N == 512)) ? 6 : (POW2 && N <= 256) ? 4 : 2;
-  static constexpr int WPE_IN = (POW2 && N <= 512) ? 4 : 2;
   static_assert(NB >= 1 && V >= 4, "column tile too narrow");
 };
 
@@ -119,7 +111,7 @@ __global__ __launch_bounds__(kThreads) void k_rows(KArgs a) {
 }
 
 // rows_fanout: W_r = scale * F_x([labels == r] * in) for every reference r
-template <int N>
+template <int N, bool MAP>
 __global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
   using C = RowCfg<N>;
   using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
@@ -141,7 +133,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
     if (lr < nrows) {
       u[e] = a.in[(unsigned)(lr * N + i)];
       const int p = lr / a.NY, iy = lr - p * a.NY;
-      lab[e] = a.labels[(unsigned)((model_iw(a, plane_iw(a, p)) * a.NY + iy) * N + i)];
+      lab[e] = a.labels[(unsigned)((model_iw<MAP>(a, plane_iw(a, p)) * a.NY + iy) * N + i)];
     }
   }
   const float s = a.scale;
@@ -152,7 +144,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
       const int lr = rb + b;
       if (lr < nrows) {
         const int p = lr / a.NY;
-        wm |= a.rowmask[model_iw(a, plane_iw(a, p)) * a.NY + (lr - p * a.NY)];
+        wm |= a.rowmask[model_iw<MAP>(a, plane_iw(a, p)) * a.NY + (lr - p * a.NY)];
       }
     }
     wm = __builtin_amdgcn_readfirstlane(wm);
@@ -175,7 +167,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
 }
 
 // rows_select: out = (add ? out : 0) + sum_r [labels == r] * scale * F_x^-1(W_r)
-template <int N>
+template <int N, bool MAP>
 __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
   using C = RowCfg<N>;
   using F = WaveFft<N, C::NB, false, true, PadLds<N>>;
@@ -196,7 +188,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
     lab[e] = 255;
     if (lr < nrows) {
       const int p = lr / a.NY, iy = lr - p * a.NY;
-      lab[e] = a.labels[(unsigned)((model_iw(a, plane_iw(a, p)) * a.NY + iy) * N + i)];
+      lab[e] = a.labels[(unsigned)((model_iw<MAP>(a, plane_iw(a, p)) * a.NY + iy) * N + i)];
     }
   }
   // references present in this wave's rows (wave-uniform)
@@ -207,7 +199,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
       const int lr = rb + b;
       if (lr < nrows) {
         const int p = lr / a.NY;
-        wm |= a.rowmask[model_iw(a, plane_iw(a, p)) * a.NY + (lr - p * a.NY)];
+        wm |= a.rowmask[model_iw<MAP>(a, plane_iw(a, p)) * a.NY + (lr - p * a.NY)];
       }
     }
     wm = __builtin_amdgcn_readfirstlane(wm);
@@ -433,7 +425,7 @@ __device__ __forceinline__ void tile_write(float2* tile, const float2* v, int wa
 
 // out = (add ? out : 0) + scale * F_y(in) on each plane (plain 2-D FFT column pass)
 template <int N, bool INV>
-__global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(ColCfg<N>::WPE))) void k_cols(KArgs a) {
+__global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(col_wpe_default(0, 0, N)))) void k_cols(KArgs a) {
   using C = ColCfg<N>;
   using F = WaveFft<N, C::NB, false, INV, PadLds<N>>;
   __shared__ float2 smem[C::SMEM];
@@ -459,8 +451,8 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
 }
 
 // cols_fanout: W_r = F_y^-1[M_r * F_y(K)] / NY for every term r (M_r: term_mult<MODE>).
-template <int N, int MODE>
-__global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(ColCfg<N>::WPE))) void k_cols_fanout(KArgs a) {
+template <int N, int MODE, int WPE>
+__global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_cols_fanout(KArgs a) {
   using C = ColCfg<N>;
   using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
   using Fi = WaveFft<N, C::NB, false, true, PadLds<N>>;
@@ -482,7 +474,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   const float w2 = a.w2[iw];
   const float s2 = a.scale;  // 1/NY: forward and inverse column normalisation together
   const int colw = c0 + wave * C::NB;
-  const int im = model_iw(a, iw);
+  const int im = model_iw_any(a, iw);
   const uint32_t* rm = (MODE != 2 && a.rowmask) ? a.rowmask + im * a.NY : nullptr;
   const uint32_t pm = (MODE != 2 && a.planemask) ? a.planemask[im] : 0xffffffffu;
   __shared__ uint32_t rmk[C::NROW];
@@ -511,8 +503,8 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
 }
 
 // cols_fanin: K = F_y^-1[sum_r M_r * F_y(W_r)] / NY (M_r: term_mult<MODE>).
-template <int N, int MODE>
-__global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(ColCfg<N>::WPE_IN))) void k_cols_fanin(KArgs a) {
+template <int N, int MODE, int WPE>
+__global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_cols_fanin(KArgs a) {
   using C = ColCfg<N>;
   using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
   using Fi = WaveFft<N, C::NB, false, true, PadLds<N>>;
@@ -530,7 +522,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   float2 acc[C::V];
 #pragma unroll
   for (int e = 0; e < C::V; ++e) acc[e] = czero();
-  const int im = model_iw(a, iw);
+  const int im = model_iw_any(a, iw);
   const uint32_t* rm = (MODE != 2 && a.rowmask) ? a.rowmask + im * a.NY : nullptr;
   const uint32_t pm = (MODE != 2 && a.planemask) ? a.planemask[im] : 0xffffffffu;
   const uint32_t live = a.nterm >= 32 ? pm : (pm & ((1u << a.nterm) - 1u));
@@ -681,7 +673,8 @@ hipError_t launch_rows(int kind, const KArgs& a, hipStream_t st) {
 hipError_t launch_rows_fanout(const KArgs& a, hipStream_t st) {
   return dispatch_n(a.NX, [&](auto nc) -> hipError_t {
     constexpr int N = decltype(nc)::value;
-    hipLaunchKernelGGL((k_rows_fanout<N>), row_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    if (a.mplane) hipLaunchKernelGGL((k_rows_fanout<N, true>), row_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    else hipLaunchKernelGGL((k_rows_fanout<N, false>), row_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
     return hipGetLastError();
   });
 }
@@ -689,7 +682,8 @@ hipError_t launch_rows_fanout(const KArgs& a, hipStream_t st) {
 hipError_t launch_rows_select(const KArgs& a, hipStream_t st) {
   return dispatch_n(a.NX, [&](auto nc) -> hipError_t {
     constexpr int N = decltype(nc)::value;
-    hipLaunchKernelGGL((k_rows_select<N>), row_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    if (a.mplane) hipLaunchKernelGGL((k_rows_select<N, true>), row_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    else hipLaunchKernelGGL((k_rows_select<N, false>), row_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
     return hipGetLastError();
   });
 }
@@ -704,14 +698,65 @@ hipError_t launch_cols(bool inv, const KArgs& a, hipStream_t st) {
   });
 }
 
+// ------------------------------------------------------------------ occupancy variants
+// Every fan-out / fan-in column kernel is built for three register budgets:
+// 2, 4 or 6 waves per SIMD (256, 128 or 80 VGPRs).  More waves hide more
+// latency but may spill; which wins depends on the size and the transform,
+// so the choice is a runtime table (kernel, mode, size) that an autotuner
+// sets from timed launches; the defaults (col_wpe_default) are the winners
+// of such timings.
+static constexpr int kSizeList[] = {8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024};
+static constexpr int kNumSizes = sizeof(kSizeList) / sizeof(int);
+static int size_slot(int n) {
+  for (int i = 0; i < kNumSizes; ++i)
+    if (kSizeList[i] == n) return i;
+  return -1;
+}
+static int g_wpe[2][3][kNumSizes];
+static bool g_wpe_ready = false;
+static void wpe_init() {
+  if (g_wpe_ready) return;
+  for (int k = 0; k < 2; ++k)
+    for (int m = 0; m < 3; ++m)
+      for (int i = 0; i < kNumSizes; ++i) g_wpe[k][m][i] = col_wpe_default(k, m, kSizeList[i]);
+  g_wpe_ready = true;
+}
+int col_wpe_get(int kind, int mode, int n) {
+  const int i = size_slot(n);
+  if (kind < 0 || kind > 1 || mode < 0 || mode > 2 || i < 0) return -1;
+  wpe_init();
+  return g_wpe[kind][mode][i];
+}
+int col_wpe_set(int kind, int mode, int n, int wpe) {
+  const int i = size_slot(n);
+  if (kind < 0 || kind > 1 || mode < 0 || mode > 2 || i < 0) return -1;
+  if (wpe != 2 && wpe != 4 && wpe != 6) return -1;
+  wpe_init();
+  g_wpe[kind][mode][i] = wpe;
+  return 0;
+}
+
+template <class Fn>
+static hipError_t dispatch_wpe(int wpe, Fn&& fn) {
+  switch (wpe) {
+    case 2: return fn(std::integral_constant<int, 2>{});
+    case 4: return fn(std::integral_constant<int, 4>{});
+    case 6: return fn(std::integral_constant<int, 6>{});
+    default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_cols_fanout(int mode, const KArgs& a, hipStream_t st) {
   return dispatch_n(a.NY, [&](auto nc) -> hipError_t {
     constexpr int N = decltype(nc)::value;
     const dim3 g = col_grid(N, a.NX, a.np);
-    if (mode == 0) hipLaunchKernelGGL((k_cols_fanout<N, 0>), g, dim3(kColThreads), 0, st, a);
-    else if (mode == 1) hipLaunchKernelGGL((k_cols_fanout<N, 1>), g, dim3(kColThreads), 0, st, a);
-    else hipLaunchKernelGGL((k_cols_fanout<N, 2>), g, dim3(kColThreads), 0, st, a);
-    return hipGetLastError();
+    return dispatch_wpe(col_wpe_get(0, mode, N), [&](auto wc) -> hipError_t {
+      constexpr int W = decltype(wc)::value;
+      if (mode == 0) hipLaunchKernelGGL((k_cols_fanout<N, 0, W>), g, dim3(kColThreads), 0, st, a);
+      else if (mode == 1) hipLaunchKernelGGL((k_cols_fanout<N, 1, W>), g, dim3(kColThreads), 0, st, a);
+      else hipLaunchKernelGGL((k_cols_fanout<N, 2, W>), g, dim3(kColThreads), 0, st, a);
+      return hipGetLastError();
+    });
   });
 }
 
@@ -719,10 +764,13 @@ hipError_t launch_cols_fanin(int mode, const KArgs& a, hipStream_t st) {
   return dispatch_n(a.NY, [&](auto nc) -> hipError_t {
     constexpr int N = decltype(nc)::value;
     const dim3 g = col_grid(N, a.NX, a.np);
-    if (mode == 0) hipLaunchKernelGGL((k_cols_fanin<N, 0>), g, dim3(kColThreads), 0, st, a);
-    else if (mode == 1) hipLaunchKernelGGL((k_cols_fanin<N, 1>), g, dim3(kColThreads), 0, st, a);
-    else hipLaunchKernelGGL((k_cols_fanin<N, 2>), g, dim3(kColThreads), 0, st, a);
-    return hipGetLastError();
+    return dispatch_wpe(col_wpe_get(1, mode, N), [&](auto wc) -> hipError_t {
+      constexpr int W = decltype(wc)::value;
+      if (mode == 0) hipLaunchKernelGGL((k_cols_fanin<N, 0, W>), g, dim3(kColThreads), 0, st, a);
+      else if (mode == 1) hipLaunchKernelGGL((k_cols_fanin<N, 1, W>), g, dim3(kColThreads), 0, st, a);
+      else hipLaunchKernelGGL((k_cols_fanin<N, 2, W>), g, dim3(kColThreads), 0, st, a);
+      return hipGetLastError();
+    });
   });
 }
 

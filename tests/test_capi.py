@@ -108,3 +108,21 @@ def test_refsampler_kmeans_is_seeded_and_finds_clusters():
     d = np.abs(s[0, 0].reshape(-1, 1) - r1[0, :, 0][None])
     assert np.array_equal(l1[0, 0, :30, :40].ravel(), d.argmin(axis=1))  # nearest-center labels
     assert np.array_equal(l1[0, 0, 30:, :40], np.repeat(l1[0, 0, 29:30, :40], 2, axis=0))
+
+
+def test_column_budget_table():
+    """Register-budget table of the column kernels: defaults, set/get, validation."""
+    from pyfwix_amd import autotune
+    assert autotune.get(0, 0, 512) == 6 and autotune.get(1, 1, 512) == 4 and autotune.get(0, 0, 1024) == 4
+    assert autotune.get(1, 0, 1024) == 2 and autotune.get(1, 2, 768) == 4 and autotune.get(0, 1, 384) == 4
+    old = autotune.get(1, 2, 768)
+    try:
+        autotune.set_budget(1, 2, 768, 4)
+        assert autotune.get(1, 2, 768) == 4 and autotune.table((768,))["768"]["fanin"][2] == 4
+    finally:
+        autotune.set_budget(1, 2, 768, old)
+    for bad in ((2, 0, 512, 4), (0, 3, 512, 4), (0, 0, 500, 4), (0, 0, 512, 3)):
+        with pytest.raises(RuntimeError, match="tune_columns"):
+            autotune.set_budget(*bad)
+    with pytest.raises(ValueError):
+        autotune.get(0, 0, 500)
