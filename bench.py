@@ -413,6 +413,8 @@ def main():
                     help="planes per launch of the fused passes (0: the library's choice)")
     ap.add_argument("--no-pass-timing", action="store_true",
                     help="no event markers at all (counter-collection runs)")
+    ap.add_argument("--graph", action="store_true",
+                    help="also time the step captured as one HIP graph (reported as `graph`)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default column-kernel register budgets (no tuning before the timed region)")
     ap.add_argument("--shots", type=int, default=0, help="shot-batch configs: first n shots only (0: all)")
@@ -500,6 +502,21 @@ def main():
     per = steps_exec // args.steps
     value = steps_exec * w.points / wall / 1e9
     ms_step = wall / args.steps * 1e3
+    graph = None
+    if args.graph:
+        from pyfwix_amd.graph import StepGraph
+        g = StepGraph(step)
+        g.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g.replay()
+        torch.cuda.synchronize()
+        gw = time.perf_counter() - t0
+        graph = {"value": round(per * args.steps * w.points / gw / 1e9, 4), "ms_per_step": round(gw / args.steps * 1e3, 3),
+                 "form": "background + forward + adjoint captured once (torch.cuda.CUDAGraph on the library's launches), "
+                         "replayed per step"}
+        del g
 
     roof = None
     if passes:
@@ -537,6 +554,7 @@ def main():
                    "receiver_depth_m": float(w.rec[2][0]), "depth_steps_per_step": per,
                    "planes_per_chunk": args.chunk or "auto", "checkpointing": "all depths in HBM"},
         "roofline": roof,
+        "graph": graph,
         "column_budgets": tuned,
         "step_roofline": {"achieved": round(alg_total / wall / 1e9, 1), "unit": "GB/s",
                           "frac": round(alg_total / wall / 1e9 / HBM_PEAK_GBS, 4),

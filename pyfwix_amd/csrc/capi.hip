@@ -58,6 +58,9 @@ hipEvent_t take_event() {
 
 PassTimer::PassTimer(const char* name, double bytes, hipStream_t s) {
   if (!g_timing) return;
+  // no event pairs inside a captured graph: its replays would never be collected
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return;
   std::lock_guard<std::mutex> g(g_tmu);
   const long long k = g_seen[name]++;
   if (!g_only.empty() && g_only != name) return;
