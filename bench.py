@@ -130,6 +130,21 @@ def traffic_for(config, kernel):
     return None
 
 
+VALU_PEAK_GINSTR = 614.4   # wave64 VALU instructions per second (1e9): 256 CUs x 1 per cycle x 2.4 GHz
+
+
+def valu_for(config, kernel):
+    """VALU wave-instructions per dispatch of `kernel` from profiles/valu_<config>.md, or None."""
+    path = os.path.join(ROOT, "profiles", f"valu_{config}.md")
+    if kernel is None or not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 7 and cells[0] == f"`{kernel}`":
+            return float(cells[2])
+    return None
+
+
 def _step_alg_bytes(w, steps_bg, steps_fwd, steps_adj):
     """Compulsory HBM bytes of the executed depth steps (labels as uint8)."""
     P = w.points
@@ -527,9 +542,13 @@ def main():
         avg_us = dom["ms"] * 1e3 / dom["launches"]
         per_launch = dom["bytes"] / dom["launches"]
         achieved = per_launch / (avg_us * 1e-6) / 1e9
+        vi = valu_for(args.config, pass_kernel(dom_name, w))
         roof = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic_for(args.config, pass_kernel(dom_name, w)),
+                "valu_issue": None if vi is None else {
+                    "instr_per_launch": vi, "achieved": round(vi / avg_us / 1e3, 1), "peak": VALU_PEAK_GINSTR,
+                    "unit": "G wave-instr/s", "frac": round(vi / avg_us / 1e3 / VALU_PEAK_GINSTR, 4)},
                 "alg_bytes_per_launch": round(per_launch), "avg_us": round(avg_us, 2),
                 "launches_timed": dom["launches"], "launches": dom["total_launches"],
                 "empty_event_pair_us": round(ev_us, 2)}
