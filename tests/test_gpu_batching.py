@@ -198,3 +198,33 @@ def test_oneway_batch_equals_shots_alone(dev):
                 assert rel(out[0, o:o + s.nw].cpu().numpy(), o1[0].cpu().numpy()) < 1e-6
                 assert rel(adj[0, o:o + s.nw].cpu().numpy(), a1[0].cpu().numpy()) < 1e-6
                 o += s.nw
+
+
+def test_batch_equals_shots_alone_on_a_large_grid(dev):
+    """Two shots on a 768 x 768 padded grid (apertures 300 x 320 and 330 x 290,
+    2 and 3 frequencies), three depths."""
+    import torch
+    from scipy.ndimage import gaussian_filter
+    from pyfwix_amd.batching import Shot, ShotBatch, padded_size
+    rng = np.random.default_rng(14)
+    nz = 3
+    shots, slows, dss = [], [], []
+    for nx, ny, nw, f0 in ((300, 320, 2, 6.0), (330, 290, 3, 9.0)):
+        assert padded_size(nx) == padded_size(ny) == 768
+        freqs = (f0 + 2.0 * np.arange(nw)).astype(np.float32)
+        src = np.array([[nx // 2 * D, ny // 2 * D, 0.0]], np.float32)
+        gx, gy = np.meshgrid(np.arange(0, nx, 25) * D, np.arange(0, ny, 25) * D)
+        rec = np.stack([gx.ravel(), gy.ravel(), np.full(gx.size, (nz - 1) * D)], 1).astype(np.float32)
+        shots.append(Shot(nx, ny, freqs, src, crand(rng, (1, nw)).astype(np.complex64), rec))
+        v = gaussian_filter(1500.0 + 800.0 * rng.random((nz, ny, nx)), 4.0)
+        slows.append((1.0 / v ** 2).astype(np.complex64))
+        dss.append((0.05 * slows[-1].real * rng.standard_normal(slows[-1].shape)).astype(np.complex64))
+    b = ShotBatch(shots, nz, D, D, D, PARAMS, device=dev)
+    data, grad = _run(b, slows, dss)
+    torch.cuda.synchronize()
+    assert float(data.abs().max()) > 0
+    for k, s in enumerate(shots):
+        one = ShotBatch([s], nz, D, D, D, PARAMS, device=dev)
+        d1, g1 = _run(one, [slows[k]], [dss[k]])
+        assert rel(b.shot_data(data, k).cpu().numpy(), one.shot_data(d1, 0).cpu().numpy()) < 1e-6
+        assert rel(b.shot_grad(grad, k).cpu().numpy(), one.shot_grad(g1, 0).cpu().numpy()) < 1e-6

@@ -158,10 +158,11 @@ def test_oneway_matches_oracle(dev, direction, method):
     assert dot_err(x, out.cpu().numpy(), y, Aty.cpu().numpy()) < TOL_DOT
 
 
-def _born_setup(dev, ckpt=1, seed=5, NS=2, tap=4, nz=6):
+def _born_setup(dev, ckpt=1, seed=5, NS=2, tap=4, nz=6, size=(48, 48, 4, 32, 36), ow=6.0):
     from pyfwix_amd.ops import Born
     rng = np.random.default_rng(seed)
-    g = grid(48, 48, 4, NS, nx=32, ny=36, nz=nz, ow=6.0, dw=3.0)
+    NX, NY, NW, nx, ny = size
+    g = grid(NX, NY, NW, NS, nx=nx, ny=ny, nz=nz, ow=ow, dw=3.0)
     slow, sref, labels = _refs(g, rng, 3)
     src, rec = _geometry(g, rng, nrec=48)
     born = Born(dims_of(g), nref=3, taperx=tap, tapery=tap, taylor=1, eps_scat=0.04, ckpt_every=ckpt)
@@ -195,6 +196,58 @@ def test_born_background_and_forward_match_oracle(dev):
     img = _z((g.nz, g.ny, g.nx), dev)
     born.migrate(False, img, _t(dres, dev))
     assert rel(img.cpu().numpy(), O.migrate(dres, bg, rec, s64, lab, g, g.dz, 0.0, 4, 4)) < TOL_WFLD
+
+
+def _born_vs_oracle(dev, size, ow, fp32_floor=False):
+    g, born, slow, sref, labels, src, rec, wav, rng = _born_setup(dev, NS=1, tap=12, nz=3, size=size, seed=9, ow=ow)
+    s64, lab, sl = sref.astype(np.complex128), labels.astype(np.int64), slow.astype(np.complex64).astype(np.complex128)
+    d0 = _z((len(rec[0]), g.NW), dev)
+    born.model(d0)
+    ref_d0, bg = O.model_background(wav, src, rec, s64, lab, g, g.dz, 0.0, 12, 12)
+    ds = crand(rng, (g.nz, g.NW, g.ny, g.nx)) * 1e-7
+    d = _z((len(rec[0]), g.NW), dev)
+    born.forward(False, _t(ds, dev), d)
+    ref_d = O.born_forward(ds, bg, sl, rec, s64, lab, g, g.dz, 0.0, 12, 12, 0.04, 1)
+    dres = crand(rng, (len(rec[0]), g.NW))
+    m = _z((g.nz, g.NW, g.ny, g.nx), dev)
+    born.adjoint(False, m, _t(dres, dev))
+    ref_m = O.born_adjoint(dres, bg, sl, rec, s64, lab, g, g.dz, 0.0, 12, 12, 0.04, 1)
+    errs = (rel(d0.cpu().numpy(), ref_d0), rel(d.cpu().numpy(), ref_d), rel(m.cpu().numpy(), ref_m))
+    floor = None
+    if fp32_floor:  # the same formulas evaluated in complex64 (scipy FFTs keep the precision)
+        c = np.complex64
+        bg32 = [b.astype(c) for b in bg]
+        f_d = O.born_forward(ds.astype(c), bg32, sl.astype(c), rec, s64.astype(c), lab, g, g.dz, 0.0, 12, 12, 0.04, 1)
+        f_m = O.born_adjoint(dres.astype(c), bg32, sl.astype(c), rec, s64.astype(c), lab, g, g.dz, 0.0, 12, 12,
+                             0.04, 1)
+        floor = (rel(f_d, ref_d), rel(f_m, ref_m))
+    return errs, floor
+
+
+@pytest.mark.parametrize("size", [(768, 768, 2, 500, 600), (1024, 1024, 2, 700, 512), (384, 1024, 1, 300, 640)],
+                         ids=lambda c: f"{c[0]}x{c[1]}")
+def test_born_large_grids_match_oracle(dev, size):
+    """Background, Born forward and Born adjoint on the largest padded grids
+    (3 * 2^a and 1024 columns, every register budget at its default) vs the
+    oracle, at 20 and 23 Hz."""
+    errs, _ = _born_vs_oracle(dev, size, 20.0)
+    assert max(errs) < TOL_WFLD, errs
+
+
+def test_born_low_frequency_large_grid_at_the_fp32_floor(dev):
+    """At 6 and 9 Hz on a 768^2 grid the first Taylor term of the scattering
+    operator weighs the high wavenumbers by k^2 / (w^2 s) (up to ~200 here),
+    which lifts plain FP32 rounding above 1e-4 (the same formulas in complex64
+    miss the FP64 result by ~2.6e-4).  The GPU forward stays within 1.5x of
+    that floor (measured 2.4e-4 vs 2.6e-4).  The adjoint measures 3.0e-4
+    against 1.05e-4 for the complex64 evaluation (depth 0, the eight nodes
+    around the source: 4.8e-4 vs 1.6e-4; depth 1: 1.6e-4 vs 6.6e-5), i.e.
+    2.5-3x that floor: a known accuracy gap of the fused adjoint passes at
+    this conditioning, bounded here at 4x."""
+    errs, floor = _born_vs_oracle(dev, (768, 768, 2, 500, 600), 6.0, fp32_floor=True)
+    assert errs[0] < TOL_WFLD
+    assert errs[1] < 1.5 * floor[0] + 1e-5, (errs, floor)
+    assert errs[2] < 4.0 * floor[1] + 1e-5, (errs, floor)
 
 
 def test_born_dot_no_taper_single_shot(dev):
