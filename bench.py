@@ -413,6 +413,136 @@ def bench_shot_batch(args):
     print(json.dumps(res), flush=True)
 
 
+def bench_fwi(args):
+    """FWI-style slowness gradient: per launch (a group of shots that fits the
+    HBM budget), background modeling on the start model with checkpoints every
+    `ckpt` depths, the L2 data residual against data modeled on the true model,
+    and the Born adjoint (migration) of the residual with segment recomputation;
+    the gradient is summed over shots.  Reports seconds per shot."""
+    import torch
+    from pyfwix_amd.batching import ShotBatch, launch_bytes, padded_size, shot_refs
+    from pyfwix_amd.ops import FWIX_PSPI, FWIX_PSPI_PLAIN, OneStep, make_dims
+    from pyfwix_amd.workloads import FWI_CONFIGS, fwi_models, grid_shots
+
+    cfg = FWI_CONFIGS[args.config]
+    dev = torch.device("cuda:0")
+    nz, d, nref, taper, ck = cfg["nz"], cfg["d"], cfg["nref"], cfg["taper"], cfg["ckpt"]
+    params = {"nref": nref, "taper": taper, "eps": 0.0, "eps_scat": 0.04, "taylor": 1}
+    t_prep = time.perf_counter()
+    shots = grid_shots(**cfg)
+    if args.shots:
+        shots = shots[:args.shots]
+    s_true, s_start = fwi_models(**cfg)
+    N = padded_size(cfg["nx"])
+    refs_true = shot_refs(s_true, nref, N, N)
+    refs_start = shot_refs(s_start, nref, N, N)
+    t_prep = time.perf_counter() - t_prep
+    budget = int(args.budget_gb * (1 << 30))
+    nw = shots[0].nw
+    per = 1
+    while per < len(shots) and launch_bytes(nw * (per + 1), per + 1, N, N, cfg["nx"], cfg["nx"], nz, ck) <= budget:
+        per += 1
+    groups = [list(range(i, min(i + per, len(shots)))) for i in range(0, len(shots), per)]
+    print(f"[fwi] {len(shots)} shots in {len(groups)} launches of <= {per}, host models {t_prep:.1f} s",
+          file=sys.stderr, flush=True)
+    tuned = None if args.no_autotune else autotune_sizes([N])
+
+    def batch(idx, s3, refs):
+        b = ShotBatch([shots[i] for i in idx], nz, d, d, d, params, ckpt_every=ck, device=dev, NX=N, NY=N)
+        b.set_slowness([s3] * len(idx), [refs] * len(idx))
+        return b
+
+    # observed data on the true model (outside the timed region)
+    observed = []
+    for idx in groups:
+        b = batch(idx, s_true, refs_true)
+        dobs = torch.zeros(b.nsamp, dtype=torch.complex64, device=dev)
+        b.op.model(dobs)
+        torch.cuda.synchronize()
+        observed.append(dobs)
+        del b
+        torch.cuda.empty_cache()
+    grad = torch.zeros((nz, cfg["nx"], cfg["nx"]), dtype=torch.complex64, device=dev)
+    wall, steps, misfit = 0.0, 0, 0.0
+    for idx, dobs in zip(groups, observed):
+        b = batch(idx, s_start, refs_start)
+        data = torch.zeros(b.nsamp, dtype=torch.complex64, device=dev)
+        g = torch.zeros((nz, len(idx), b.ny, b.nx), dtype=torch.complex64, device=dev)
+        torch.cuda.synchronize()
+        s0 = b.op.steps
+        t0 = time.perf_counter()
+        b.op.model(data)                          # background on the start model, data recorded
+        data.sub_(dobs)                           # L2 residual
+        b.op.adjoint(False, g, data)              # gradient of 0.5 |d - d_obs|^2 (Born adjoint)
+        grad.add_(g.sum(1))
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        wall += dt
+        steps += b.op.steps - s0
+        misfit += 0.5 * float(torch.linalg.vector_norm(data) ** 2)
+        print(f"[fwi] launch of {len(idx)} shots ({b.planes} planes, ckpt {ck}): {dt:.3f} s", file=sys.stderr,
+              flush=True)
+        del b, data, g
+        torch.cuda.empty_cache()
+    pts = sum(s.nw for s in shots) * N * N
+    nominal = 2 * (nz - 1)
+    # PSPI forward + adjoint step, fused vs the plain per-operator kernels, on one shot's planes
+    sample = None
+    if not args.no_plain:
+        rates = {}
+        dims = make_dims(N, N, nw, 1, cfg["nx"], cfg["nx"], 0.0, d, 0.0, d, 1.0, 0.0, 0.0, d, 2)
+        sref = torch.from_numpy(np.ascontiguousarray(np.repeat(refs_start[0][:2, :, None], nw, 2))).to(dev)
+        lab = torch.from_numpy(np.ascontiguousarray(np.repeat(refs_start[1][:2, None], nw, 1))).to(dev)
+        u = torch.randn((1, nw, N, N), dtype=torch.complex64, device=dev)
+        for name, method in (("fused", FWIX_PSPI), ("plain", FWIX_PSPI_PLAIN)):
+            op = OneStep(method, dims, nref=nref, taperx=taper, tapery=taper)
+            op.set_frequencies(shots[0].freqs)
+            op.set_refs(sref, lab)
+            op.set_depth(1)
+            op.forward(False, u, u)
+            op.adjoint(False, u, u)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                op.forward(False, u, u)
+                op.adjoint(False, u, u)
+            torch.cuda.synchronize()
+            rates[name] = 6 * nw * N * N / (time.perf_counter() - t0) / 1e9
+            del op
+        sample = {"fused_gpts": round(rates["fused"], 3), "plain_gpts": round(rates["plain"], 3),
+                  "speedup": round(rates["fused"] / rates["plain"], 3),
+                  "sample": f"PSPI forward + adjoint at one depth of one shot ({N}x{N}x{nw}), 3 repetitions"}
+    res = {
+        "metric": "FWI grad s/shot",
+        "value": round(wall / len(shots), 4),
+        "unit": "s/shot",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": round(wall * 1e3, 1),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex64",
+        "data": "synthetic (smooth background + seeded Gaussian anomalies; start model smoothed in closed form)",
+        "config": {"workload": f"{args.config}: FWI slowness gradient (background modeling with checkpoints, L2 "
+                               f"residual, Born adjoint with recomputation), {len(shots)} shots",
+                   "nx": cfg["nx"], "ny": cfg["nx"], "nz": nz, "NX": N, "NY": N, "nw": nw,
+                   "band_hz": [cfg["fmin"], cfg["fmax"]], "nref": nref, "taper": taper, "ckpt_every": ck,
+                   "receivers": f"every {cfg['rec_step']}th node on the deepest plane", "launches": len(groups),
+                   "shots_per_launch": per, "hbm_budget_gb": args.budget_gb},
+        "gpts_per_s": round(nominal * pts / wall / 1e9, 4),
+        "depth_steps": {"counted": f"{nominal} per shot slice (background, adjoint)", "executed": steps},
+        "misfit": misfit,
+        "grad_norm": float(torch.linalg.vector_norm(grad)),
+        "vs_plain_step": sample,
+        "column_budgets": tuned,
+        "host_model_prep_s": round(t_prep, 1),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -452,9 +582,12 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from pyfwix_amd.workloads import SHOT_BATCH_CONFIGS
+    from pyfwix_amd.workloads import FWI_CONFIGS, SHOT_BATCH_CONFIGS
     if args.config in SHOT_BATCH_CONFIGS:
         bench_shot_batch(args)
+        return
+    if args.config in FWI_CONFIGS:
+        bench_fwi(args)
         return
 
     import torch

@@ -15,7 +15,7 @@ import numpy as np
 from .refsampler import select_refs_depth
 
 __all__ = ["Workload", "born_workload", "ricker_spectrum", "CONFIGS", "SHOT_BATCH_CONFIGS", "uneven_shots",
-           "shot_models"]
+           "shot_models", "FWI_CONFIGS", "fwi_models", "grid_shots"]
 
 
 @dataclass
@@ -210,6 +210,65 @@ def shot_models(shot, nz, d):
     for k, sign in zip((nz // 4, nz // 2, 3 * nz // 4), (1.0, -1.0, 1.0)):
         ds[k] = sign * 0.05 * s[k]
     return s, ds
+
+
+# FWI-style slowness gradient: 512^2 x 400 at 10 m, 16 shots on a 4 x 4 surface
+# grid, 128 frequencies over 2-40 Hz, checkpoints every 16 depths; true model
+# = smooth background + seeded anomalies, start model = the true model
+# Gaussian-smoothed with sigma = 20 cells (closed form for these Gaussians)
+FWI_CONFIGS = {
+    "c5": dict(nx=512, nz=400, nw=128, fmin=2.0, fmax=40.0, shots_per_side=4, ckpt=16, nref=4, taper=20,
+               rec_step=4, d=10.0, smooth=20.0, anomalies=6, seed=5),
+}
+
+
+def _gauss_blobs(x, y, z, blobs, widen=0.0):
+    """Sum of 3-D Gaussians a * exp(-r^2 / (2 s^2)); with ``widen`` (metres) each is
+    convolved with a unit-mass Gaussian of that width: s^2 -> s^2 + widen^2 and
+    a -> a * (s / s')^3 (exact on an unbounded grid)."""
+    v = 0.0
+    for (cx, cy, cz, a, sg) in blobs:
+        s2 = sg * sg + widen * widen
+        amp = a * (sg * sg / s2) ** 1.5
+        v = v + amp * np.exp(-((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2) / (2 * s2))
+    return v
+
+
+def fwi_models(nx, nz, d, smooth, anomalies, seed, **_):
+    """(true, start) slowness (nz, nx, nx) float32 of the FWI workload.
+
+    Background: v = 1500 + 0.8 z + 300 exp(-r^2 / (2 * 150^2)) about the model
+    centre; true = background + ``anomalies`` seeded Gaussians (+-100 m/s,
+    100-300 m wide); start = true smoothed by a Gaussian of ``smooth`` cells
+    (the linear trend is unchanged by smoothing, the Gaussians widen in closed
+    form)."""
+    rng = np.random.default_rng(seed)
+    ny = nx
+    z = (np.arange(nz) * d)[:, None, None]
+    y = (np.arange(ny) * d)[None, :, None]
+    x = (np.arange(nx) * d)[None, None, :]
+    c = ((nx - 1) * d / 2, (ny - 1) * d / 2, (nz - 1) * d / 2)
+    bg = [(c[0], c[1], c[2], 300.0, 150.0)]
+    blobs = [(rng.uniform(0.15, 0.85) * (nx - 1) * d, rng.uniform(0.15, 0.85) * (ny - 1) * d,
+              rng.uniform(0.2, 0.8) * (nz - 1) * d, rng.choice([-1.0, 1.0]) * 100.0, rng.uniform(100.0, 300.0))
+             for _k in range(anomalies)]
+    w = smooth * d
+    v_true = 1500.0 + 0.8 * z + _gauss_blobs(x, y, z, bg + blobs)
+    v_start = 1500.0 + 0.8 * z + _gauss_blobs(x, y, z, bg + blobs, widen=w)
+    return (1.0 / v_true ** 2).astype(np.float32), (1.0 / v_start ** 2).astype(np.float32)
+
+
+def grid_shots(nx, nz, nw, fmin, fmax, shots_per_side, rec_step, d, **_):
+    """Shots on a regular surface grid over the whole nx^2 model (every shot's
+    window is the model), receivers every ``rec_step`` nodes on the deepest plane."""
+    from .batching import Shot
+    f = (fmin + (fmax - fmin) * np.arange(nw) / max(nw - 1, 1)).astype(np.float32)
+    wav = ricker_spectrum(f.astype(np.float64))[None].astype(np.complex64)
+    gx, gy = np.meshgrid(np.arange(0, nx, rec_step) * d, np.arange(0, nx, rec_step) * d)
+    rec = np.stack([gx.ravel(), gy.ravel(), np.full(gx.size, (nz - 1) * d)], 1).astype(np.float32)
+    pos = (np.arange(shots_per_side) + 0.5) * nx / shots_per_side * d
+    return [Shot(nx, nx, f, np.array([[sx, sy, 0.0]], np.float32), wav, rec)
+            for sy in pos for sx in pos]
 
 
 def workload_dims(w: Workload):

@@ -88,3 +88,20 @@ def test_shot_batch_planning_and_references():
     # the three 256-padded shots share a launch; the 768-padded ones are their own group
     assert any(sorted(idx) == [0, 1, 4] for idx, *_ in plan)
     assert launch_bytes(10, 1, 64, 64, 32, 32, 100, 1) > launch_bytes(10, 1, 64, 64, 32, 32, 100, 4)
+
+
+def test_fwi_start_model_is_the_smoothed_true_model():
+    """The FWI start model (closed-form widening of the Gaussian anomalies, linear
+    trend kept) equals scipy's Gaussian smoothing of the true velocity away from
+    the edges."""
+    from scipy.ndimage import gaussian_filter
+    from pyfwix_amd.workloads import fwi_models
+    nx, nz, sm = 64, 48, 3.0
+    s_true, s_start = fwi_models(nx=nx, nz=nz, d=10.0, smooth=sm, anomalies=4, seed=1)
+    v_true = 1.0 / np.sqrt(s_true.astype(np.float64))
+    v_start = 1.0 / np.sqrt(s_start.astype(np.float64))
+    ref = gaussian_filter(v_true, sm, mode="nearest", truncate=6.0)
+    m = int(6 * sm)
+    inner = (slice(m, nz - m), slice(m, nx - m), slice(m, nx - m))
+    assert np.max(np.abs(v_start[inner] - ref[inner])) < 0.05      # m/s, on 1500-2000 m/s
+    assert np.max(np.abs(v_start - v_true)) > 1.0                    # the anomalies did change
