@@ -87,7 +87,8 @@ class StreamingPropagator:
     ``slow_axes``: ((nx, ox, dx), (ny, oy, dy), (nw, ow, dw), (nz, oz, dz));
     ``geometry`` holds sx, sy, sz, s_ids, rx, ry, rz, r_ids; ``wavelet``
     (nsrc, nw); ``params``: nref, padx, pady (>= 0), taperx/tapery, eps,
-    eps_scat, taylor, ginsu_x/ginsu_y, ckpt_every.
+    eps_scat, taylor, ginsu_x/ginsu_y, ckpt_every, and pad_to_window (padding
+    of each batch = its window's size, as ``fwix/workers.py:50-52`` sets it).
     """
 
     def __init__(self, slow_axes, wavelet, geometry, params: dict, nbatches=(1, 1), device=None):
@@ -118,7 +119,9 @@ class StreamingPropagator:
                 b = _Batch()
                 b.src, b.rec, b.f0, b.nf = si, ri, f0, nf
                 b.x0, b.y0, b.wx, b.wy = x0, y0, wx, wy
-                NX, NY = next_fft_size(wx + padx), next_fft_size(wy + pady)
+                # pad_to_window: padding = the window's own size (workers.py:50-52)
+                px, py = (wx, wy) if params.get("pad_to_window") else (padx, pady)
+                NX, NY = next_fft_size(wx + px), next_fft_size(wy + py)
                 b.dims = ops.make_dims(NX, NY, nf, nshot, wx, wy, ox + x0 * dx, dx, oy + y0 * dy, dy,
                                        ow + f0 * dw, dw, oz, dz, nz)
                 b.op = ops.Born(b.dims, nref=self.nref, taperx=int(params.get("taperx", 0)),
