@@ -181,7 +181,16 @@ def cpu_baseline(w, budget_s=30.0):
     mu = O.pspi_adjoint(lam, sref, lab, g, w.dz, 0.0, w.taper, w.taper)               # Born adjoint step
     O.ic_adjoint(O.scatter_adjoint(mu, slow_z, g, w.dz, 0.04, 1), U, w.nx, w.ny)
     dt = time.perf_counter() - t0
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": round(3 * w.points / dt / 1e9, 4), "unit": "Gpts*freq/s", "cores": cores,
+            "cpu_model": model, "host_cpus": os.cpu_count(),
             "kind": "port",
             "sample": (f"one background PSPI step + one Born forward step + one Born adjoint step at "
                        f"depth {z} of the full {w.NX}x{w.NY}x{w.NW} slice, numpy/scipy.fft complex64, "
@@ -210,6 +219,31 @@ def plain_vs_fused(w, slow_dims, sref, labels, u, reps=5):
         del op
     return {"fused_gpts": round(out["fused"], 3), "plain_gpts": round(out["plain"], 3),
             "speedup": round(out["fused"] / out["plain"], 3)}
+
+
+def hbm_copy_gbs(dev, nbytes=2 << 30, reps=10):
+    """Achievable HBM bandwidth: the library's 16-byte stream copy of `nbytes`,
+    read + write bytes over the mean HIP-event time of `reps` copies."""
+    import torch
+    from pyfwix_amd import _lib
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    y = torch.empty_like(x)
+    st = torch.cuda.current_stream()
+    for _ in range(2):
+        _lib.call("fwix_copy", x.data_ptr(), y.data_ptr(), nbytes, st.cuda_stream)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(st)
+    for _ in range(reps):
+        _lib.call("fwix_copy", x.data_ptr(), y.data_ptr(), nbytes, st.cuda_stream)
+    b.record(st)
+    torch.cuda.synchronize()
+    gbs = 2.0 * nbytes * reps / (a.elapsed_time(b) * 1e-3) / 1e9
+    ok = bool(torch.equal(x[:1024], y[:1024]))
+    del x, y
+    torch.cuda.empty_cache()
+    if not ok:
+        raise RuntimeError("stream copy probe produced wrong data")
+    return gbs
 
 
 def autotune_sizes(sizes):
@@ -605,6 +639,7 @@ def main():
         del x, y
     cfg = CONFIGS[args.config]
     w = born_workload(args.config, **cfg)
+    copy_gbs = hbm_copy_gbs(dev)
     tuned = None if args.no_autotune else autotune_sizes([w.NY])
     slow, ds, sref, labels, wav = device_inputs(w, dev)
     dims = workload_dims(w)
@@ -678,6 +713,7 @@ def main():
         vi = valu_for(args.config, pass_kernel(dom_name, w))
         roof = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "peak_measured": round(copy_gbs, 1), "frac_measured": round(achieved / copy_gbs, 4),
                 "traffic": traffic_for(args.config, pass_kernel(dom_name, w)),
                 "valu_issue": None if vi is None else {
                     "instr_per_launch": vi, "achieved": round(vi / avg_us / 1e3, 1), "peak": VALU_PEAK_GINSTR,

@@ -318,6 +318,12 @@ int fwix_timing_report(char* buf, int len);
 int fwix_tune_columns_set(int kind, int mode, int n, int waves_per_simd);
 int fwix_tune_columns_get(int kind, int mode, int n);
 
+/* Device copy of `bytes` (a multiple of 16, 16-byte aligned buffers) with
+ * 16-byte loads and stores: the stream-copy kernel that measures the
+ * achievable HBM bandwidth beside the nominal peak.  No counterpart in the
+ * reference. */
+int fwix_copy(const void* in, void* out, size_t bytes, fwix_stream s);
+
 #ifdef __cplusplus
 }
 #endif

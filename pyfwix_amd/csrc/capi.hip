@@ -707,6 +707,14 @@ int fwix_tune_columns_set(int kind, int mode, int n, int waves_per_simd) {
   });
 }
 int fwix_tune_columns_get(int kind, int mode, int n) { return col_wpe_get(kind, mode, n); }
+int fwix_copy(const void* in, void* out, size_t bytes, fwix_stream s) {
+  return guard([&] {
+    if (bytes % 16) throw Error("fwix_copy: bytes must be a multiple of 16");
+    if ((!in || !out) && bytes) throw Error("null buffer");
+    if (((uintptr_t)in | (uintptr_t)out) % 16) throw Error("fwix_copy: buffers must be 16-byte aligned");
+    hip_check(launch_copy16(in, out, bytes / 16, S(s)), "copy");
+  });
+}
 int fwix_timing_select(const char* pass, int every) {
   return guard([&] {
     if (every < 1) throw Error("every must be >= 1");

@@ -41,6 +41,7 @@ hipError_t launch_phase_shift(int mode, const float2* m, float2* d, const PwArgs
 hipError_t launch_select(const float2* m, float2* d, const PwArgs& a, hipStream_t st);
 hipError_t launch_taper(const float2* m, float2* d, const PwArgs& a, hipStream_t st);
 hipError_t launch_axpy(const float2* m, float2* d, size_t n, float alpha, int add, hipStream_t st);
+hipError_t launch_copy16(const void* in, void* out, size_t n16, hipStream_t st);  // n16 16-byte words
 hipError_t launch_zero(float2* d, size_t n, hipStream_t st);
 // 16-bit block floating point checkpoint codec (64 complex samples per block)
 hipError_t launch_bfp16_pack(const float2* in, void* q, float* scale, size_t n, hipStream_t st);

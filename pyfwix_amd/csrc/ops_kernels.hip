@@ -62,6 +62,24 @@ __global__ __launch_bounds__(kThreads) void k_axpy(const float2* __restrict__ m,
   }
 }
 
+// plain stream copy, four 16-byte words per lane (coalesced across the
+// workgroup, all loads issued before the stores): the achievable HBM
+// bandwidth reference
+__global__ __launch_bounds__(kThreads) void k_copy16(const float4* __restrict__ in, float4* __restrict__ out, size_t n) {
+  const size_t base = (size_t)blockIdx.x * (4 * kThreads) + threadIdx.x;
+  float4 v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const size_t i = base + (size_t)k * kThreads;
+    if (i < n) v[k] = in[i];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const size_t i = base + (size_t)k * kThreads;
+    if (i < n) out[i] = v[k];
+  }
+}
+
 // zero fill (used instead of hipMemsetAsync so that captured graphs replay correctly)
 __global__ __launch_bounds__(kThreads) void k_zero(float2* d, size_t n) {
   for (size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x; idx < n; idx += (size_t)gridDim.x * kThreads)
@@ -121,6 +139,13 @@ hipError_t launch_select(const float2* m, float2* d, const PwArgs& a, hipStream_
 }
 hipError_t launch_taper(const float2* m, float2* d, const PwArgs& a, hipStream_t st) {
   hipLaunchKernelGGL(k_taper, flat_grid((size_t)a.NX * a.NY * a.NW * a.NS), dim3(kThreads), 0, st, m, d, a);
+  return hipGetLastError();
+}
+hipError_t launch_copy16(const void* in, void* out, size_t n16, hipStream_t st) {
+  if (n16 == 0) return hipSuccess;
+  const size_t g = (n16 + 4 * kThreads - 1) / (4 * kThreads);
+  if (g > 0x7fffffffu) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_copy16, dim3((unsigned)g), dim3(kThreads), 0, st, (const float4*)in, (float4*)out, n16);
   return hipGetLastError();
 }
 hipError_t launch_axpy(const float2* m, float2* d, size_t n, float alpha, int add, hipStream_t st) {
