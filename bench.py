@@ -518,6 +518,28 @@ def bench_fwi(args):
               flush=True)
         del b, data, g
         torch.cuda.empty_cache()
+    # the gradient of one shot, fused vs reference-granularity operators
+    one = None
+    if not args.no_plain:
+        from pyfwix_amd.ops import FWIX_PSPI as _FUSED, FWIX_PSPI_PLAIN as _PLAIN
+        t_one = {}
+        b = batch([0], s_start, refs_start)
+        data1 = torch.zeros(b.nsamp, dtype=torch.complex64, device=dev)
+        g1 = torch.zeros((nz, 1, b.ny, b.nx), dtype=torch.complex64, device=dev)
+        for name, method in (("fused", _FUSED), ("plain", _PLAIN)):
+            b.op.set_method(method)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            b.op.model(data1)
+            b.op.adjoint(False, g1, data1)
+            torch.cuda.synchronize()
+            t_one[name] = time.perf_counter() - t0
+        one = {"fused_s": round(t_one["fused"], 3), "plain_s": round(t_one["plain"], 3),
+               "speedup": round(t_one["plain"] / t_one["fused"], 3),
+               "sample": "background modeling + Born adjoint of shot 0 alone (one-shot launch), fused steps vs "
+                         "imaging condition / scattering / PSPI as separate operators"}
+        del b, data1, g1
+        torch.cuda.empty_cache()
     pts = sum(s.nw for s in shots) * N * N
     nominal = 2 * (nz - 1)
     # PSPI forward + adjoint step, fused vs the plain per-operator kernels, on one shot's planes
@@ -570,6 +592,7 @@ def bench_fwi(args):
         "misfit": misfit,
         "grad_norm": float(torch.linalg.vector_norm(grad)),
         "vs_plain_step": sample,
+        "vs_plain_shot": one,
         "column_budgets": tuned,
         "host_model_prep_s": round(t_prep, 1),
         "cpu_baseline": None,
@@ -755,6 +778,22 @@ def main():
                        "GBs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)} for k, v in
                    sorted(passes.items(), key=lambda kv: -kv[1]["ms"] / kv[1]["launches"] * kv[1]["total_launches"])},
     }
+    if not args.no_plain:
+        # the same Born step through separate operators, one shot per launch
+        from pyfwix_amd.ops import FWIX_PSPI, FWIX_PSPI_PLAIN
+        born.set_method(FWIX_PSPI_PLAIN)
+        step()
+        torch.cuda.synchronize()
+        s1 = born.steps
+        t0 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        pw = time.perf_counter() - t0
+        res["vs_plain_born"] = {"plain_gpts": round((born.steps - s1) * w.points / pw / 1e9, 3),
+                                "plain_ms_per_step": round(pw * 1e3, 1), "speedup": round(pw / (wall / args.steps), 3),
+                                "form": "imaging condition, scattering operator and PSPI step as separate operators "
+                                        "(taper, FFT, per-reference phase shift / IFFT / select), one shot per launch"}
+        born.set_method(FWIX_PSPI)
     del born, grad, data
     if not args.no_plain:
         u = torch.randn((w.NS, w.NW, w.NY, w.NX), dtype=torch.complex64, device=dev)

@@ -232,6 +232,12 @@ int fwix_born_migrate(fwix_op op, int add, fwix_c64* image, const fwix_c64* data
  * Slowness, references, sources, receivers and wavelet must be set again; the
  * migration image and the reflection / extended channels are refused. */
 int fwix_born_set_shot_planes(fwix_op op, int nshot, const int* nplanes);
+/* FWIX_PSPI (default, fused depth steps) or FWIX_PSPI_PLAIN: the imaging
+ * condition, the scattering operator (Scatter.cpp:90-116) and the PSPI step
+ * (PSPI.cpp:66-103) as separate operators, one shot per launch, chained as
+ * Scattering.cpp:43-61 and ExtendedBorn.cpp:152-208 do -- the comparator of the
+ * fused path.  Same results; the background must be modeled again. */
+int fwix_born_set_method(fwix_op op, int method);
 /* samples of all receiver (receivers != 0) or source traces: nrec * NW
  * without a shot batch; -1 for an invalid handle */
 long long fwix_born_trace_samples(fwix_op op, int receivers);

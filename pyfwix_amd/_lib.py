@@ -95,6 +95,7 @@ _SIGS = {
     "fwix_born_set_refs": [_P, _P, _P],
     "fwix_born_set_frequencies": [_P, _P],
     "fwix_born_set_shot_planes": [_P, _I, _P],
+    "fwix_born_set_method": [_P, _I],
     "fwix_born_set_slow": [_P, _P],
     "fwix_born_set_sources": [_P, _I, _P, _P, _P, _P],
     "fwix_born_set_receivers": [_P, _I, _P, _P, _P, _P],

@@ -272,6 +272,85 @@ struct BornOp : fwix_op_s {
   DevBuf gscr, sfirst_dev, scount_dev;  // per-plane gradient of one depth; plane ranges
   BornOp() { kind = KIND_BORN; }
   bool batched() const { return !sfirst.empty(); }
+  // Reference-granularity form (FWIX_PSPI_PLAIN): the imaging condition, the
+  // scattering operator and the PSPI step as separate operators, one shot per
+  // launch, as Scattering.cpp:43-61 chains them -- the comparator of the fused
+  // steps.  pa / pb: whole-wavefield scratch; pw: the scattering terms of one shot.
+  DevBuf pa, pb, pw;
+  void set_method(int method) {
+    if (method != FWIX_PSPI && method != FWIX_PSPI_PLAIN) throw Error("method must be FWIX_PSPI or FWIX_PSPI_PLAIN");
+    if (batched()) throw Error("shot batches run on the fused path only");
+    const float2* sr = eng.sref;
+    const uint8_t* lb = eng.labels;
+    eng.init(method, d, par, par.taylor + 1);
+    if (sr && lb) eng.set_refs(sr, lb);
+    const size_t n = P() * sizeof(float2);
+    if (eng.plain) {
+      pa.alloc(n);
+      pb.alloc(n);
+      pw.alloc((size_t)std::max(par.taylor, 1) * d.NW * d.NX * d.NY * sizeof(float2));
+    } else {
+      pa.release();
+      pb.release();
+      pw.release();
+    }
+    have_bg = have_up = false;
+  }
+  // one shot's scattering operator (Scatter.cpp:90-116), forward or adjoint
+  void scatter_shot(bool adjoint, int z, int is, const float2* in, float2* out, hipStream_t st) {
+    const size_t plane = (size_t)d.NX * d.NY;
+    const float sx = (float)(1.0 / std::sqrt((double)d.NX)), sy2 = (float)(1.0 / (double)d.NY);
+    const double sb = 8.0 * d.NW * plane;
+    KArgs a = eng.tables.kargs();
+    a.K = (float2*)eng.K.p;
+    a.W = (float2*)pw.p;
+    a.p0 = is * d.NW; a.np = d.NW;
+    a.wstride = (long long)d.NW * plane;
+    a.eps_s = par.eps_scat;
+    a.slow = slow + (size_t)z * MZ();
+    const int T = par.taylor;
+    if (!adjoint) {
+      if (T >= 1) {
+        a.in = in; a.out = a.K; a.add = 0; a.scale = sx;
+        { PassTimer t("plain_scatter_rows", 2 * sb, st); hip_check(launch_rows(0, a, st), "plain scatter rows"); }
+        a.nterm = T; a.scale = sy2;
+        { PassTimer t("plain_scatter_cols", (1 + T) * sb, st); hip_check(launch_cols_fanout(2, a, st), "plain scatter columns"); }
+      }
+      a.in = in; a.out = out; a.add = 0; a.nterm = T + 1; a.scale = sx;
+      { PassTimer t("plain_scatter_combine", (T + 2) * sb, st); hip_check(launch_rows_scatter(false, a, st), "plain scatter combine"); }
+    } else {
+      a.in = in; a.nterm = T + 1; a.scale = sx;
+      if (T >= 1) {
+        { PassTimer t("plain_scatter_adj_rows", (1 + T) * sb, st); hip_check(launch_rows_scatter_adj(false, a, st), "plain scatter adjoint rows"); }
+        a.nterm = T; a.scale = sy2;
+        { PassTimer t("plain_scatter_adj_cols", (1 + T) * sb, st); hip_check(launch_cols_fanin(2, a, st), "plain scatter adjoint columns"); }
+      }
+      a.in = in; a.out = out; a.add = 0; a.nterm = T + 1; a.scale = sx;
+      { PassTimer t("plain_scatter_adj_combine", 3 * sb, st); hip_check(launch_rows_adj_out(false, a, st), "plain scatter adjoint combine"); }
+    }
+  }
+  // out = P_z[ addin + S_z(bg * ds_z) ] through separate operators
+  void step_fwd_plain(int z, const float2* bg, const float2* dsz, const float2* addin, float2* out, hipStream_t st) {
+    const size_t n = P(), shot = (size_t)d.NW * d.NX * d.NY;
+    float2* a_ = (float2*)pa.p;
+    float2* b_ = (float2*)pb.p;
+    { PassTimer t("plain_ic", 24.0 * n, st);
+      hip_check(launch_ic_fwd(dsz, a_, bg, d.NX, d.NY, d.NW, d.NS, d.nx, d.ny, 0, st), "plain imaging condition"); }
+    for (int is = 0; is < d.NS; ++is) scatter_shot(false, z, is, a_ + is * shot, b_ + is * shot, st);
+    { PassTimer t("plain_add", 24.0 * n, st); hip_check(launch_axpy(addin, b_, n, 1.f, 1, st), "plain add"); }
+    eng.apply(false, z, 0, b_, out, st);
+    ++steps;
+  }
+  // lam <- P_z^H lam; gz += conj(bg) S_z^H lam through separate operators
+  void step_adj_plain(int z, float2* lam, const float2* bg, float2* gz, hipStream_t st) {
+    const size_t n = P(), shot = (size_t)d.NW * d.NX * d.NY;
+    float2* a_ = (float2*)pa.p;
+    eng.apply(true, z, 0, lam, lam, st);
+    for (int is = 0; is < d.NS; ++is) scatter_shot(true, z, is, lam + is * shot, a_ + is * shot, st);
+    { PassTimer t("plain_ic_adj", 16.0 * n + 16.0 * MZ(), st);
+      hip_check(launch_ic_adj(gz, a_, bg, d.NX, d.NY, d.NW, d.NS, d.nx, d.ny, 1, st), "plain imaging adjoint"); }
+    ++steps;
+  }
   void no_batch(const char* what) const {
     if (batched()) throw Error(std::string(what) + " is not available for a shot batch");
   }
@@ -406,6 +485,7 @@ struct BornOp : fwix_op_s {
   // fused): addin and out may be the same wavefield.  The scattering operator
   // and the PSPI step use the slowness and references of depth z.
   void step_fwd(int z, const float2* bg, const float2* dsz, const float2* addin, float2* out, hipStream_t st) {
+    if (eng.plain) return step_fwd_plain(z, bg, dsz, addin, out, st);
     const size_t plane = (size_t)d.NX * d.NY;
     const float sx = (float)(1.0 / std::sqrt((double)d.NX)), sy2 = (float)(1.0 / (double)d.NY);
     const int cw = std::min(eng.chunk, d.NW);
@@ -442,6 +522,7 @@ struct BornOp : fwix_op_s {
   // lam <- P_z^H lam in place, and gz += conj(bg) S_z^H (P_z^H lam) on the model
   // region (the adjoint of step_fwd's scattering term, Scattering.cpp:54-61)
   void step_adj(int z, float2* lam, const float2* bg, float2* gz, hipStream_t st) {
+    if (eng.plain) return step_adj_plain(z, lam, bg, gz, st);
     const size_t plane = (size_t)d.NX * d.NY;
     const float sx = (float)(1.0 / std::sqrt((double)d.NX)), sy2 = (float)(1.0 / (double)d.NY);
     const int cw = std::min(eng.chunk, d.NW);
@@ -929,6 +1010,9 @@ int fwix_born_set_density(fwix_op op, const fwix_c64* den_dev) {
     p->den = (const float2*)den_dev;
     p->have_up = false;
   });
+}
+int fwix_born_set_method(fwix_op op, int method) {
+  return guard([&] { as<BornOp>(op, KIND_BORN)->set_method(method); });
 }
 int fwix_born_set_shot_planes(fwix_op op, int nshot, const int* nplanes) {
   return guard([&] { as<BornOp>(op, KIND_BORN)->set_shot_planes(nshot, nplanes); });

@@ -239,10 +239,12 @@ void StepEngine::init(int method_, const fwix_dims& d, const fwix_params& p, int
   const size_t plane = (size_t)d.NX * d.NY;
   if (plain) {  // two whole-shot scratch wavefields
     chunk = d.NW;
+    W.release();
     K.alloc(plane * d.NW * sizeof(float2));
     T.alloc(plane * d.NW * sizeof(float2));
     return;
   }
+  T.release();
   if (wterms < p.nref) wterms = p.nref;
   chunk = auto_chunk(d, wterms, p.planes_per_chunk);
   K.alloc(plane * chunk * sizeof(float2));

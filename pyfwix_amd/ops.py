@@ -353,6 +353,10 @@ class Born:
             return (n, self.dims.NW)
         return (int(_lib.lib.fwix_born_trace_samples(self._h.h, int(receivers))),)
 
+    def set_method(self, method: int) -> None:
+        """FWIX_PSPI (fused steps) or FWIX_PSPI_PLAIN (separate operators, one shot per launch)."""
+        call("fwix_born_set_method", self._h.h, int(method))
+
     def set_shot_planes(self, counts) -> None:
         """Lay a batch of shots along the plane axis (shot k: the next counts[k] planes)."""
         c = _i32(counts, None, "plane counts")
