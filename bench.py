@@ -518,27 +518,50 @@ def bench_fwi(args):
               flush=True)
         del b, data, g
         torch.cuda.empty_cache()
-    # the gradient of one shot, fused vs reference-granularity operators
+    # the gradient of one shot, fused vs reference-granularity operators, on the
+    # regular operator layout (frequency-replicated slowness and labels: 107 GB
+    # and 54 GB at the full depth, so the first `zw` depths are sampled)
     one = None
     if not args.no_plain:
-        from pyfwix_amd.ops import FWIX_PSPI as _FUSED, FWIX_PSPI_PLAIN as _PLAIN
+        from pyfwix_amd.ops import FWIX_PSPI as _FUSED, FWIX_PSPI_PLAIN as _PLAIN, Born
+        sh, nx = shots[0], cfg["nx"]
+        zw = min(nz, 3 * ck)
+        dims = make_dims(N, N, nw, 1, nx, nx, 0.0, d, 0.0, d, float(sh.freqs[0]), 1.0, 0.0, d, zw)
+        born = Born(dims, nref=nref, taperx=taper, tapery=taper, eps_scat=0.04, taylor=1, ckpt_every=ck)
+        s3 = torch.from_numpy(s_start[:zw]).to(dev)
+        slow = torch.zeros((zw, nw, nx, nx), dtype=torch.complex64, device=dev)
+        slow.real.copy_(s3[:, None].expand(zw, nw, nx, nx))
+        del s3
+        sref = torch.from_numpy(refs_start[0][:zw]).to(dev)[:, :, None].expand(zw, nref, nw).contiguous()
+        labels = torch.from_numpy(refs_start[1][:zw]).to(dev)[:, None].expand(zw, nw, N, N).contiguous()
+        sh_rec = sh.rec.copy()
+        sh_rec[:, 2] = (zw - 1) * d                    # receivers on the window's deepest plane
+        wav1 = torch.from_numpy(np.ascontiguousarray(sh.wavelet)).to(dev)
+        data1 = torch.zeros((len(sh.rec), nw), dtype=torch.complex64, device=dev)
+        g1 = torch.zeros_like(slow)
         t_one = {}
-        b = batch([0], s_start, refs_start)
-        data1 = torch.zeros(b.nsamp, dtype=torch.complex64, device=dev)
-        g1 = torch.zeros((nz, 1, b.ny, b.nx), dtype=torch.complex64, device=dev)
         for name, method in (("fused", _FUSED), ("plain", _PLAIN)):
-            b.op.set_method(method)
+            born.set_method(method)
+            born.set_frequencies(sh.freqs)
+            born.set_refs(sref, labels)
+            born.set_slow(slow)
+            born.set_sources(sh.src[:, 0], sh.src[:, 1], sh.src[:, 2], np.zeros(1, np.int32))
+            born.set_receivers(sh_rec[:, 0], sh_rec[:, 1], sh_rec[:, 2], np.zeros(len(sh_rec), np.int32))
+            born.set_wavelet(wav1)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            b.op.model(data1)
-            b.op.adjoint(False, g1, data1)
+            born.model(data1)
+            born.adjoint(False, g1, data1)
             torch.cuda.synchronize()
             t_one[name] = time.perf_counter() - t0
+            print(f"[fwi] one shot, {name}: {t_one[name]:.3f} s", file=sys.stderr, flush=True)
         one = {"fused_s": round(t_one["fused"], 3), "plain_s": round(t_one["plain"], 3),
-               "speedup": round(t_one["plain"] / t_one["fused"], 3),
-               "sample": "background modeling + Born adjoint of shot 0 alone (one-shot launch), fused steps vs "
-                         "imaging condition / scattering / PSPI as separate operators"}
-        del b, data1, g1
+               "speedup": round(t_one["plain"] / t_one["fused"], 3), "depths": zw,
+               "plain_s_per_shot_est": round(wall / len(shots) * t_one["plain"] / t_one["fused"], 3),
+               "sample": f"background modeling + Born adjoint of shot 0 alone over the first {zw} depths on the "
+                         f"regular operator layout, fused steps vs imaging condition / scattering / PSPI as separate "
+                         f"operators; plain_s_per_shot_est scales the measured s/shot by that ratio"}
+        del born, slow, sref, labels, wav1, data1, g1
         torch.cuda.empty_cache()
     pts = sum(s.nw for s in shots) * N * N
     nominal = 2 * (nz - 1)
