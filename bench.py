@@ -10,7 +10,19 @@ step (NX*NY*NW*NS points each) per second.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2] [--no-cpu]
 
-Prints one JSON line.
+Prints one JSON line.  Before the timed region the column kernels' register
+budget is tuned for the padded size (`--no-autotune` keeps the defaults) and
+a stream copy measures the achievable HBM bandwidth (`roofline.peak_measured`).
+`--graph` also times the step replayed as one HIP graph.
+
+Other workloads (same JSON shape, their own metric where it differs):
+
+    --config c1        the c1 grid through the same Born step
+    --config c4        64 shots of uneven aperture / frequency count as shot
+                       batches (`pyfwix_amd/batching.py`), vs one shot per
+                       operator, plus a PSPI step sample vs the plain path
+    --config c5        FWI gradient, 16 shots, checkpoints every 16 depths:
+                       seconds per shot (use --budget-gb 200)
 
 Profiling helpers (the summaries under profiles/ are made this way):
 
