@@ -261,12 +261,14 @@ struct TermMult {
   float w2, wsre, b, iw2, dz;
   int r;
   bool real;
-  __device__ __forceinline__ TermMult(const KArgs& a, int r_, int iw, float w2_) : w2(w2_), r(r_), dz(a.dz) {
+  __device__ __forceinline__ TermMult(const KArgs& a, int r_, int iw, float w2_)
+      : TermMult(a, r_, w2_, MULT == 2 ? make_float2(0.f, 0.f) : a.sref[r_ * a.NW + iw]) {}
+  // sr: the reference slowness of term r (already read, e.g. from LDS)
+  __device__ __forceinline__ TermMult(const KArgs& a, int r_, float w2_, float2 sr) : w2(w2_), r(r_), dz(a.dz) {
     if constexpr (MULT == 2) {
       iw2 = 1.f / w2;
       real = true; wsre = 0.f; b = 0.f;
     } else {
-      const float2 sr = a.sref[r * a.NW + iw];
       wsre = w2 * sr.x;
       b = w2 * (sr.y - a.eps * sr.x);
       real = b == 0.f;
@@ -450,6 +452,20 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   tile_store<C>(smem, PlaneBuf(a.out + pb, plane_bytes(a)), to, a.add);
 }
 
+// Whether the twiddle table (N values) and the reference slownesses (up to
+// 256) fit in LDS beside the tile and the row masks without lowering the
+// workgroups a CU holds (at most WPE / 2 by registers; 160 KB of LDS).
+template <int N, int WPE>
+constexpr bool col_stage_lds() {
+  using C = ColCfg<N>;
+  constexpr int base = C::SMEM * 8 + C::NROW * 4;
+  constexpr int extra = N * 8 + 256 * 8;
+  constexpr int by_regs = WPE / 2;
+  constexpr int without = (160 * 1024) / base < by_regs ? (160 * 1024) / base : by_regs;
+  constexpr int with = (160 * 1024) / (base + extra) < by_regs ? (160 * 1024) / (base + extra) : by_regs;
+  return with >= without;
+}
+
 // cols_fanout: W_r = F_y^-1[M_r * F_y(K)] / NY for every term r (M_r: term_mult<MODE>).
 template <int N, int MODE, int WPE>
 __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_cols_fanout(KArgs a) {
@@ -458,6 +474,14 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   using Fi = WaveFft<N, C::NB, false, true, PadLds<N>>;
   static_assert(F::P::last() == Fi::P::first(), "register hand-off needs mirrored plans");
   __shared__ float2 smem[C::SMEM];
+  // The loop below stores each term's tile and then starts the next term; on
+  // this ISA one wait counter covers loads and stores, so any global load
+  // inside the loop (twiddles, reference slownesses) would also wait for the
+  // previous term's stores to reach memory.  Both are staged in LDS first
+  // where that costs no occupancy (STAGE).
+  constexpr bool STAGE = N <= 512 && col_stage_lds<N, WPE>();
+  __shared__ float2 twl[STAGE ? N : 1];
+  __shared__ float2 srl[STAGE ? 256 : 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int p, c0;
   col_tile<C>(p, c0);
@@ -465,12 +489,21 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   float2* priv = smem + wave * C::PRIV;
   const TileOffs<C> to(a.NX, c0);
   tile_load<C>(smem, PlaneBuf(a.K + pb, plane_bytes(a)), to);
+  const int iw = plane_iw(a, p);
+  const float2* tw = a.twy;
+  if constexpr (STAGE) {
+    for (int i = threadIdx.x; i < N; i += C::THREADS) twl[i] = a.twy[i];
+    if constexpr (MODE != 2) {
+      const int nt = a.nterm < 256 ? a.nterm : 256;
+      for (int i = threadIdx.x; i < nt; i += C::THREADS) srl[i] = a.sref[i * a.NW + iw];
+    }
+    tw = twl;
+  }
   __syncthreads();
   float2 spec[C::V];
   tile_read<N, C, F::P::first()>(spec, smem, wave, lane);
   __syncthreads();
-  F::run(spec, priv, a.twy, lane);
-  const int iw = plane_iw(a, p);
+  F::run(spec, priv, tw, lane);
   const float w2 = a.w2[iw];
   const float s2 = a.scale;  // 1/NY: forward and inverse column normalisation together
   const int colw = c0 + wave * C::NB;
@@ -483,7 +516,9 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   column_k2<N, C, F::P::last()>(k2v, a, colw, lane);
   for (int r = 0; r < a.nterm; ++r) {
     if (r < 32 && !((pm >> r) & 1u)) continue;  // reference absent from the plane: W_r is never read
-    const TermMult<MODE> tm(a, r, iw, w2);
+    float2 srv = make_float2(0.f, 0.f);  // nref <= 255
+    if constexpr (MODE != 2) srv = STAGE ? srl[r] : a.sref[r * a.NW + iw];
+    const TermMult<MODE> tm(a, r, w2, srv);
     float2 x[C::V];
     auto mult = [&](auto real) {
 #pragma unroll
@@ -491,7 +526,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
     };
     if (tm.real) mult(std::true_type{});
     else mult(std::false_type{});
-    Fi::run(x, priv, a.twy, lane);
+    Fi::run(x, priv, tw, lane);
     __syncthreads();
     tile_write<N, C, Fi::P::last()>(smem, x, wave, lane);
     __syncthreads();
@@ -529,26 +564,42 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   const bool masked = rm != nullptr;
   const int rend = masked ? 32 : a.nterm;
   __shared__ uint32_t rmk[C::NROW];
+  // twiddles and reference slownesses in LDS where that costs no occupancy
+  // (STAGE): a global load in the loop would wait (one counter for all vector
+  // memory operations) for the next term's tile, which is fetched into
+  // registers while this term is transformed
+  constexpr bool STAGE = N <= 512 && col_stage_lds<N, WPE>();
+  __shared__ float2 twl[STAGE ? N : 1];
+  __shared__ float2 srl[STAGE ? 256 : 1];
   tile_row_masks<C>(rmk, rm);
+  const float2* tw = a.twy;
+  if constexpr (STAGE) {
+    for (int i = threadIdx.x; i < N; i += C::THREADS) twl[i] = a.twy[i];
+    if constexpr (MODE != 2) {
+      const int nt = a.nterm < 256 ? a.nterm : 256;
+      for (int i = threadIdx.x; i < nt; i += C::THREADS) srl[i] = a.sref[i * a.NW + iw];
+    }
+    tw = twl;
+  }
   __syncthreads();
   float k2v[C::V];
   column_k2<N, C, F::P::last()>(k2v, a, colw, lane);
-  int r = masked ? next_bit(live, -1) : 0;
-  while (r < rend) {
-    {
-      TileRegs<C> cur;
-      const PlaneBuf wb(a.W + (size_t)r * a.wstride + pb, plane_bytes(a));
-      if (masked) cur.fetch_masked(wb, to, rmk, 1u << r);
-      else cur.fetch(wb, to);
-      cur.commit(smem);
-    }
-    __syncthreads();
-    const int rn = masked ? next_bit(live, r) : r + 1;
+  auto fetch = [&](TileRegs<C>& t, int rr) {
+    const PlaneBuf wb(a.W + (size_t)rr * a.wstride + pb, plane_bytes(a));
+    if (masked) t.fetch_masked(wb, to, rmk, 1u << rr);
+    else t.fetch(wb, to);
+  };
+  // one term: transform its tile (already in LDS), multiply, accumulate;
+  // after_read runs once the tile is in registers
+  auto term = [&](int rc, auto&& after_read) {
     float2 v[C::V];
     tile_read<N, C, F::P::first()>(v, smem, wave, lane);
+    after_read();
     __syncthreads();
-    F::run(v, priv, a.twy, lane);
-    const TermMult<MODE> tm(a, r, iw, w2);
+    F::run(v, priv, tw, lane);
+    float2 srv = make_float2(0.f, 0.f);  // nref <= 255
+    if constexpr (MODE != 2) srv = STAGE ? srl[rc] : a.sref[rc * a.NW + iw];
+    const TermMult<MODE> tm(a, rc, w2, srv);
     auto mult = [&](auto real) {
 #pragma unroll
       for (int e = 0; e < C::V; ++e) acc[e] = cadd(acc[e], cmul(v[e], tm.template at<decltype(real)::value>(k2v[e])));
@@ -556,12 +607,37 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
     if (tm.real) mult(std::true_type{});
     else mult(std::false_type{});
     __syncthreads();
-    r = rn;
+  };
+  int r = masked ? next_bit(live, -1) : 0;
+  // the next term's tile is fetched into registers during this term's
+  // transform where the register budget holds it without spilling
+  if constexpr (N <= 512 || WPE <= 2) {
+    TileRegs<C> tile;
+    if (r < rend) fetch(tile, r);
+    while (r < rend) {
+      tile.commit(smem);
+      __syncthreads();
+      const int rn = masked ? next_bit(live, r) : r + 1;
+      term(r, [&] { if (rn < rend) fetch(tile, rn); });
+      r = rn;
+    }
+  } else {
+    while (r < rend) {
+      {
+        TileRegs<C> cur;
+        fetch(cur, r);
+        cur.commit(smem);
+      }
+      __syncthreads();
+      const int rn = masked ? next_bit(live, r) : r + 1;
+      term(r, [] {});
+      r = rn;
+    }
   }
   const float s2 = a.scale;
 #pragma unroll
   for (int e = 0; e < C::V; ++e) acc[e] = cscale(acc[e], s2);
-  Fi::run(acc, priv, a.twy, lane);
+  Fi::run(acc, priv, tw, lane);
   __syncthreads();
   tile_write<N, C, Fi::P::last()>(smem, acc, wave, lane);
   __syncthreads();
