@@ -17,6 +17,11 @@ if not _LIB_PATH.exists():
         f"{_LIB_PATH} is missing: build it with `python -m pyfwix_amd.build` "
         "(there is no CPU fallback for the depth-stepping operators)")
 
+# torch ships its own HIP runtime; loading it first makes this library bind to
+# that same runtime (one libamdhip64 per process).  Loading this library first
+# would bring in a second copy, and torch then finds no GPU.
+import torch  # noqa: E402,F401
+
 lib = C.CDLL(str(_LIB_PATH))
 
 FWIX_PSPI = 0
