@@ -70,6 +70,12 @@ __global__ __launch_bounds__(kThreads) void k_rows_ic_fwd(KArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nrows = a.np * a.NY;
   const int rb = (blockIdx.x * kWaves + wave) * C::NB;
+  // twiddles in LDS: inside the term loops a global load would wait (one
+  // counter for all vector memory operations) for the term stores / the next
+  // term's loads still in flight
+  __shared__ float2 twl[N];
+  for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = a.twx[i];
+  __syncthreads();
   if (rb >= nrows) return;
   float2 v[C::V];
 #pragma unroll
@@ -87,7 +93,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_ic_fwd(KArgs a) {
       }
     }
   }
-  F::run(v, lds + wave * C::LDSW, a.twx, lane);
+  F::run(v, lds + wave * C::LDSW, twl, lane);
 #pragma unroll
   for (int e = 0; e < C::V; ++e) {
     int b, i;
@@ -112,6 +118,12 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nrows = a.np * a.NY;
   const int rb = (blockIdx.x * kWaves + wave) * C::NB;
+  // twiddles in LDS: inside the term loops a global load would wait (one
+  // counter for all vector memory operations) for the term stores / the next
+  // term's loads still in flight
+  __shared__ float2 twl[N];
+  for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = a.twx[i];
+  __syncthreads();
   if (rb >= nrows) return;
   float2* wl = lds + wave * C::LDSW;
   float2 acc[C::V], pw[C::V], inv[C::V];
@@ -148,7 +160,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
       const int lr = rb + b;
       x[e] = lr < nrows ? W[(unsigned)(lr * N + i)] : czero();
     }
-    Fi::run(x, wl, a.twx, lane);
+    Fi::run(x, wl, twl, lane);
     const float c = taylor_coef(it) * a.scale;  // the inverse row transform's 1/sqrt(NX)
 #pragma unroll
     for (int e = 0; e < C::V; ++e) {
@@ -193,7 +205,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
         acc[e] = cscale(y, a.tapx[i] * a.tapy[lr % a.NY]);
       }
     }
-    F::run(acc, wl, a.twx, lane);
+    F::run(acc, wl, twl, lane);
 #pragma unroll
     for (int e = 0; e < C::V; ++e) {
       int b, i;
@@ -219,6 +231,12 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nrows = a.np * a.NY;
   const int rb = (blockIdx.x * kWaves + wave) * C::NB;
+  // twiddles in LDS: inside the term loops a global load would wait (one
+  // counter for all vector memory operations) for the term stores / the next
+  // term's loads still in flight
+  __shared__ float2 twl[N];
+  for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = a.twx[i];
+  __syncthreads();
   if (rb >= nrows) return;
   float2* wl = lds + wave * C::LDSW;
   float2 u[C::V];
@@ -230,7 +248,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
       const int lr = rb + b;
       u[e] = lr < nrows ? a.K[(unsigned)(lr * N + i)] : czero();
     }
-    Fi::run(u, wl, a.twx, lane);
+    Fi::run(u, wl, twl, lane);
 #pragma unroll
     for (int e = 0; e < C::V; ++e) {
       int b, i;
@@ -275,7 +293,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
       x[e] = cscale(cmulc(u[e], pw[e]), c);                // conj(c_it s~^(-1/2-it)) * u
       pw[e] = cmul(pw[e], inv[e]);
     }
-    F::run(x, wl, a.twx, lane);
+    F::run(x, wl, twl, lane);
     float2* W = a.W + (size_t)(it - 1) * a.wstride;
 #pragma unroll
     for (int e = 0; e < C::V; ++e) {
@@ -302,6 +320,12 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj_out(KArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nrows = a.np * a.NY;
   const int rb = (blockIdx.x * kWaves + wave) * C::NB;
+  // twiddles in LDS: inside the term loops a global load would wait (one
+  // counter for all vector memory operations) for the term stores / the next
+  // term's loads still in flight
+  __shared__ float2 twl[N];
+  for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = a.twx[i];
+  __syncthreads();
   if (rb >= nrows) return;
   float2 x[C::V];
   if (a.nterm > 1) {
@@ -312,7 +336,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj_out(KArgs a) {
       const int lr = rb + b;
       x[e] = lr < nrows ? a.K[(unsigned)(lr * N + i)] : czero();
     }
-    Fi::run(x, lds + wave * C::LDSW, a.twx, lane);
+    Fi::run(x, lds + wave * C::LDSW, twl, lane);
   } else {
 #pragma unroll
     for (int e = 0; e < C::V; ++e) x[e] = czero();

@@ -119,6 +119,12 @@ __global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nrows = a.np * a.NY;
   const int rb = (blockIdx.x * kWaves + wave) * C::NB;
+  // twiddles in LDS: inside the term loops a global load would wait (one
+  // counter for all vector memory operations) for the term stores / the next
+  // term's loads still in flight
+  __shared__ float2 twl[N];
+  for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = a.twx[i];
+  __syncthreads();
   if (rb >= nrows) return;
   float2* wl = lds + wave * C::LDSW;
   float2 u[C::V];
@@ -154,7 +160,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
     float2 x[C::V];
 #pragma unroll
     for (int e = 0; e < C::V; ++e) x[e] = (lab[e] == r) ? u[e] : czero();
-    F::run(x, wl, a.twx, lane);
+    F::run(x, wl, twl, lane);
     float2* W = a.W + (size_t)r * a.wstride;
 #pragma unroll
     for (int e = 0; e < C::V; ++e) {
@@ -175,6 +181,12 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nrows = a.np * a.NY;
   const int rb = (blockIdx.x * kWaves + wave) * C::NB;
+  // twiddles in LDS: inside the term loops a global load would wait (one
+  // counter for all vector memory operations) for the term stores / the next
+  // term's loads still in flight
+  __shared__ float2 twl[N];
+  for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = a.twx[i];
+  __syncthreads();
   if (rb >= nrows) return;
   float2* wl = lds + wave * C::LDSW;
   uint8_t lab[C::V];
@@ -226,7 +238,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
     for (int e = 0; e < C::V; ++e) x[e] = nxt[e];
     const int rn = a.rowmask ? next_bit(live, r) : r + 1;
     if (rn < rend) fetch(rn);
-    F::run(x, wl, a.twx, lane);
+    F::run(x, wl, twl, lane);
 #pragma unroll
     for (int e = 0; e < C::V; ++e)
       if (lab[e] == r) acc[e] = cadd(acc[e], x[e]);
