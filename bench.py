@@ -258,14 +258,15 @@ def hbm_copy_gbs(dev, nbytes=2 << 30, reps=10):
     return gbs
 
 
-def autotune_sizes(sizes):
+def autotune_sizes(sizes, refs=None, freqs=None):
     """Tune the column kernels' register budget at every padded size (outside
-    the timed region); -> {size: {"chosen": ..., "us": {...}}, "seconds": t}."""
+    the timed region), on a real depth's references when given; -> {size:
+    {"chosen": ..., "us": {...}}, "seconds": t}."""
     from pyfwix_amd import autotune
     t0 = time.perf_counter()
     out = {}
     for n in sizes:
-        r = autotune.tune_size(int(n))
+        r = autotune.tune_size(int(n), refs=refs, freqs=freqs)
         out[str(n)] = {"chosen": r["chosen"], "us": {k: v for k, v in r.items() if k != "chosen"}}
         print(f"[tune] {n}: {r['chosen']}", file=sys.stderr, flush=True)
     out["seconds"] = round(time.perf_counter() - t0, 2)
@@ -698,8 +699,11 @@ def main():
     cfg = CONFIGS[args.config]
     w = born_workload(args.config, **cfg)
     copy_gbs = hbm_copy_gbs(dev)
-    tuned = None if args.no_autotune else autotune_sizes([w.NY])
     slow, ds, sref, labels, wav = device_inputs(w, dev)
+    zt = w.nz // 2          # tune on the references of the middle depth
+    tuned = None if args.no_autotune else autotune_sizes(
+        [w.NY], refs=(sref[zt:zt + 1].contiguous(), labels[zt:zt + 1].contiguous()),
+        freqs=w.ow + w.dw * np.arange(w.NW))
     dims = workload_dims(w)
     born = Born(dims, nref=w.nref, taperx=w.taper, tapery=w.taper, taylor=1, eps_scat=0.04, ckpt_every=1,
                 planes_per_chunk=args.chunk)

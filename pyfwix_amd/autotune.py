@@ -79,26 +79,42 @@ def _time_passes(run, names, reps):
 
 
 def tune_size(n: int, planes: int | None = None, nref: int = 4, taylor: int = 1, reps: int = 5,
-              candidates=CANDIDATES, device=None, seed: int = 0, margin: float = 0.03) -> dict:
+              candidates=CANDIDATES, device=None, seed: int = 0, margin: float = 0.03, refs=None,
+              freqs=None) -> dict:
     """Tune the budgets of every column kernel at column length ``n`` (an n x n
     padded grid with ``planes`` planes, default about 16 M points) and set the
     fastest, keeping the budget in use unless another beats it by ``margin``
-    (launch-time noise is a few per cent).  Returns {"fanout": {wpe: us}, "fanin": {...}, "scatter_fanout":
-    {...}, "scatter_fanin": {...}, "chosen": {...}}."""
+    (launch-time noise is a few per cent).  ``refs`` = (sref (1, nref, planes),
+    labels (1, planes, n, n)) device tensors and ``freqs`` (planes values, Hz)
+    tune on a real depth of a workload (which references are present in which
+    rows changes what the fan-out / fan-in transforms do); by default the
+    labels are random.  Returns {"fanout": {wpe: us}, "fanin": {...},
+    "scatter_fanout": {...}, "scatter_fanin": {...}, "chosen": {...}}."""
     import torch
     from .ops import FWIX_PSPI, OneStep, Scatter, make_dims
     dev = torch.device(device or "cuda:0")
+    if refs is not None:
+        nref, planes = int(refs[0].shape[1]), int(refs[0].shape[2])
     planes = planes or max(8, min(256, (16 << 20) // (n * n)))
     rng = np.random.default_rng(seed)
     dims = make_dims(n, n, planes, 1, n // 2, n // 2, 0.0, 10.0, 0.0, 10.0, 3.0, 0.5, 0.0, 10.0, 1)
     step = OneStep(FWIX_PSPI, dims, nref=nref, taperx=min(20, n // 4), tapery=min(20, n // 4))
-    sref = torch.from_numpy((1.0 / (1500.0 + 1500.0 * rng.random((1, nref, 1))) ** 2).astype(np.complex64))
-    sref = sref.to(dev).expand(1, nref, planes).contiguous()
-    lab = torch.from_numpy(rng.integers(0, nref, (1, 1, n, n)).astype(np.uint8)).to(dev)
-    lab = lab.expand(1, planes, n, n).contiguous()
+    if freqs is not None:
+        step.set_frequencies(freqs)
+    if refs is not None:
+        sref, lab = refs
+    else:
+        sref = torch.from_numpy((1.0 / (1500.0 + 1500.0 * rng.random((1, nref, 1))) ** 2).astype(np.complex64))
+        sref = sref.to(dev).expand(1, nref, planes).contiguous()
+        lab = torch.from_numpy(rng.integers(0, nref, (1, 1, n, n)).astype(np.uint8)).to(dev)
+        lab = lab.expand(1, planes, n, n).contiguous()
     step.set_refs(sref, lab)
     step.set_depth(0)
     scat = Scatter(dims, taylor=taylor)
+    if freqs is not None:
+        call_freq = getattr(scat, "set_frequencies", None)
+        if call_freq is not None:
+            call_freq(freqs)
     s = torch.from_numpy((1.0 / (1500.0 + 1500.0 * rng.random((1, 1, n // 2, n // 2))) ** 2).astype(np.complex64))
     slow = s.to(dev).expand(1, planes, n // 2, n // 2).contiguous()
     scat.set_slow(slow)
@@ -152,6 +168,6 @@ def tune_size(n: int, planes: int | None = None, nref: int = 4, taylor: int = 1,
     out["chosen"] = chosen
     out = {k: ({str(b): (None if t is None else round(t, 2)) for b, t in v.items()} if k != "chosen" else v)
            for k, v in out.items()}
-    del step, scat, u, v, w, slow, sref, lab
+    del step, scat, u, v, w, slow
     torch.cuda.empty_cache()
     return out
