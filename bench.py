@@ -18,6 +18,8 @@ a stream copy measures the achievable HBM bandwidth (`roofline.peak_measured`).
 Other workloads (same JSON shape, their own metric where it differs):
 
     --config c1        the c1 grid through the same Born step
+    --config c3        NSPS Downward forward + adjoint in the salt model,
+                       beside the same recursion with PSPI steps
     --config c4        64 shots of uneven aperture / frequency count as shot
                        batches (`pyfwix_amd/batching.py`), vs one shot per
                        operator, plus a PSPI step sample vs the plain path
@@ -460,6 +462,94 @@ def bench_shot_batch(args):
     print(json.dumps(res), flush=True)
 
 
+def bench_nsps(args):
+    """NSPS propagation (Downward forward + adjoint over every depth) in the
+    layered + salt model, beside the same recursion with PSPI steps."""
+    import torch
+    from pyfwix_amd import _lib
+    from pyfwix_amd.ops import FWIX_NSPS, FWIX_PSPI, OneWay
+    from pyfwix_amd.workloads import NSPS_CONFIGS, nsps_workload, workload_dims
+
+    dev = torch.device("cuda:0")
+    t_prep = time.perf_counter()
+    w = nsps_workload(args.config, **NSPS_CONFIGS[args.config])
+    t_prep = time.perf_counter() - t_prep
+    dims = workload_dims(w)
+    sref = torch.from_numpy(w.sref1).to(dev).expand(w.nz, w.nref, w.NW).contiguous()
+    labels = torch.from_numpy(w.labels1).to(dev).expand(w.nz, w.NW, w.NY, w.NX).contiguous()
+    zt = w.nz // 2
+    tuned = None if args.no_autotune else autotune_sizes(
+        [w.NY], refs=(sref[zt:zt + 1].contiguous(), labels[zt:zt + 1].contiguous()),
+        freqs=w.ow + w.dw * np.arange(w.NW))
+    # the wavelet injected at the source node of every frequency plane
+    u0 = torch.zeros((1, w.NW, w.NY, w.NX), dtype=torch.complex64, device=dev)
+    u0[0, :, w.ny // 2, w.nx // 2] = torch.from_numpy(w.wavelet[0]).to(dev)
+    out = {}
+    passes = {}
+    for name, method in (("nsps", FWIX_NSPS), ("pspi", FWIX_PSPI)):
+        op = OneWay(0, method, dims, nref=w.nref, taperx=w.taper, tapery=w.taper)
+        op.set_refs(sref, labels)
+        d = torch.zeros_like(u0)
+        m = torch.zeros_like(u0)
+        op.forward(False, u0, d)
+        op.adjoint(False, m, d)
+        torch.cuda.synchronize()
+        timed = name == "nsps" and not args.no_pass_timing
+        if timed:
+            _lib.timing_reset()
+            _lib.timing_select(None, max(1, args.pass_sample))
+            _lib.timing_enable(1 << 15)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            op.forward(False, u0, d)
+            op.adjoint(False, m, d)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        if timed:
+            _lib.timing_disable()
+            passes = _lib.timing_collect()
+            _lib.timing_select(None, 1)
+        out[name] = (2 * w.nz * w.points * args.steps / wall / 1e9, wall / args.steps)
+        del op, d, m
+        torch.cuda.empty_cache()
+    roof = None
+    if passes:
+        est = {k: v["ms"] / v["launches"] * v["total_launches"] for k, v in passes.items()}
+        dom_name = max(est, key=est.get)
+        dom = passes[dom_name]
+        avg_us = dom["ms"] * 1e3 / dom["launches"]
+        achieved = dom["bytes"] / dom["launches"] / (avg_us * 1e-6) / 1e9
+        roof = {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "alg_bytes_per_launch": round(dom["bytes"] / dom["launches"]), "avg_us": round(avg_us, 2),
+                "launches_timed": dom["launches"], "launches": dom["total_launches"]}
+    res = {
+        "metric": "wavefield Gpts*freq/s per depth step (fwd+adj)",
+        "value": round(out["nsps"][0], 4),
+        "unit": "Gpts*freq/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": round(out["nsps"][1] * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex64",
+        "data": "synthetic (layered 1500-4500 m/s + salt ellipsoid + 3 % seeded fluctuation, Ricker point source)",
+        "config": {"workload": f"{args.config}: NSPS Downward forward + adjoint over every depth, single shot",
+                   "nx": w.nx, "ny": w.ny, "nz": w.nz, "NX": w.NX, "NY": w.NY, "nw": w.NW, "nref": w.nref,
+                   "band_hz": [w.ow, w.ow + w.dw * (w.NW - 1)]},
+        "roofline": roof,
+        "pspi_same_recursion": {"value": round(out["pspi"][0], 4), "ms_per_step": round(out["pspi"][1] * 1e3, 3)},
+        "passes": {k: {"avg_us": round(v["ms"] * 1e3 / v["launches"], 2),
+                       "GBs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)} for k, v in passes.items()},
+        "column_budgets": tuned,
+        "host_model_prep_s": round(t_prep, 1),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(res), flush=True)
+
+
 def bench_fwi(args):
     """FWI-style slowness gradient: per launch (a group of shots that fits the
     HBM budget), background modeling on the start model with checkpoints every
@@ -675,7 +765,10 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from pyfwix_amd.workloads import FWI_CONFIGS, SHOT_BATCH_CONFIGS
+    from pyfwix_amd.workloads import FWI_CONFIGS, NSPS_CONFIGS, SHOT_BATCH_CONFIGS
+    if args.config in NSPS_CONFIGS:
+        bench_nsps(args)
+        return
     if args.config in SHOT_BATCH_CONFIGS:
         bench_shot_batch(args)
         return
