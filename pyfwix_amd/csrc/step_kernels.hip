@@ -580,7 +580,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   // (STAGE): a global load in the loop would wait (one counter for all vector
   // memory operations) for the next term's tile, which is fetched into
   // registers while this term is transformed
-  constexpr bool STAGE = (N <= 512 || WPE <= 2) && col_stage_lds<N, WPE>();  // 384 / 768 at 4 waves would spill
+  constexpr bool STAGE = col_stage_lds<N, WPE>();
   __shared__ float2 twl[STAGE ? N : 1];
   __shared__ float2 srl[STAGE ? 256 : 1];
   tile_row_masks<C>(rmk, rm);
