@@ -23,7 +23,7 @@ constexpr int row_nb(int N) {
 constexpr int col_wpe_default(int kind, int mode, int N) {
   const bool pow2 = !size_has3(N);
   (void)mode;
-  if (kind == 0) return (pow2 && N <= 512) ? 6 : (N >= 384) ? 4 : 2;
+  if (kind == 0) return (pow2 && N <= 512) ? 6 : (N == 384 || N == 768) ? 4 : 2;
   return (pow2 && N <= 512) || N == 768 ? 4 : 2;
 }
 constexpr int col_cw(int N) {

@@ -113,7 +113,7 @@ def test_refsampler_kmeans_is_seeded_and_finds_clusters():
 def test_column_budget_table():
     """Register-budget table of the column kernels: defaults, set/get, validation."""
     from pyfwix_amd import autotune
-    assert autotune.get(0, 0, 512) == 6 and autotune.get(1, 1, 512) == 4 and autotune.get(0, 0, 1024) == 4
+    assert autotune.get(0, 0, 512) == 6 and autotune.get(1, 1, 512) == 4 and autotune.get(0, 0, 1024) == 2
     assert autotune.get(1, 0, 1024) == 2 and autotune.get(1, 2, 768) == 4 and autotune.get(0, 1, 384) == 4
     assert autotune.get(1, 1, 384) == 2
     old = autotune.get(1, 2, 768)
