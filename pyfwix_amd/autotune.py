@@ -110,11 +110,7 @@ def tune_size(n: int, planes: int | None = None, nref: int = 4, taylor: int = 1,
         lab = lab.expand(1, planes, n, n).contiguous()
     step.set_refs(sref, lab)
     step.set_depth(0)
-    scat = Scatter(dims, taylor=taylor)
-    if freqs is not None:
-        call_freq = getattr(scat, "set_frequencies", None)
-        if call_freq is not None:
-            call_freq(freqs)
+    scat = Scatter(dims, taylor=taylor)  # its frequency axis stays the regular one of `dims`
     s = torch.from_numpy((1.0 / (1500.0 + 1500.0 * rng.random((1, 1, n // 2, n // 2))) ** 2).astype(np.complex64))
     slow = s.to(dev).expand(1, planes, n // 2, n // 2).contiguous()
     scat.set_slow(slow)

@@ -163,10 +163,8 @@ class ShotBatch:
         self.ny = max(s.ny for s in self.shots)
         self.NX = NX or padded_size(self.nx)
         self.NY = NY or padded_size(self.ny)
-        for s in self.shots:
-            if padded_size(s.nx) != self.NX or padded_size(s.ny) != self.NY:
-                if NX is None:
-                    raise ValueError("shots of a batch must share one padded grid")
+        if NX is None and any(padded_size(s.nx) != self.NX or padded_size(s.ny) != self.NY for s in self.shots):
+            raise ValueError("shots of a batch must share one padded grid (or pass NX, NY)")
         self.nref = int(params.get("nref", 4))
         self.counts = np.array([s.nw for s in self.shots], np.int32)
         self.first = np.concatenate([[0], np.cumsum(self.counts)[:-1]]).astype(np.int64)
