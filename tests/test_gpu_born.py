@@ -70,6 +70,51 @@ def test_injection_matches_oracle(dev):
         assert dot_err(tr, Ax.cpu().numpy(), wf, t2.cpu().numpy()) < TOL_DOT
 
 
+def test_injection_edges_and_empty(dev):
+    """Edge cases of injection.cu:38-75: a trace on the last interpolating node
+    (ix = NX-2, iy = NY-2), traces one node further (skipped), above the first
+    level or below the last (only their in-range level is used); then traces
+    that are all skipped, and no traces at all."""
+    import torch
+    from pyfwix_amd.ops import Injection
+    rng = np.random.default_rng(43)
+    g = grid(32, 24, 3, 2, nx=28, ny=20, nz=3)
+    shape = (g.NS, g.NW, g.NY, g.NX)
+    f32 = np.float32
+    cx = np.array([g.ox + (g.NX - 1.5) * g.dx, g.ox + (g.NX - 0.9) * g.dx, g.ox + 3.2 * g.dx,
+                   g.ox + 4.7 * g.dx, g.ox + 5.1 * g.dx, g.ox - 0.2 * g.dx], f32)
+    cy = np.array([g.oy + (g.NY - 1.6) * g.dy, g.oy + 2.0 * g.dy, g.oy + (g.NY - 0.5) * g.dy,
+                   g.oy + 6.3 * g.dy, g.oy + 7.9 * g.dy, g.oy + 1.0 * g.dy], f32)
+    cz = np.array([g.oz + 0.4 * g.dz, g.oz + 1.2 * g.dz, g.oz, g.oz - 0.3 * g.dz,
+                   g.oz + (g.nz - 0.6) * g.dz, g.oz + 0.5 * g.dz], f32)
+    ids = np.array([0, 1, 0, 1, 0, 1], np.int32)
+    cases = {"edges": (cx, cy, cz, ids),
+             "all skipped": (cx[[1, 2, 5]], cy[[1, 2, 5]], cz[[1, 2, 5]], ids[[1, 2, 5]]),
+             "no traces": (cx[:0], cy[:0], cz[:0], ids[:0])}
+    for name, (x, y, z, i) in cases.items():
+        op = Injection(dims_of(g), x, y, z, i)
+        tr = crand(rng, (len(x), g.NW))
+        wf = crand(rng, shape)
+        for iz in range(g.nz):
+            op.set_depth(iz)
+            out = _t(wf, dev)
+            op.forward(True, _t(tr, dev), out)
+            ref = O.injection_forward(tr, x, y, z, i, g, iz, wfld=wf, add=True, shape=shape)
+            assert rel(out.cpu().numpy(), ref) < 1e-6, (name, iz)
+            if name != "edges":
+                assert torch.equal(out, _t(wf, dev)), (name, iz)  # nothing injected, nothing touched
+            out = _t(wf, dev)
+            op.forward(False, _t(tr, dev), out)
+            assert rel(out.cpu().numpy(), O.injection_forward(tr, x, y, z, i, g, iz, shape=shape)) < 1e-6, (name, iz)
+            t2 = _t(crand(rng, (len(x), g.NW)), dev)
+            op.adjoint(False, t2, _t(wf, dev))
+            ref = O.injection_adjoint(wf, x, y, z, i, g, iz)
+            if name == "edges":
+                assert rel(t2.cpu().numpy(), ref) < 1e-6, (name, iz)
+            else:
+                assert not t2.cpu().numpy().any(), (name, iz)
+
+
 @pytest.mark.parametrize("taylor", [0, 1, 2])
 def test_scatter_matches_oracle(dev, taylor):
     from pyfwix_amd.ops import Scatter

@@ -188,6 +188,40 @@ def test_step_sparse_references(dev, NX, NY, method):
     assert dot_err(x, out.cpu().numpy(), y, m.cpu().numpy()) < TOL_DOT
 
 
+@pytest.mark.parametrize("nref", [32, 40])
+@pytest.mark.parametrize("method", ["pspi", "nsps"])
+def test_step_many_references(dev, nref, method):
+    """32 references (the widest presence masks) and more than 32 (no masks:
+    every reference's tile is transformed); labels reach the last reference."""
+    import torch
+    from pyfwix_amd.ops import FWIX_NSPS, FWIX_PSPI, OneStep
+    g = grid(64, 48, 3, 1, nz=2)
+    rng = np.random.default_rng(91)
+    sref = (1.0 / rng.uniform(1500, 4500, (2, nref, g.NW)) ** 2).astype(np.complex64)
+    labels = np.zeros((2, g.NW, g.NY, g.NX), np.uint8)
+    labels[1, 0] = rng.integers(0, nref, (g.NY, g.NX))      # every reference present
+    labels[1, 1, : g.NY // 2] = nref - 1                    # the last reference alone in some rows
+    labels[1, 1, g.NY // 2:] = rng.choice([0, 17], (g.NY - g.NY // 2, g.NX))
+    labels[1, 2] = rng.choice([nref - 2, 30], (g.NY, g.NX))  # most references absent
+    op = OneStep(FWIX_PSPI if method == "pspi" else FWIX_NSPS, dims_of(g), nref=nref, taperx=4, tapery=3)
+    op.set_refs(_t(sref, dev), _t(labels, dev, np.uint8))
+    op.set_depth(1)
+    shape = (g.NS, g.NW, g.NY, g.NX)
+    x, y = crand(rng, shape), crand(rng, shape)
+    s64, lab = sref[1].astype(np.complex128), labels[1].astype(np.int64)
+    out = torch.zeros(shape, dtype=torch.complex64, device=dev)
+    op.forward(False, _t(x, dev), out)
+    ref = (O.pspi_forward(x, s64, lab, g, g.dz, 0.0, 4, 3) if method == "pspi"
+           else O.nsps_forward(x, s64, lab, g, g.dz, 0.0))
+    assert rel(out.cpu().numpy(), ref) < TOL_WFLD
+    m = torch.zeros_like(out)
+    op.adjoint(False, m, _t(y, dev))
+    ref = (O.pspi_adjoint(y, s64, lab, g, g.dz, 0.0, 4, 3) if method == "pspi"
+           else O.nsps_adjoint(y, s64, lab, g, g.dz, 0.0))
+    assert rel(m.cpu().numpy(), ref) < TOL_WFLD
+    assert dot_err(x, out.cpu().numpy(), y, m.cpu().numpy()) < TOL_DOT
+
+
 @pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[3]], ids=lambda c: "x".join(map(str, c)))
 def test_plain_path_matches_fused_and_oracle(dev, case):
     """The per-operator comparator path computes the same PSPI step."""
