@@ -20,6 +20,9 @@ CASES = [  # NX, NY, NW, NS, nref, taper
     (512, 384, 1, 1, 5, 0),
     (768, 128, 1, 1, 2, 8),
     (128, 1024, 1, 1, 3, 0),
+    (8, 12, 3, 2, 2, 2),      # smallest row / column transforms
+    (12, 8, 2, 1, 3, 0),
+    (1024, 8, 1, 2, 2, 4),    # 8-point columns in two 512-column tiles
 ]
 
 
