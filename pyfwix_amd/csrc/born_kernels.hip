@@ -41,21 +41,22 @@ __device__ __forceinline__ void scatter_factors(const KArgs& a, int im, int iy, 
   const int xx = ix < a.nx ? ix : a.nx - 1;
   const float2 s = a.slow[(unsigned)((im * a.ny + yy) * a.nx + xx)];
   const float sr = s.x, si = s.y - a.eps_s * s.x;
-  // principal sqrt of (sr, si)
-  const float r = sqrtf(fmaf(sr, sr, si * si));
+  // principal sqrt q of (sr, si), from the hardware square root and
+  // reciprocal (1 ulp each); |q|^2 = |s~| = r gives both factors from 1/r
+  const float r = __builtin_amdgcn_sqrtf(fmaf(sr, sr, si * si));
   float qr, qi;
   if (sr >= 0.f) {
-    qr = sqrtf(0.5f * (r + sr));
-    qi = qr > 0.f ? 0.5f * si / qr : 0.f;
+    qr = __builtin_amdgcn_sqrtf(0.5f * (r + sr));
+    qi = qr > 0.f ? 0.5f * si * __builtin_amdgcn_rcpf(qr) : 0.f;
   } else {
-    const float t = sqrtf(0.5f * (r - sr));
-    qr = 0.5f * fabsf(si) / t;
+    const float t = __builtin_amdgcn_sqrtf(0.5f * (r - sr));
+    qr = 0.5f * fabsf(si) * __builtin_amdgcn_rcpf(t);
     qi = copysignf(t, si);
   }
-  const float iq = 1.f / fmaf(qr, qr, qi * qi);
-  rsq = make_float2(qr * iq, -qi * iq);                // 1/sqrt(s~)
-  const float is2 = 1.f / fmaf(sr, sr, si * si);
-  inv = make_float2(sr * is2, -si * is2);              // 1/s~
+  const float ir = __builtin_amdgcn_rcpf(r);
+  rsq = make_float2(qr * ir, -qi * ir);                // 1/sqrt(s~) = conj(q)/|s~|
+  const float is2 = ir * ir;
+  inv = make_float2(sr * is2, -si * is2);              // 1/s~ = conj(s~)/|s~|^2
 }
 __device__ __forceinline__ float taylor_coef(int it) {  // Scatter.h:73
   return it == 0 ? 1.f : it == 1 ? 0.5f : it == 2 ? 0.375f : it == 3 ? 0.3125f : 0.2734375f;
