@@ -752,6 +752,13 @@ def main():
     ap.add_argument("--summarize-traffic", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT"))
     ap.add_argument("--command", action="append", default=[], help="profiling command recorded in a summary")
     args = ap.parse_args()
+    if os.environ.get("ROCPROF_COUNTER_COLLECTION", "").strip().lower() in ("1", "true", "yes", "on"):
+        # Under rocprofv3 --pmc the profiler intercepts the dispatch queue and
+        # serializes kernels with its own packets; the barrier packets that
+        # hipEventRecord puts in the queue for the per-pass timers are not
+        # handled there ("A barrier packet in the queue!", then
+        # HSA_STATUS_ERROR_INVALID_PACKET_FORMAT), so counter runs record no events.
+        args.no_pass_timing = True
     if args.summarize_stats:
         summarize_stats(*args.summarize_stats, command=" ".join(args.command))
         return

@@ -25,7 +25,7 @@ __all__ = ["BornFunction", "OneWayFunction", "born", "oneway", "born_misfit_grad
 class BornFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, dslow: torch.Tensor, op: Born) -> torch.Tensor:
-        data = torch.empty((op.nrec, op.dims.NW), dtype=torch.complex64, device=dslow.device)
+        data = torch.empty(op.trace_shape(True), dtype=torch.complex64, device=dslow.device)
         op.forward(False, dslow.contiguous(), data)
         ctx.op = op
         return data
@@ -33,8 +33,7 @@ class BornFunction(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g: torch.Tensor):
         op = ctx.op
-        d = op.dims
-        grad = torch.empty((d.nz, d.NW, d.ny, d.nx), dtype=torch.complex64, device=g.device)
+        grad = torch.empty(op.model_shape(), dtype=torch.complex64, device=g.device)
         op.adjoint(False, grad, g.contiguous())
         return grad, None
 

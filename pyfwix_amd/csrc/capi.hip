@@ -4,6 +4,7 @@
 #include "fwix_launch.h"
 #include "capi_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -210,7 +211,10 @@ PwArgs Tables::pwargs() const {
 
 int auto_chunk(const fwix_dims& d, int nterm, int requested) {
   const int planes = d.NW * d.NS;
-  if (requested > 0) return requested < planes ? requested : planes;
+  // The pass kernels index a chunk's planes with 32-bit element offsets
+  // (row * NX + x), so one chunk stays below 2^31 complex samples.
+  const int c_index = (int)std::min<long long>(planes, ((1LL << 31) - 1) / ((long long)d.NX * d.NY));
+  if (requested > 0) return std::min(requested, c_index);
   // Enough planes that a column pass launches >= 4096 workgroups (16 per CU)
   // and a row pass >= 16 waves per SIMD slot, so every launch reaches a
   // steady state of loads overlapping transforms instead of one burst; at
@@ -223,6 +227,7 @@ int auto_chunk(const fwix_dims& d, int nterm, int requested) {
   const int c_max = (int)(4096.0 * 1024 * 1024 / per_plane);
   if (c > c_max) c = c_max;
   if (c < 1) c = 1;
+  if (c > c_index) c = c_index;
   if (c >= planes) return planes;
   const int nchunks = (planes + c - 1) / c;
   return (planes + nchunks - 1) / nchunks;
@@ -238,6 +243,8 @@ void StepEngine::init(int method_, const fwix_dims& d, const fwix_params& p, int
   mplane.clear();
   const size_t plane = (size_t)d.NX * d.NY;
   if (plain) {  // two whole-shot scratch wavefields
+    if ((long long)d.NW * d.NX * d.NY >= (1LL << 31))
+      throw Error("the per-operator path transforms a whole shot per launch: NW * NX * NY must stay below 2^31");
     chunk = d.NW;
     W.release();
     K.alloc(plane * d.NW * sizeof(float2));

@@ -343,15 +343,20 @@ class Born:
         self.nmodel = dims.NW        # model planes: NW, or the shots of a batch
         self.shot_planes = None
 
-    def _model_shape(self):
+    def model_shape(self):
+        """Shape of slowness, model and gradient: (nz, NW, ny, nx), or (nz, nshot, ny, nx) for a shot batch."""
         d = self.dims
         return (d.nz, self.nmodel, d.ny, d.nx)
 
-    def _trace_shape(self, receivers: bool):
+    def trace_shape(self, receivers: bool):
+        """Shape of the receiver (or source) traces: (n, NW), or the flat ragged (samples,) of a shot batch."""
         n = self.nrec if receivers else self.nsrc
         if self.shot_planes is None:
             return (n, self.dims.NW)
         return (int(_lib.lib.fwix_born_trace_samples(self._h.h, int(receivers))),)
+
+    _model_shape = model_shape
+    _trace_shape = trace_shape
 
     def set_method(self, method: int) -> None:
         """FWIX_PSPI (fused steps) or FWIX_PSPI_PLAIN (separate operators, one shot per launch)."""
