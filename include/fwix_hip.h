@@ -139,6 +139,22 @@ int fwix_onestep_set_frequencies(fwix_op op, const float* f_hz);
 int fwix_onestep_set_shot_planes(fwix_op op, int nshot, const int* nplanes);
 int fwix_onestep_forward(fwix_op op, int add, const fwix_c64* model, fwix_c64* data, fwix_stream s);
 int fwix_onestep_adjoint(fwix_op op, int add, fwix_c64* model, const fwix_c64* data, fwix_stream s);
+/* One PSPI or NSPS depth step that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same result as fwix_onestep_forward (adjoint = 0: data (+)= P model) or
+ * fwix_onestep_adjoint (adjoint = 1: model (+)= P^H data) of a handle made
+ * with the same method, dims and params, for one depth: sref (nref, NW) and
+ * labels (NW, NY, NX) of that depth, device pointers.  The step's tables and
+ * scratch live in one library-owned device array of
+ * FWIX_NOALLOC_SCRATCH_BYTES per GPU (held from the first call on a GPU until
+ * the process ends); the planes run in chunks that fit it, and a grid whose
+ * tables plus (nref + 1) planes do not fit is rejected with an error.  Calls
+ * that overlap in time on one GPU race on that array. */
+#define FWIX_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+int fwix_onestep_apply_noalloc(int method, int adjoint, const fwix_dims* d, const fwix_params* p,
+                               const fwix_c64* sref_dev, const uint8_t* labels_dev, int add, fwix_c64* model,
+                               fwix_c64* data, fwix_stream s);
 
 /* ---------------------------------------------------------------- Injection
  * Injection(domain = traces (ntrace, NW), range = wavefield, oz, dz, cx, cy,

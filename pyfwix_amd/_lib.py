@@ -76,6 +76,7 @@ _SIGS = {
     "fwix_onestep_set_shot_planes": [_P, _I, _P],
     "fwix_onestep_forward": [_P, _I, _P, _P, _P],
     "fwix_onestep_adjoint": [_P, _I, _P, _P, _P],
+    "fwix_onestep_apply_noalloc": [_I, _I, _D, _PP, _P, _P, _I, _P, _P, _P],
     "fwix_injection_create": [_D, _I, _OPP],
     "fwix_injection_set_coords": [_P, _P, _P, _P, _P],
     "fwix_injection_set_depth": [_P, _I],

@@ -58,7 +58,7 @@ hipEvent_t take_event() {
 }  // namespace
 
 PassTimer::PassTimer(const char* name, double bytes, hipStream_t s) {
-  if (!g_timing) return;
+  if (!g_timing || !name) return;
   // no event pairs inside a captured graph: its replays would never be collected
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return;
@@ -346,42 +346,56 @@ KArgs StepEngine::base(int iz, int c0, int np) const {
   return a;
 }
 
-// out (+)= P_z in     (forward) or    out (+)= P_z^H in    (adjoint)
-void StepEngine::apply(bool adjoint, int iz, int add, const float2* in, float2* out, hipStream_t st) {
-  check_ready(iz);
-  if (plain) return apply_plain(adjoint, iz, add, in, out, st);
-  const fwix_dims& d = tables.dims;
+// One depth step over every plane of `in`, run as chunks of `chunk` planes
+// through the three fused passes.  `proto` carries the tables, scratch,
+// depth slices and masks (its p0 / np / wstride are set per chunk);
+// `rows(p0, np)` gives the per-reference rows a chunk moves, for the pass
+// timers (null: no timers).
+void step_passes(int method, bool adjoint, const fwix_dims& d, const fwix_params& p, const KArgs& proto, int chunk,
+                 int add, const float2* in, float2* out, hipStream_t st,
+                 const std::function<double(int, int)>* rows) {
   const size_t plane = (size_t)d.NX * d.NY;
   const int planes = d.NW * d.NS;
   const float sx = (float)(1.0 / std::sqrt((double)d.NX));
   const float sy2 = (float)(1.0 / (double)d.NY);
   const bool pspi = method == FWIX_PSPI;
-  const bool taper = pspi && (params.taperx > 0 || params.tapery > 0);
-  const int nr = params.nref;
+  const bool taper = pspi && (p.taperx > 0 || p.tapery > 0);
   for (int c0 = 0; c0 < planes; c0 += chunk) {
     const int np = (planes - c0) < chunk ? (planes - c0) : chunk;
-    const double pb = 8.0 * np * plane;           // bytes of one chunk-sized complex64 array
-    const double wb = 8.0 * d.NX * present_rows(iz, c0, np);  // per-reference rows moved
-    KArgs a = base(iz, c0, np);
+    const double pb = 8.0 * np * plane;  // bytes of one chunk-sized complex64 array
+    const double wb = rows ? 8.0 * d.NX * (*rows)(c0, np) : 0.0;  // per-reference rows moved
+    auto timer = [&](const char* name, double bytes) { return PassTimer(rows ? name : nullptr, bytes, st); };
+    KArgs a = proto;
+    a.p0 = c0;
+    a.np = np;
+    a.wstride = (long long)np * plane;
     const float2* cin = in + (size_t)c0 * plane;
     float2* cout = out + (size_t)c0 * plane;
     // "shared spectrum" shape: PSPI forward and NSPS adjoint
     if ((pspi && !adjoint) || (!pspi && adjoint)) {
       a.in = cin; a.out = a.K; a.add = 0; a.scale = sx;
-      { PassTimer t("rows_fwd", 2 * pb, st); hip_check(launch_rows(taper ? 2 : 0, a, st), "rows pass"); }
+      { auto t = timer("rows_fwd", 2 * pb); hip_check(launch_rows(taper ? 2 : 0, a, st), "rows pass"); }
       a.scale = sy2;
-      { PassTimer t("cols_fanout", pb + wb, st); hip_check(launch_cols_fanout(adjoint ? 1 : 0, a, st), "column fan-out pass"); }
+      { auto t = timer("cols_fanout", pb + wb); hip_check(launch_cols_fanout(adjoint ? 1 : 0, a, st), "column fan-out pass"); }
       a.out = cout; a.add = add; a.scale = sx;
-      { PassTimer t("rows_select", wb + (1 + (add ? 1 : 0)) * pb + pb / 8, st); hip_check(launch_rows_select(a, st), "rows select pass"); }
+      { auto t = timer("rows_select", wb + (1 + (add ? 1 : 0)) * pb + pb / 8); hip_check(launch_rows_select(a, st), "rows select pass"); }
     } else {  // "masked inputs" shape: PSPI adjoint and NSPS forward
       a.in = cin; a.scale = sx;
-      { PassTimer t("rows_fanout", pb + pb / 8 + wb, st); hip_check(launch_rows_fanout(a, st), "rows fan-out pass"); }
+      { auto t = timer("rows_fanout", pb + pb / 8 + wb); hip_check(launch_rows_fanout(a, st), "rows fan-out pass"); }
       a.scale = sy2;
-      { PassTimer t("cols_fanin", wb + pb, st); hip_check(launch_cols_fanin(adjoint ? 1 : 0, a, st), "column fan-in pass"); }
+      { auto t = timer("cols_fanin", wb + pb); hip_check(launch_cols_fanin(adjoint ? 1 : 0, a, st), "column fan-in pass"); }
       a.in = a.K; a.out = cout; a.add = add; a.scale = sx;
-      { PassTimer t("rows_inv", (2 + (add ? 1 : 0)) * pb, st); hip_check(launch_rows(taper ? 3 : 1, a, st), "rows inverse pass"); }
+      { auto t = timer("rows_inv", (2 + (add ? 1 : 0)) * pb); hip_check(launch_rows(taper ? 3 : 1, a, st), "rows inverse pass"); }
     }
   }
+}
+
+// out (+)= P_z in     (forward) or    out (+)= P_z^H in    (adjoint)
+void StepEngine::apply(bool adjoint, int iz, int add, const float2* in, float2* out, hipStream_t st) {
+  check_ready(iz);
+  if (plain) return apply_plain(adjoint, iz, add, in, out, st);
+  const std::function<double(int, int)> rows = [&](int c0, int np) { return present_rows(iz, c0, np); };
+  step_passes(method, adjoint, tables.dims, params, base(iz, 0, chunk), chunk, add, in, out, st, &rows);
 }
 
 // The PSPI step at the reference's operator granularity, one shot per launch

@@ -2,6 +2,7 @@
 // scratch buffers, error plumbing).  Not part of the public ABI.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -80,14 +81,20 @@ void validate_dims(const fwix_dims* d, bool need_fft);
 // Live per-pass timing (fwix_timing_*): HIP events recorded on the launch
 // stream around every pass of the hot path, with the pass's compulsory HBM
 // bytes.  A no-op unless enabled.
+// A null name records nothing.
 struct PassTimer {
   PassTimer(const char* name, double bytes, hipStream_t st);
+  PassTimer(const PassTimer&) = delete;
+  PassTimer& operator=(const PassTimer&) = delete;
   ~PassTimer();
   int slot = -1;
   hipStream_t st = nullptr;
 };
 void validate_params(const fwix_params* p);
 int auto_chunk(const fwix_dims& d, int nterm, int requested);
+void step_passes(int method, bool adjoint, const fwix_dims& d, const fwix_params& p, const KArgs& proto, int chunk,
+                 int add, const float2* in, float2* out, hipStream_t st,
+                 const std::function<double(int, int)>* rows);
 
 // One depth step (PSPI or NSPS) over a whole wavefield, forward or adjoint,
 // run as chunks of planes through the three fused passes.
