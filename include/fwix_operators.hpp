@@ -60,9 +60,11 @@ class DeviceVector {
   fwix_c64* data() { return p_; }
   const fwix_c64* data() const { return p_; }
   size_t size() const { return n_; }
+  // returns once the host vector has been read (it may be released right after)
   void upload(const hostvec& h, hipStream_t s) {
     if (h.size() != n_) throw std::runtime_error("host vector size does not match the operator");
     hip_ok(hipMemcpyAsync(p_, h.data(), n_ * sizeof(fwix_c64), hipMemcpyHostToDevice, s), "upload");
+    hip_ok(hipStreamSynchronize(s), "synchronize");
   }
   void download(hostvec& h, hipStream_t s) const {
     h.resize(n_);

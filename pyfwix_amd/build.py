@@ -59,7 +59,29 @@ def build(verbose: bool = True, jobs: int | None = None) -> Path:
     if verbose:
         print(f"built {LIB} ({LIB.stat().st_size // 1024} KiB) from {len(srcs)} sources")
     build_cpp_tests(verbose)
+    build_pyfwix(verbose)
     return LIB
+
+
+def build_pyfwix(verbose: bool = True) -> Path:
+    """The pybind11 module ``pyfwix_amd/_pyFWIX*.so`` (the reference's pyFWIX classes)."""
+    import sysconfig
+
+    import pybind11
+    src = CSRC / "pyFWIX.cpp"
+    out = PKG / ("_pyFWIX" + sysconfig.get_config_var("EXT_SUFFIX"))
+    tmp = out.with_name(out.name + ".tmp")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}", "-fvisibility=hidden",
+           "-I", str(ROOT / "include"), "-I", sysconfig.get_paths()["include"], "-I", pybind11.get_include(),
+           str(src), "-L", str(PKG), "-lpyfwix_hip", "-Wl,-rpath,$ORIGIN", "-o", str(tmp),
+           "-Wno-unused-command-line-argument"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"pyFWIX module failed to build:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, out)
+    if verbose:
+        print(f"built {out}")
+    return out
 
 
 def build_cpp_tests(verbose: bool = True) -> Path | None:

@@ -47,6 +47,11 @@ class hypercube:
     def getAxis(self, i: int) -> axis:
         return self.axes[i - 1]
 
+    @property
+    def cppMode(self) -> "hypercube":
+        """The object the compiled module (pyFWIX) receives, as SEP's ``.cppMode``."""
+        return self
+
     def getAxes(self):
         return list(self.axes)
 
@@ -91,6 +96,11 @@ class complexVector:
     # --- SepVector-like accessors
     def getHyper(self) -> hypercube:
         return self.hyper
+
+    @property
+    def cppMode(self) -> "complexVector":
+        """The object the compiled module (pyFWIX) receives, as SEP's ``.cppMode``."""
+        return self
 
     def getNdArray(self):
         return self.arr
@@ -165,6 +175,11 @@ class paramObj:
 
     def __init__(self, params: dict | None = None, **kw):
         self.params = dict(params or {}, **kw)
+
+    @property
+    def cppMode(self) -> "paramObj":
+        """The object the compiled module (pyFWIX) receives, as SEP's ``.cppMode``."""
+        return self
 
     def _get(self, name, default, cast):
         if name in self.params:
