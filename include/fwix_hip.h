@@ -264,6 +264,17 @@ long long fwix_born_trace_samples(fwix_op op, int receivers);
  * Forward and adjoint read the same decoded fields, so they stay adjoint.
  * Set before fwix_born_model. */
 int fwix_born_set_codec(fwix_op op, int codec);
+/* Arithmetic of the Born steps.  The first Taylor term of the scattering
+ * operator weighs wavenumber k by k^2 / (w^2 s) (Scatter.h:73,
+ * scatter.cu:23-59): up to ~10^3 at a few Hz on 10 m grids, so complex64
+ * rounding of the amplified terms (in a transform, or a product in space)
+ * lands on the propagating band the recordings see.  FWIX_PREC_F64_SCATTER
+ * runs every pass of the Born adjoint step and the scattering + propagation
+ * passes of the Born forward step in float64 arithmetic (values still stored
+ * as complex64 between passes); FWIX_PREC_F32 (default) keeps float32.  No
+ * counterpart in the reference (float32 only). */
+enum { FWIX_PREC_F32 = 0, FWIX_PREC_F64_SCATTER = 1 };
+int fwix_born_set_precision(fwix_op op, int precision);
 /* The codec on any device array of n complex samples: q holds n short2,
  * scales (n + 63) / 64 floats. */
 int fwix_bfp16_pack(const fwix_c64* in, void* q, float* scales, size_t n, fwix_stream s);

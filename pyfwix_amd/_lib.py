@@ -27,6 +27,8 @@ lib = C.CDLL(str(_LIB_PATH))
 FWIX_PSPI = 0
 FWIX_NSPS = 1
 FWIX_PSPI_PLAIN = 2
+FWIX_PREC_F32 = 0
+FWIX_PREC_F64_SCATTER = 1
 
 
 class fwix_dims(C.Structure):
@@ -112,6 +114,7 @@ _SIGS = {
     "fwix_born_migrate": [_P, _I, _P, _P, _P],
     "fwix_born_set_density": [_P, _P],
     "fwix_born_set_codec": [_P, _I],
+    "fwix_born_set_precision": [_P, _I],
     "fwix_bfp16_pack": [_P, _P, _P, C.c_size_t, _P],
     "fwix_bfp16_unpack": [_P, _P, _P, C.c_size_t, _P],
     "fwix_born_model_up": [_P, _I, _P, _P],

@@ -258,6 +258,7 @@ struct BornOp : fwix_op_s {
   int ckpt = 1;
   DevBuf store, seg, U, dU, lam;
   int codec = 0;   // checkpoints: 0 lossless complex64, 1 16-bit block floating point
+  int f64 = 0;     // scattered-field transforms in float64 arithmetic (fwix_born_set_precision)
   DevBuf scales;  // codec 1: one float per 64 samples of every checkpoint
   DevBuf ubuf;    // codec 1, every depth kept: the decoded U_z
   DevBuf vstore;  // up-going background V_z of every depth (model_up)
@@ -499,21 +500,23 @@ struct BornOp : fwix_op_s {
         a.dslow = dsz;
         a.bg = bg + (size_t)p0 * plane;
         a.wstride = (long long)np * plane;
+        // with f64, the passes that carry the amplified scattering terms in
+        // space or through a transform run in float64 arithmetic
         if (T >= 1) {
           a.scale = sx;
           { PassTimer t("born_rows_ic", 2 * pb + mb, st); hip_check(launch_rows_ic_fwd(a, st), "born ic"); }
-          a.nterm = T; a.scale = sy2;
+          a.nterm = T; a.scale = sy2; a.f64 = f64;
           { PassTimer t("born_cols_kfac", (1 + T) * pb, st); hip_check(launch_cols_fanout(2, a, st), "born scatter columns"); }
         }
         a.addin = addin + (size_t)p0 * plane;
-        a.nterm = T + 1; a.scale = sx;
+        a.nterm = T + 1; a.scale = sx; a.f64 = f64;
         { PassTimer t("born_rows_scatter", (T + 3) * pb + 2 * mb, st); hip_check(launch_rows_scatter(true, a, st), "born scatter rows"); }
         a.nterm = par.nref;
         a.scale = sy2;
         { PassTimer t("cols_fanout", pb + wb, st); hip_check(launch_cols_fanout(0, a, st), "born pspi columns"); }
         a.out = out + (size_t)p0 * plane;
         a.add = 0;
-        a.scale = sx;
+        a.scale = sx; a.f64 = 0;
         { PassTimer t("rows_select", wb + pb + pb / 8, st); hip_check(launch_rows_select(a, st), "born pspi rows"); }
       }
     ++steps;
@@ -532,6 +535,7 @@ struct BornOp : fwix_op_s {
         float2* lc = lam + (size_t)p0 * plane;
         KArgs a = born_args(z, p0, np);
         a.wstride = (long long)np * plane;
+        a.f64 = f64;  // every pass of the adjoint step carries the amplified terms
         const double pb = 8.0 * np * plane, mb = 8.0 * np * d.nx * d.ny;
         const double wb = 8.0 * d.NX * eng.present_rows(z, p0, np);
         const int T = par.taylor;
@@ -990,6 +994,14 @@ int fwix_born_set_slow(fwix_op op, const fwix_c64* slow_dev) {
 }
 int fwix_born_set_codec(fwix_op op, int codec) {
   return guard([&] { as<BornOp>(op, KIND_BORN)->set_codec(codec); });
+}
+int fwix_born_set_precision(fwix_op op, int precision) {
+  return guard([&] {
+    auto* p = as<BornOp>(op, KIND_BORN);
+    if (precision != FWIX_PREC_F32 && precision != FWIX_PREC_F64_SCATTER)
+      throw Error("precision must be FWIX_PREC_F32 or FWIX_PREC_F64_SCATTER");
+    p->f64 = precision == FWIX_PREC_F64_SCATTER;
+  });
 }
 int fwix_bfp16_pack(const fwix_c64* in, void* q, float* scales, size_t n, fwix_stream s) {
   return guard([&] {

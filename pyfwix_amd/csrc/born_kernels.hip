@@ -58,6 +58,31 @@ __device__ __forceinline__ void scatter_factors(const KArgs& a, int im, int iy, 
   const float is2 = ir * ir;
   inv = make_float2(sr * is2, -si * is2);              // 1/s~ = conj(s~)/|s~|^2
 }
+// the same factors in float64 arithmetic (from the complex64 slowness)
+__device__ __forceinline__ void scatter_factors(const KArgs& a, int im, int iy, int ix, double2& rsq, double2& inv) {
+  const int yy = iy < a.ny ? iy : a.ny - 1;
+  const int xx = ix < a.nx ? ix : a.nx - 1;
+  const float2 s = a.slow[(unsigned)((im * a.ny + yy) * a.nx + xx)];
+  const double sr = s.x, si = (double)s.y - (double)a.eps_s * s.x;
+  const double r = sqrt(fma(sr, sr, si * si));
+  double qr, qi;
+  if (sr >= 0.0) {
+    qr = sqrt(0.5 * (r + sr));
+    qi = qr > 0.0 ? 0.5 * si / qr : 0.0;
+  } else {
+    const double t = sqrt(0.5 * (r - sr));
+    qr = 0.5 * fabs(si) / t;
+    qi = copysign(t, si);
+  }
+  const double ir = 1.0 / r;
+  rsq = make_double2(qr * ir, -qi * ir);
+  const double is2 = ir * ir;
+  inv = make_double2(sr * is2, -si * is2);
+}
+// taper weight of node (i, row j) in the arithmetic type T
+__device__ __forceinline__ float tap_of(const KArgs& a, int i, int j, float2*) { return a.tapx[i] * a.tapy[j]; }
+__device__ __forceinline__ double tap_of(const KArgs& a, int i, int j, double2*) { return a.tapx64[i] * a.tapy64[j]; }
+
 __device__ __forceinline__ float taylor_coef(int it) {  // Scatter.h:73
   return it == 0 ? 1.f : it == 1 ? 0.5f : it == 2 ? 0.375f : it == 3 ? 0.3125f : 0.2734375f;
 }
@@ -110,42 +135,46 @@ __global__ __launch_bounds__(kThreads) void k_rows_ic_fwd(KArgs a) {
 // (BORN: U_z * ds_z on the model region, else the input rows).
 // BORN = true : K = scale * F_x( taper * (y + addin) )   (feeds the PSPI column pass)
 // BORN = false: out = (add ? out : 0) + y                (standalone Scatter forward)
-template <int N, bool BORN, bool MAP>
+// (T: arithmetic type, float2 or double2; loads and stores stay complex64)
+template <int N, bool BORN, bool MAP, class T = float2>
 __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
   using C = BRowCfg<N>;
-  using Fi = WaveFft<N, C::NB, false, true, PadLds<N>>;
-  using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
-  __shared__ float2 lds[kWaves * C::LDSW];
+  using S = real_of<T>;
+  using Fi = WaveFft<N, C::NB, false, true, PadLds<N>, T>;
+  using F = WaveFft<N, C::NB, false, false, PadLds<N>, T>;
+  __shared__ T lds[kWaves * C::LDSW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nrows = a.np * a.NY;
   const int rb = (blockIdx.x * kWaves + wave) * C::NB;
   // twiddles in LDS: inside the term loops a global load would wait (one
   // counter for all vector memory operations) for the term stores / the next
   // term's loads still in flight
-  __shared__ float2 twl[N];
-  for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = a.twx[i];
+  __shared__ T twl[N];
+  const T* twg = twx_of(a, (T*)nullptr);
+  for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = twg[i];
   __syncthreads();
   if (rb >= nrows) return;
-  float2* wl = lds + wave * C::LDSW;
-  float2 acc[C::V], pw[C::V], inv[C::V];
+  T* wl = lds + wave * C::LDSW;
+  T acc[C::V], pw[C::V], inv[C::V];
 #pragma unroll
   for (int e = 0; e < C::V; ++e) {
     int b, i;
     edge_pos<N, C::NB, Fi::P::last(), false>(lane, e, b, i);
     const int lr = rb + b;
-    acc[e] = czero();
-    pw[e] = czero();
-    inv[e] = czero();
+    acc[e] = cz<T>();
+    pw[e] = cz<T>();
+    inv[e] = cz<T>();
     if (lr < nrows) {
       const int p = lr / a.NY, iy = lr - p * a.NY;
       scatter_factors(a, model_iw<MAP>(a, plane_iw(a, p)), iy, i, pw[e], inv[e]);
-      float2 v;
+      T v;
       if constexpr (BORN) {
-        v = czero();
+        v = cz<T>();
         if (iy < a.ny && i < a.nx)
-          v = cmul(a.bg[(unsigned)(lr * N + i)], a.dslow[(unsigned)((model_iw<MAP>(a, plane_iw(a, p)) * a.ny + iy) * a.nx + i)]);
+          v = cmul(from_f2<T>(a.bg[(unsigned)(lr * N + i)]),
+                   from_f2<T>(a.dslow[(unsigned)((model_iw<MAP>(a, plane_iw(a, p)) * a.ny + iy) * a.nx + i)]));
       } else {
-        v = a.in[(unsigned)(lr * N + i)];
+        v = from_f2<T>(a.in[(unsigned)(lr * N + i)]);
       }
       acc[e] = cmul(v, pw[e]);                 // c_0 = 1
       pw[e] = cmul(pw[e], inv[e]);
@@ -153,16 +182,16 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
   }
   for (int it = 1; it < a.nterm; ++it) {
     const float2* W = a.W + (size_t)(it - 1) * a.wstride;
-    float2 x[C::V];
+    T x[C::V];
 #pragma unroll
     for (int e = 0; e < C::V; ++e) {
       int b, i;
       edge_pos<N, C::NB, Fi::P::first(), false>(lane, e, b, i);
       const int lr = rb + b;
-      x[e] = lr < nrows ? W[(unsigned)(lr * N + i)] : czero();
+      x[e] = lr < nrows ? from_f2<T>(W[(unsigned)(lr * N + i)]) : cz<T>();
     }
     Fi::run(x, wl, twl, lane);
-    const float c = taylor_coef(it) * a.scale;  // the inverse row transform's 1/sqrt(NX)
+    const S c = (S)taylor_coef(it) * (S)a.scale;  // the inverse row transform's 1/sqrt(NX)
 #pragma unroll
     for (int e = 0; e < C::V; ++e) {
       acc[e] = cadd(acc[e], cscale(cmul(x[e], pw[e]), c));   // c_it s~^(-1/2-it) * term
@@ -177,8 +206,8 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
     const int lr = rb + b;
     if (lr < nrows) {
       const int p = lr / a.NY;
-      const float h = 0.5f * a.w[plane_iw(a, p)] * a.dz;
-      acc[e] = make_float2(h * acc[e].y, -h * acc[e].x);
+      const S h = (S)(0.5f * a.w[plane_iw(a, p)] * a.dz);
+      acc[e] = CX<T>::mk(h * acc[e].y, -h * acc[e].x);
     }
   }
   if constexpr (!BORN) {
@@ -189,9 +218,9 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
       const int lr = rb + b;
       if (lr < nrows) {
         const unsigned idx = lr * N + i;
-        float2 y = acc[e];
-        if (a.add) y = cadd(y, a.out[idx]);
-        a.out[idx] = y;
+        T y = acc[e];
+        if (a.add) y = cadd(y, from_f2<T>(a.out[idx]));
+        a.out[idx] = to_f2(y);
       }
     }
   } else {
@@ -202,8 +231,8 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
       edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
       const int lr = rb + b;
       if (lr < nrows) {
-        float2 y = cadd(acc[e], a.addin[(unsigned)(lr * N + i)]);
-        acc[e] = cscale(y, a.tapx[i] * a.tapy[lr % a.NY]);
+        T y = cadd(acc[e], from_f2<T>(a.addin[(unsigned)(lr * N + i)]));
+        acc[e] = cscale(y, tap_of(a, i, lr % a.NY, (T*)nullptr));
       }
     }
     F::run(acc, wl, twl, lane);
@@ -212,7 +241,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
       int b, i;
       edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
       const int lr = rb + b;
-      if (lr < nrows) a.K[(unsigned)(lr * N + i)] = cscale(acc[e], a.scale);
+      if (lr < nrows) a.K[(unsigned)(lr * N + i)] = to_f2(cscale(acc[e], a.scale));
     }
   }
 }
@@ -222,32 +251,34 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
 // FUSED = true : mu = taper * scale * F_x^-1(K) is the end of the PSPI adjoint
 //                step; it is written to out (the adjoint wavefield) as well.
 // FUSED = false: mu = in (standalone Scatter adjoint).
-template <int N, bool FUSED, bool MAP>
+template <int N, bool FUSED, bool MAP, class T = float2>
 __global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
   using C = BRowCfg<N>;
-  using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
-  using Fi = WaveFft<N, C::NB, false, true, PadLds<N>>;
+  using S = real_of<T>;
+  using F = WaveFft<N, C::NB, false, false, PadLds<N>, T>;
+  using Fi = WaveFft<N, C::NB, false, true, PadLds<N>, T>;
   static_assert(Fi::P::last() == F::P::first(), "register hand-off needs mirrored plans");
-  __shared__ float2 lds[kWaves * C::LDSW];
+  __shared__ T lds[kWaves * C::LDSW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nrows = a.np * a.NY;
   const int rb = (blockIdx.x * kWaves + wave) * C::NB;
   // twiddles in LDS: inside the term loops a global load would wait (one
   // counter for all vector memory operations) for the term stores / the next
   // term's loads still in flight
-  __shared__ float2 twl[N];
-  for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = a.twx[i];
+  __shared__ T twl[N];
+  const T* twg = twx_of(a, (T*)nullptr);
+  for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = twg[i];
   __syncthreads();
   if (rb >= nrows) return;
-  float2* wl = lds + wave * C::LDSW;
-  float2 u[C::V];
+  T* wl = lds + wave * C::LDSW;
+  T u[C::V];
   if constexpr (FUSED) {
 #pragma unroll
     for (int e = 0; e < C::V; ++e) {
       int b, i;
       edge_pos<N, C::NB, Fi::P::first(), false>(lane, e, b, i);
       const int lr = rb + b;
-      u[e] = lr < nrows ? a.K[(unsigned)(lr * N + i)] : czero();
+      u[e] = lr < nrows ? from_f2<T>(a.K[(unsigned)(lr * N + i)]) : cz<T>();
     }
     Fi::run(u, wl, twl, lane);
 #pragma unroll
@@ -256,8 +287,8 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
       edge_pos<N, C::NB, Fi::P::last(), false>(lane, e, b, i);
       const int lr = rb + b;
       if (lr < nrows) {
-        u[e] = cscale(u[e], a.scale * a.tapx[i] * a.tapy[lr % a.NY]);
-        a.out[(unsigned)(lr * N + i)] = u[e];
+        u[e] = cscale(u[e], (S)a.scale * tap_of(a, i, lr % a.NY, (T*)nullptr));
+        a.out[(unsigned)(lr * N + i)] = to_f2(u[e]);
       }
     }
   } else {
@@ -266,29 +297,29 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
       int b, i;
       edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
       const int lr = rb + b;
-      u[e] = lr < nrows ? a.in[(unsigned)(lr * N + i)] : czero();
+      u[e] = lr < nrows ? from_f2<T>(a.in[(unsigned)(lr * N + i)]) : cz<T>();
     }
   }
   if (a.nterm < 2) return;
-  float2 pw[C::V], inv[C::V];
+  T pw[C::V], inv[C::V];
 #pragma unroll
   for (int e = 0; e < C::V; ++e) {
     int b, i;
     edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
     const int lr = rb + b;
-    pw[e] = czero();
-    inv[e] = czero();
+    pw[e] = cz<T>();
+    inv[e] = cz<T>();
     if (lr < nrows) {
       const int p = lr / a.NY;
-      const float h = 0.5f * a.w[plane_iw(a, p)] * a.dz;   // scatter.cu:197-238
-      u[e] = make_float2(-h * u[e].y, h * u[e].x);
+      const S h = (S)(0.5f * a.w[plane_iw(a, p)] * a.dz);   // scatter.cu:197-238
+      u[e] = CX<T>::mk(-h * u[e].y, h * u[e].x);
       scatter_factors(a, model_iw<MAP>(a, plane_iw(a, p)), lr - p * a.NY, i, pw[e], inv[e]);
       pw[e] = cmul(pw[e], inv[e]);                          // start at order 1
     }
   }
   for (int it = 1; it < a.nterm; ++it) {
-    float2 x[C::V];
-    const float c = taylor_coef(it);
+    T x[C::V];
+    const S c = (S)taylor_coef(it);
 #pragma unroll
     for (int e = 0; e < C::V; ++e) {
       x[e] = cscale(cmulc(u[e], pw[e]), c);                // conj(c_it s~^(-1/2-it)) * u
@@ -301,7 +332,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
       int b, i;
       edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
       const int lr = rb + b;
-      if (lr < nrows) W[(unsigned)(lr * N + i)] = cscale(x[e], a.scale);
+      if (lr < nrows) W[(unsigned)(lr * N + i)] = to_f2(cscale(x[e], a.scale));
     }
   }
 }
@@ -313,34 +344,36 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
 //             is owned by exactly one lane: no atomics, deterministic (the
 //             reference's ic_adj uses float atomicAdd, imaging_condition.cu:103-104).
 // IC = false: out = (add ? out : 0) + x  (standalone Scatter adjoint).
-template <int N, bool IC, bool MAP>
+template <int N, bool IC, bool MAP, class T = float2>
 __global__ __launch_bounds__(kThreads) void k_rows_adj_out(KArgs a) {
   using C = BRowCfg<N>;
-  using Fi = WaveFft<N, C::NB, false, true, PadLds<N>>;
-  __shared__ float2 lds[kWaves * C::LDSW];
+  using S = real_of<T>;
+  using Fi = WaveFft<N, C::NB, false, true, PadLds<N>, T>;
+  __shared__ T lds[kWaves * C::LDSW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nrows = a.np * a.NY;
   const int rb = (blockIdx.x * kWaves + wave) * C::NB;
   // twiddles in LDS: inside the term loops a global load would wait (one
   // counter for all vector memory operations) for the term stores / the next
   // term's loads still in flight
-  __shared__ float2 twl[N];
-  for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = a.twx[i];
+  __shared__ T twl[N];
+  const T* twg = twx_of(a, (T*)nullptr);
+  for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = twg[i];
   __syncthreads();
   if (rb >= nrows) return;
-  float2 x[C::V];
+  T x[C::V];
   if (a.nterm > 1) {
 #pragma unroll
     for (int e = 0; e < C::V; ++e) {
       int b, i;
       edge_pos<N, C::NB, Fi::P::first(), false>(lane, e, b, i);
       const int lr = rb + b;
-      x[e] = lr < nrows ? a.K[(unsigned)(lr * N + i)] : czero();
+      x[e] = lr < nrows ? from_f2<T>(a.K[(unsigned)(lr * N + i)]) : cz<T>();
     }
     Fi::run(x, lds + wave * C::LDSW, twl, lane);
   } else {
 #pragma unroll
-    for (int e = 0; e < C::V; ++e) x[e] = czero();
+    for (int e = 0; e < C::V; ++e) x[e] = cz<T>();
   }
 #pragma unroll
   for (int e = 0; e < C::V; ++e) {
@@ -351,20 +384,20 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj_out(KArgs a) {
       const int p = lr / a.NY, iy = lr - p * a.NY;
       const int iw = plane_iw(a, p);
       const unsigned idx = lr * N + i;
-      float2 rsq, inv;
+      T rsq, inv;
       scatter_factors(a, model_iw<MAP>(a, iw), iy, i, rsq, inv);
-      const float h = 0.5f * a.w[iw] * a.dz;
-      const float2 d = a.in[idx];
-      const float2 u = make_float2(-h * d.y, h * d.x);
-      const float2 y = cadd(cscale(x[e], a.scale), cmulc(u, rsq));
+      const S h = (S)(0.5f * a.w[iw] * a.dz);
+      const T d = from_f2<T>(a.in[idx]);
+      const T u = CX<T>::mk(-h * d.y, h * d.x);
+      const T y = cadd(cscale(x[e], a.scale), cmulc(u, rsq));
       if constexpr (IC) {
         if (iy < a.ny && i < a.nx) {
           const unsigned m = (iw * a.ny + iy) * a.nx + i;
-          const float2 g = cmulc(y, a.bg[idx]);
-          a.aux[m] = a.add ? cadd(a.aux[m], g) : g;
+          const T g = cmulc(y, from_f2<T>(a.bg[idx]));
+          a.aux[m] = to_f2(a.add ? cadd(from_f2<T>(a.aux[m]), g) : g);
         }
       } else {
-        a.out[idx] = a.add ? cadd(a.out[idx], y) : y;
+        a.out[idx] = to_f2(a.add ? cadd(from_f2<T>(a.out[idx]), y) : y);
       }
     }
   }
@@ -602,29 +635,44 @@ static inline dim3 bflat(long long n) {
 
 // Born / scattering row passes; the shot-batch flavour (a.mplane set) is a
 // separate instantiation (MAP).  ScatterOp and the unfused forms never map.
-template <template <int, bool, bool> class K, bool FLAG>
+// F64: a float64-arithmetic instantiation exists (used when a.f64 is set;
+// the fused Born forms only).
+template <template <int, bool, bool, class> class K, bool FLAG, bool F64 = false>
 static hipError_t launch_mapped(const KArgs& a, hipStream_t st) {
   return bdispatch(a.NX, [&](auto nc) -> hipError_t {
     constexpr int N = decltype(nc)::value;
-    if (a.mplane) hipLaunchKernelGGL((K<N, FLAG, true>::fn), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
-    else hipLaunchKernelGGL((K<N, FLAG, false>::fn), brow_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    const dim3 g = brow_grid(N, a.np * a.NY);
+    if constexpr (F64) {
+      if (a.f64) {
+        if (a.mplane) hipLaunchKernelGGL((K<N, FLAG, true, double2>::fn), g, dim3(kThreads), 0, st, a);
+        else hipLaunchKernelGGL((K<N, FLAG, false, double2>::fn), g, dim3(kThreads), 0, st, a);
+        return hipGetLastError();
+      }
+    }
+    if (a.f64) return hipErrorInvalidValue;
+    if (a.mplane) hipLaunchKernelGGL((K<N, FLAG, true, float2>::fn), g, dim3(kThreads), 0, st, a);
+    else hipLaunchKernelGGL((K<N, FLAG, false, float2>::fn), g, dim3(kThreads), 0, st, a);
     return hipGetLastError();
   });
 }
-template <int N, bool F, bool M> struct KIcFwd { static constexpr auto fn = k_rows_ic_fwd<N, M>; };
-template <int N, bool F, bool M> struct KScatter { static constexpr auto fn = k_rows_scatter<N, F, M>; };
-template <int N, bool F, bool M> struct KScatterAdj { static constexpr auto fn = k_rows_scatter_adj<N, F, M>; };
-template <int N, bool F, bool M> struct KAdjOut { static constexpr auto fn = k_rows_adj_out<N, F, M>; };
+template <int N, bool F, bool M, class T> struct KIcFwd { static constexpr auto fn = k_rows_ic_fwd<N, M>; };
+template <int N, bool F, bool M, class T> struct KScatter { static constexpr auto fn = k_rows_scatter<N, F, M, T>; };
+template <int N, bool F, bool M, class T> struct KScatterAdj { static constexpr auto fn = k_rows_scatter_adj<N, F, M, T>; };
+template <int N, bool F, bool M, class T> struct KAdjOut { static constexpr auto fn = k_rows_adj_out<N, F, M, T>; };
 
-hipError_t launch_rows_ic_fwd(const KArgs& a, hipStream_t st) { return launch_mapped<KIcFwd, true>(a, st); }
+hipError_t launch_rows_ic_fwd(const KArgs& a, hipStream_t st) {
+  KArgs b = a;
+  b.f64 = 0;  // the imaging-condition product and the first row transform keep float32 arithmetic
+  return launch_mapped<KIcFwd, true>(b, st);
+}
 hipError_t launch_rows_scatter(bool born, const KArgs& a, hipStream_t st) {
-  return born ? launch_mapped<KScatter, true>(a, st) : launch_mapped<KScatter, false>(a, st);
+  return born ? launch_mapped<KScatter, true, true>(a, st) : launch_mapped<KScatter, false>(a, st);
 }
 hipError_t launch_rows_scatter_adj(bool fused, const KArgs& a, hipStream_t st) {
-  return fused ? launch_mapped<KScatterAdj, true>(a, st) : launch_mapped<KScatterAdj, false>(a, st);
+  return fused ? launch_mapped<KScatterAdj, true, true>(a, st) : launch_mapped<KScatterAdj, false>(a, st);
 }
 hipError_t launch_rows_adj_out(bool ic, const KArgs& a, hipStream_t st) {
-  return ic ? launch_mapped<KAdjOut, true>(a, st) : launch_mapped<KAdjOut, false>(a, st);
+  return ic ? launch_mapped<KAdjOut, true, true>(a, st) : launch_mapped<KAdjOut, false>(a, st);
 }
 hipError_t launch_refl_src(int mode, const ReflArgs& a, hipStream_t st) {
   const dim3 g = bflat((long long)a.nx * a.ny * a.NW);

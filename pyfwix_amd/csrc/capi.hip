@@ -102,22 +102,32 @@ static std::vector<float> k_table(int n, double d) {  // PhaseShift.h:43-57
   return k;
 }
 
-static std::vector<float> taper_table(int n, int t) {  // taper.cu:43-55
-  std::vector<float> w(n, 1.f);
+static std::vector<double> taper_table64(int n, int t) {  // taper.cu:43-55
+  std::vector<double> w(n, 1.0);
   if (t > 0)
     for (int i = 0; i < n; ++i) {
-      if (i < t) w[i] = (float)(0.5 * (1.0 - std::cos(M_PI * i / t)));
-      else if (i >= n - t) w[i] = (float)(0.5 * (1.0 + std::cos(M_PI * (i - n + t) / t)));
+      if (i < t) w[i] = 0.5 * (1.0 - std::cos(M_PI * i / t));
+      else if (i >= n - t) w[i] = 0.5 * (1.0 + std::cos(M_PI * (i - n + t) / t));
     }
   return w;
 }
+static std::vector<float> taper_table(int n, int t) {
+  const auto w64 = taper_table64(n, t);
+  return std::vector<float>(w64.begin(), w64.end());
+}
 
-static std::vector<float2> twiddle_table(int n) {  // exp(-2 pi i k / n) in double, rounded once
-  std::vector<float2> t(n);
+static std::vector<double2> twiddle_table64(int n) {  // exp(-2 pi i k / n) in double
+  std::vector<double2> t(n);
   for (int k = 0; k < n; ++k) {
     const double a = -2.0 * M_PI * (double)k / (double)n;
-    t[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    t[k] = make_double2(std::cos(a), std::sin(a));
   }
+  return t;
+}
+static std::vector<float2> twiddle_table(int n) {  // the float64 table rounded once
+  std::vector<float2> t(n);
+  const auto t64 = twiddle_table64(n);
+  for (int k = 0; k < n; ++k) t[k] = make_float2((float)t64[k].x, (float)t64[k].y);
   return t;
 }
 
@@ -152,6 +162,9 @@ void Tables::init(const fwix_dims& d, int tx, int ty) {
     w[i] = (float)f;
   }
   auto tx_ = taper_table(d.NX, tx), ty_ = taper_table(d.NY, ty);
+  auto tx64 = taper_table64(d.NX, tx), ty64 = taper_table64(d.NY, ty);
+  upload_bytes(b_tapx64, tx64.data(), tx64.size() * 8);
+  upload_bytes(b_tapy64, ty64.data(), ty64.size() * 8);
   upload_bytes(b_kx, kx.data(), kx.size() * 4);
   upload_bytes(b_ky, ky.data(), ky.size() * 4);
   upload_bytes(b_w2, w2.data(), w2.size() * 4);
@@ -162,6 +175,9 @@ void Tables::init(const fwix_dims& d, int tx, int ty) {
     auto twx = twiddle_table(d.NX), twy = twiddle_table(d.NY);
     upload_bytes(b_twx, twx.data(), twx.size() * 8);
     upload_bytes(b_twy, twy.data(), twy.size() * 8);
+    auto twx64 = twiddle_table64(d.NX), twy64 = twiddle_table64(d.NY);
+    upload_bytes(b_twx64, twx64.data(), twx64.size() * 16);
+    upload_bytes(b_twy64, twy64.data(), twy64.size() * 16);
   }
   dims = d;
 }
@@ -191,6 +207,10 @@ KArgs Tables::kargs() const {
   a.tapy = (const float*)b_tapy.p;
   a.twx = (const float2*)b_twx.p;
   a.twy = (const float2*)b_twy.p;
+  a.twx64 = (const double2*)b_twx64.p;
+  a.twy64 = (const double2*)b_twy64.p;
+  a.tapx64 = (const double*)b_tapx64.p;
+  a.tapy64 = (const double*)b_tapy64.p;
   a.NX = dims.NX; a.NY = dims.NY; a.NW = dims.NW; a.NS = dims.NS;
   a.nx = dims.nx; a.ny = dims.ny;
   a.dz = dims.dz;

@@ -69,6 +69,7 @@ struct DevBuf {
 struct Tables {
   fwix_dims dims{};
   DevBuf b_kx, b_ky, b_w2, b_w, b_tapx, b_tapy, b_twx, b_twy;
+  DevBuf b_twx64, b_twy64, b_tapx64, b_tapy64;  // float64 twiddles and taper weights
   void init(const fwix_dims& d, int tx, int ty);
   // per-plane frequencies in Hz (NW of them) in place of ow + iw * dw
   void set_frequencies(const float* f_hz, bool nonzero);

@@ -24,6 +24,9 @@
 namespace fwix {
 
 // ----------------------------------------------------------------- complex helpers
+// float2 (complex64) is the storage and default arithmetic type; the same
+// helpers exist for double2, so a transform can run in float64 arithmetic on
+// values loaded from and stored to complex64 arrays.
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -36,10 +39,38 @@ __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 __device__ __forceinline__ float2 czero() { return make_float2(0.f, 0.f); }
 
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {
+  return make_double2(fma(a.x, b.x, a.y * b.y), fma(a.y, b.x, -a.x * b.y));
+}
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+
+// the arithmetic type T's scalar and constructor; conversions from / to the
+// complex64 storage type
+template <class T> struct CX;
+template <> struct CX<float2> {
+  using S = float;
+  __device__ __forceinline__ static float2 mk(float x, float y) { return make_float2(x, y); }
+};
+template <> struct CX<double2> {
+  using S = double;
+  __device__ __forceinline__ static double2 mk(double x, double y) { return make_double2(x, y); }
+};
+template <class T> using real_of = typename CX<T>::S;
+template <class T> __device__ __forceinline__ T cz() { return CX<T>::mk(0, 0); }
+template <class T> __device__ __forceinline__ T from_f2(float2 a) { return CX<T>::mk(a.x, a.y); }
+__device__ __forceinline__ float2 to_f2(float2 a) { return a; }
+__device__ __forceinline__ float2 to_f2(double2 a) { return make_float2((float)a.x, (float)a.y); }
+
 // multiply by sgn*i, sgn = -1 for the forward (e^{-i}) transform, +1 for the inverse
-template <bool INV>
-__device__ __forceinline__ float2 mul_si(float2 a) {
-  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+template <bool INV, class T>
+__device__ __forceinline__ T mul_si(T a) {
+  return INV ? CX<T>::mk(-a.y, a.x) : CX<T>::mk(a.y, -a.x);
 }
 
 // ----------------------------------------------------------------- compile-time plan
@@ -85,27 +116,31 @@ struct Plan {
 template <int R, bool INV> struct Dft;
 
 template <bool INV> struct Dft<2, INV> {
-  __device__ __forceinline__ static void run(float2* x) {
-    float2 a = x[0], b = x[1];
+  template <class T>
+  __device__ __forceinline__ static void run(T* x) {
+    T a = x[0], b = x[1];
     x[0] = cadd(a, b); x[1] = csub(a, b);
   }
 };
 
 template <bool INV> struct Dft<4, INV> {
-  __device__ __forceinline__ static void run(float2* x) {
-    float2 a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
-    float2 c = cadd(x[1], x[3]), e = mul_si<INV>(csub(x[1], x[3]));
+  template <class T>
+  __device__ __forceinline__ static void run(T* x) {
+    T a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
+    T c = cadd(x[1], x[3]), e = mul_si<INV>(csub(x[1], x[3]));
     x[0] = cadd(a, c); x[2] = csub(a, c);
     x[1] = cadd(b, e); x[3] = csub(b, e);
   }
 };
 
 template <bool INV> struct Dft<3, INV> {
-  __device__ __forceinline__ static void run(float2* x) {
-    const float h = 0.86602540378443864676f;  // sin(2*pi/3)
-    float2 t1 = cadd(x[1], x[2]);
-    float2 t2 = mul_si<INV>(cscale(csub(x[1], x[2]), h));
-    float2 m = make_float2(x[0].x - 0.5f * t1.x, x[0].y - 0.5f * t1.y);
+  template <class T>
+  __device__ __forceinline__ static void run(T* x) {
+    using S = real_of<T>;
+    const S h = (S)0.86602540378443864676;  // sin(2*pi/3)
+    T t1 = cadd(x[1], x[2]);
+    T t2 = mul_si<INV>(cscale(csub(x[1], x[2]), h));
+    T m = CX<T>::mk(x[0].x - (S)0.5 * t1.x, x[0].y - (S)0.5 * t1.y);
     x[0] = cadd(x[0], t1);
     x[1] = cadd(m, t2);
     x[2] = csub(m, t2);
@@ -113,16 +148,18 @@ template <bool INV> struct Dft<3, INV> {
 };
 
 template <bool INV> struct Dft<8, INV> {
-  __device__ __forceinline__ static void run(float2* x) {
-    float2 e[4] = {x[0], x[2], x[4], x[6]};
-    float2 o[4] = {x[1], x[3], x[5], x[7]};
+  template <class T>
+  __device__ __forceinline__ static void run(T* x) {
+    using S = real_of<T>;
+    T e[4] = {x[0], x[2], x[4], x[6]};
+    T o[4] = {x[1], x[3], x[5], x[7]};
     Dft<4, INV>::run(e);
     Dft<4, INV>::run(o);
-    const float r = 0.70710678118654752440f;
+    const S r = (S)0.70710678118654752440;
     // W8^k = exp(sgn*2*pi*i*k/8), sgn = -1 forward
-    float2 o1 = cscale(cadd(o[1], mul_si<INV>(o[1])), r);           // (1 + sgn i)/sqrt2
-    float2 o2 = mul_si<INV>(o[2]);                                   // sgn i
-    float2 o3 = cscale(csub(mul_si<INV>(o[3]), o[3]), r);           // (-1 + sgn i)/sqrt2
+    T o1 = cscale(cadd(o[1], mul_si<INV>(o[1])), r);           // (1 + sgn i)/sqrt2
+    T o2 = mul_si<INV>(o[2]);                                   // sgn i
+    T o3 = cscale(csub(mul_si<INV>(o[3]), o[3]), r);           // (-1 + sgn i)/sqrt2
     x[0] = cadd(e[0], o[0]); x[4] = csub(e[0], o[0]);
     x[1] = cadd(e[1], o1);   x[5] = csub(e[1], o1);
     x[2] = cadd(e[2], o2);   x[6] = csub(e[2], o2);
@@ -158,19 +195,20 @@ __device__ __forceinline__ void edge_pos(int lane, int v, int& b, int& i) {
 }
 
 // ----------------------------------------------------------------- the transform
-// LDS layout policy: L::addr(b, i) gives the float2 offset of batch b, point i
-// inside the wave's region.
-template <int N, int NB, bool BFAST, bool INV, class L>
+// LDS layout policy: L::addr(b, i) gives the element offset of batch b, point i
+// inside the wave's region.  T is the arithmetic type of the values, the LDS
+// region and the twiddle table (float2, or double2 for float64 arithmetic).
+template <int N, int NB, bool BFAST, bool INV, class L, class T = float2>
 struct WaveFft {
   static constexpr int V = NB * N / 64;
   using P = Plan<N, V, INV>;
   static constexpr int S = P::S;
 
   template <int R, int Ns>
-  __device__ __forceinline__ static void butterflies(float2* v, const float2* __restrict__ tw, int lane) {
+  __device__ __forceinline__ static void butterflies(T* v, const T* __restrict__ tw, int lane) {
 #pragma unroll
     for (int q = 0; q < V / R; ++q) {
-      float2* x = v + q * R;
+      T* x = v + q * R;
       if constexpr (Ns > 1) {
         int b, j;
         bj<N, NB, R, BFAST>(lane + 64 * q, b, j);
@@ -178,7 +216,7 @@ struct WaveFft {
         constexpr int step = N / (Ns * R);
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          float2 w = tw[r * k * step];
+          T w = tw[r * k * step];
           if (INV) w.y = -w.y;
           x[r] = cmul(x[r], w);
         }
@@ -188,7 +226,7 @@ struct WaveFft {
   }
 
   template <int R, int Ns, int R2>
-  __device__ __forceinline__ static void exchange(float2* v, float2* lds, int lane) {
+  __device__ __forceinline__ static void exchange(T* v, T* lds, int lane) {
 #pragma unroll
     for (int q = 0; q < V / R; ++q) {
       int b, j;
@@ -209,7 +247,7 @@ struct WaveFft {
   }
 
   template <int s>
-  __device__ __forceinline__ static void stages(float2* v, float2* lds, const float2* __restrict__ tw, int lane) {
+  __device__ __forceinline__ static void stages(T* v, T* lds, const T* __restrict__ tw, int lane) {
     if constexpr (s < S) {
       constexpr int R = P::radix(s);
       constexpr int Ns = P::ns(s);
@@ -224,7 +262,7 @@ struct WaveFft {
   // v holds the values at first-stage input positions (edge_pos<..., P::first()>);
   // on return it holds the spectrum at last-stage output positions
   // (edge_pos<..., P::last()>).  Unnormalized.
-  __device__ __forceinline__ static void run(float2* v, float2* lds, const float2* __restrict__ tw, int lane) {
+  __device__ __forceinline__ static void run(T* v, T* lds, const T* __restrict__ tw, int lane) {
     // Keep the twiddles and the per-element twiddle / LDS addresses next to
     // their use: without these fences the compiler hoists every stage's
     // (loop-invariant) loads and addresses out of the per-reference loops

@@ -62,6 +62,10 @@ struct KArgs {
   const float* tapy;       // (NY)
   const float2* twx;       // (NX) twiddles exp(-2 pi i k/NX)
   const float2* twy;       // (NY)
+  const double2* twx64;    // (NX), (NY) twiddles and taper weights in float64, for
+  const double2* twy64;    //   the passes that run in float64 arithmetic (f64)
+  const double* tapx64;
+  const double* tapy64;
   const float2* slow;      // (NW, ny, nx) unpadded slowness slice (Born scatter)
   const float2* dslow;     // (NW, ny, nx) model perturbation slice (Born IC)
   const float2* bg;        // background wavefield slice, chunk offset (Born IC)
@@ -74,7 +78,14 @@ struct KArgs {
   float dz, eps, eps_s;
   float scale;             // normalisation applied by the kernel (set per launch)
   int add;
+  int f64;                 // the launch runs its transforms in float64 arithmetic
 };
+
+// twiddle tables of the arithmetic type T (float2 or double2)
+__device__ __forceinline__ const float2* twx_of(const KArgs& a, float2*) { return a.twx; }
+__device__ __forceinline__ const double2* twx_of(const KArgs& a, double2*) { return a.twx64; }
+__device__ __forceinline__ const float2* twy_of(const KArgs& a, float2*) { return a.twy; }
+__device__ __forceinline__ const double2* twy_of(const KArgs& a, double2*) { return a.twy64; }
 
 __device__ __forceinline__ int plane_iw(const KArgs& a, int p_local) { return (a.p0 + p_local) % a.NW; }
 // Index into the model-sized arrays (labels, presence masks, slowness, model
@@ -141,6 +152,33 @@ __device__ __forceinline__ float2 ps_general(float a, float b, float dz) {
   fast_sincos(kr * dz, &sn, &cs);
   return MODE == 0 ? make_float2(att * cs, -att * sn) : make_float2(att * cs, att * sn);
 }
+// The same multipliers in float64 arithmetic (the float64 scattering passes:
+// near the evanescent boundary a = w2 Re s - k^2 cancels, and a float32
+// phase there is the largest error left once the transforms run in float64).
+template <int MODE>
+__device__ __forceinline__ double2 ps_real64(double a, double dz) {
+  const double t = sqrt(fabs(a)) * dz;
+  if (a >= 0.0) {
+    double sn, cs;
+    sincos(t, &sn, &cs);
+    return MODE == 0 ? make_double2(cs, -sn) : make_double2(cs, sn);
+  }
+  return make_double2(exp(MODE == 2 ? t : -t), 0.0);
+}
+template <int MODE>
+__device__ __forceinline__ double2 ps_general64(double a, double b, double dz) {
+  const double c = sqrt(fma(a, a, b * b));
+  const double big = sqrt(0.5 * (c + fabs(a)));
+  const double small = 0.5 * fabs(b) / big;
+  double kr = a >= 0.0 ? big : small;
+  const double ki = a >= 0.0 ? small : big;
+  if (MODE != 0 && b > 0.0) kr = -kr;
+  const double att = exp((MODE == 2 ? ki : -ki) * dz);
+  double sn, cs;
+  sincos(kr * dz, &sn, &cs);
+  return MODE == 0 ? make_double2(att * cs, -att * sn) : make_double2(att * cs, att * sn);
+}
+
 template <int MODE>
 __device__ __forceinline__ float2 ps_mult(float w2, float sre, float sim, float k2, float dz, float eps) {
   const float a = fmaf(w2, sre, -k2);

@@ -111,33 +111,35 @@ __global__ __launch_bounds__(kThreads) void k_rows(KArgs a) {
 }
 
 // rows_fanout: W_r = scale * F_x([labels == r] * in) for every reference r
-template <int N, bool MAP>
+// (T: arithmetic type of the transforms, float2 or double2)
+template <int N, bool MAP, class T = float2>
 __global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
   using C = RowCfg<N>;
-  using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
-  __shared__ float2 lds[kWaves * C::LDSW];
+  using F = WaveFft<N, C::NB, false, false, PadLds<N>, T>;
+  __shared__ T lds[kWaves * C::LDSW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nrows = a.np * a.NY;
   const int rb = (blockIdx.x * kWaves + wave) * C::NB;
   // twiddles in LDS: inside the term loops a global load would wait (one
   // counter for all vector memory operations) for the term stores / the next
   // term's loads still in flight
-  __shared__ float2 twl[N];
-  for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = a.twx[i];
+  __shared__ T twl[N];
+  const T* twg = twx_of(a, (T*)nullptr);
+  for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = twg[i];
   __syncthreads();
   if (rb >= nrows) return;
-  float2* wl = lds + wave * C::LDSW;
-  float2 u[C::V];
+  T* wl = lds + wave * C::LDSW;
+  T u[C::V];
   uint8_t lab[C::V];
 #pragma unroll
   for (int e = 0; e < C::V; ++e) {
     int b, i;
     edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
     const int lr = rb + b;
-    u[e] = czero();
+    u[e] = cz<T>();
     lab[e] = 255;
     if (lr < nrows) {
-      u[e] = a.in[(unsigned)(lr * N + i)];
+      u[e] = from_f2<T>(a.in[(unsigned)(lr * N + i)]);
       const int p = lr / a.NY, iy = lr - p * a.NY;
       lab[e] = a.labels[(unsigned)((model_iw<MAP>(a, plane_iw(a, p)) * a.NY + iy) * N + i)];
     }
@@ -157,9 +159,9 @@ __global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
   }
   for (int r = 0; r < a.nref; ++r) {
     if (r < 32 && !((wm >> r) & 1u)) continue;  // rows without reference r: the column pass reads zeros
-    float2 x[C::V];
+    T x[C::V];
 #pragma unroll
-    for (int e = 0; e < C::V; ++e) x[e] = (lab[e] == r) ? u[e] : czero();
+    for (int e = 0; e < C::V; ++e) x[e] = (lab[e] == r) ? u[e] : cz<T>();
     F::run(x, wl, twl, lane);
     float2* W = a.W + (size_t)r * a.wstride;
 #pragma unroll
@@ -167,7 +169,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
       int b, i;
       edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
       const int lr = rb + b;
-      if (lr < nrows) W[(unsigned)(lr * N + i)] = cscale(x[e], s);
+      if (lr < nrows) W[(unsigned)(lr * N + i)] = to_f2(cscale(x[e], s));
     }
   }
 }
@@ -271,6 +273,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
 template <int MULT>
 struct TermMult {
   float w2, wsre, b, iw2, dz;
+  double wsre64, b64;  // the same in float64 (the float64 passes)
   int r;
   bool real;
   __device__ __forceinline__ TermMult(const KArgs& a, int r_, int iw, float w2_)
@@ -280,11 +283,27 @@ struct TermMult {
     if constexpr (MULT == 2) {
       iw2 = 1.f / w2;
       real = true; wsre = 0.f; b = 0.f;
+      wsre64 = 0.0; b64 = 0.0;
     } else {
       wsre = w2 * sr.x;
       b = w2 * (sr.y - a.eps * sr.x);
       real = b == 0.f;
       iw2 = 0.f;
+      wsre64 = (double)w2 * sr.x;
+      b64 = (double)w2 * ((double)sr.y - (double)a.eps * sr.x);
+    }
+  }
+  template <bool REAL>
+  __device__ __forceinline__ double2 at(double k2) const {
+    if constexpr (MULT == 2) {
+      const double q = k2 / (double)w2;
+      double m = q;
+      for (int i = 0; i < r; ++i) m *= q;
+      return make_double2(m, 0.0);
+    } else if constexpr (REAL) {
+      return ps_real64<MULT>(wsre64 - k2, (double)dz);
+    } else {
+      return ps_general64<MULT>(wsre64 - k2, b64, (double)dz);
     }
   }
   template <bool REAL>
@@ -304,16 +323,23 @@ struct TermMult {
 
 // k^2 = kx^2 + ky^2 at this wave's values after a column transform
 // (positions of the last stage of radix R); loop-invariant over the terms.
-template <int N, class C, int R>
-__device__ __forceinline__ void column_k2(float* k2v, const KArgs& a, int colw, int lane) {
+template <int N, class C, int R, class K2>
+__device__ __forceinline__ void column_k2(K2* k2v, const KArgs& a, int colw, int lane) {
 #pragma unroll
   for (int e = 0; e < C::V; ++e) {
     int b, i;
     edge_pos<N, C::NB, R, false>(lane, e, b, i);
     const int col = colw + b;
     const float kxv = a.kx[col < a.NX ? col : 0], kyv = a.ky[i];
-    k2v[e] = fmaf(kxv, kxv, kyv * kyv);
+    if constexpr (sizeof(K2) == 8) k2v[e] = fma((double)kxv, (double)kxv, (double)kyv * kyv);
+    else k2v[e] = fmaf(kxv, kxv, kyv * kyv);
   }
+}
+// multiplier of term tm at k^2 in the arithmetic type T
+template <class T, bool REAL, class TM, class K2>
+__device__ __forceinline__ T term_at(const TM& tm, K2 k2) {
+  if constexpr (sizeof(T) == 16) return tm.template at<REAL>((double)k2);
+  else return tm.template at<REAL>((float)k2);
 }
 
 // Coalesced tile load / store: every thread first issues all of its global
@@ -418,23 +444,32 @@ __device__ __forceinline__ void tile_store(const float2* tile, const PlaneBuf& b
   for (int k = 0; k < PER; ++k) b.store(t[k], o.at(k));
 }
 // this wave's columns at the first-stage positions of radix R, from the tile
-template <int N, class C, int R>
-__device__ __forceinline__ void tile_read(float2* v, const float2* tile, int wave, int lane) {
+template <int N, class C, int R, class T>
+__device__ __forceinline__ void tile_read(T* v, const float2* tile, int wave, int lane) {
 #pragma unroll
   for (int e = 0; e < C::V; ++e) {
     int b, i;
     edge_pos<N, C::NB, R, false>(lane, e, b, i);
-    v[e] = tile[i * C::CWP + wave * C::NB + b];
+    v[e] = from_f2<T>(tile[i * C::CWP + wave * C::NB + b]);
   }
 }
-template <int N, class C, int R>
-__device__ __forceinline__ void tile_write(float2* tile, const float2* v, int wave, int lane) {
+template <int N, class C, int R, class T>
+__device__ __forceinline__ void tile_write(float2* tile, const T* v, int wave, int lane) {
 #pragma unroll
   for (int e = 0; e < C::V; ++e) {
     int b, i;
     edge_pos<N, C::NB, R, false>(lane, e, b, i);
-    tile[i * C::CWP + wave * C::NB + b] = v[e];
+    tile[i * C::CWP + wave * C::NB + b] = to_f2(v[e]);
   }
+}
+
+// LDS of a column workgroup in float2 units: the complex64 tile, reused
+// between barriers as the waves' private exchange regions of type T
+template <int N, class T>
+constexpr int col_smem() {
+  using C = ColCfg<N>;
+  constexpr int priv = C::WAVES * C::PRIV * (int)(sizeof(T) / sizeof(float2));
+  return C::TILE > priv ? C::TILE : priv;
 }
 
 // out = (add ? out : 0) + scale * F_y(in) on each plane (plain 2-D FFT column pass)
@@ -467,11 +502,11 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
 // Whether the twiddle table (N values) and the reference slownesses (up to
 // 256) fit in LDS beside the tile and the row masks without lowering the
 // workgroups a CU holds (at most WPE / 2 by registers; 160 KB of LDS).
-template <int N, int WPE>
+template <int N, int WPE, class T = float2>
 constexpr bool col_stage_lds() {
   using C = ColCfg<N>;
-  constexpr int base = C::SMEM * 8 + C::NROW * 4;
-  constexpr int extra = N * 8 + 256 * 8;
+  constexpr int base = col_smem<N, T>() * 8 + C::NROW * 4;
+  constexpr int extra = N * (int)sizeof(T) + 256 * 8;
   constexpr int by_regs = WPE / 2;
   constexpr int without = (160 * 1024) / base < by_regs ? (160 * 1024) / base : by_regs;
   constexpr int with = (160 * 1024) / (base + extra) < by_regs ? (160 * 1024) / (base + extra) : by_regs;
@@ -479,32 +514,33 @@ constexpr bool col_stage_lds() {
 }
 
 // cols_fanout: W_r = F_y^-1[M_r * F_y(K)] / NY for every term r (M_r: term_mult<MODE>).
-template <int N, int MODE, int WPE>
+// (T: arithmetic type of the transforms, float2 or double2)
+template <int N, int MODE, int WPE, class T = float2>
 __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_cols_fanout(KArgs a) {
   using C = ColCfg<N>;
-  using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
-  using Fi = WaveFft<N, C::NB, false, true, PadLds<N>>;
+  using F = WaveFft<N, C::NB, false, false, PadLds<N>, T>;
+  using Fi = WaveFft<N, C::NB, false, true, PadLds<N>, T>;
   static_assert(F::P::last() == Fi::P::first(), "register hand-off needs mirrored plans");
-  __shared__ float2 smem[C::SMEM];
+  __shared__ __align__(16) float2 smem[col_smem<N, T>()];
   // The loop below stores each term's tile and then starts the next term; on
   // this ISA one wait counter covers loads and stores, so any global load
   // inside the loop (twiddles, reference slownesses) would also wait for the
   // previous term's stores to reach memory.  Both are staged in LDS first
   // where that costs no occupancy (STAGE).
-  constexpr bool STAGE = col_stage_lds<N, WPE>();
-  __shared__ float2 twl[STAGE ? N : 1];
+  constexpr bool STAGE = col_stage_lds<N, WPE, T>();
+  __shared__ T twl[STAGE ? N : 1];
   __shared__ float2 srl[STAGE ? 256 : 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int p, c0;
   col_tile<C>(p, c0);
   const size_t pb = (size_t)p * a.NX * N;
-  float2* priv = smem + wave * C::PRIV;
+  T* priv = (T*)smem + wave * C::PRIV;
   const TileOffs<C> to(a.NX, c0);
   tile_load<C>(smem, PlaneBuf(a.K + pb, plane_bytes(a)), to);
   const int iw = plane_iw(a, p);
-  const float2* tw = a.twy;
+  const T* tw = twy_of(a, (T*)nullptr);
   if constexpr (STAGE) {
-    for (int i = threadIdx.x; i < N; i += C::THREADS) twl[i] = a.twy[i];
+    for (int i = threadIdx.x; i < N; i += C::THREADS) twl[i] = tw[i];
     if constexpr (MODE != 2) {
       const int nt = a.nterm < 256 ? a.nterm : 256;
       for (int i = threadIdx.x; i < nt; i += C::THREADS) srl[i] = a.sref[i * a.NW + iw];
@@ -512,7 +548,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
     tw = twl;
   }
   __syncthreads();
-  float2 spec[C::V];
+  T spec[C::V];
   tile_read<N, C, F::P::first()>(spec, smem, wave, lane);
   __syncthreads();
   F::run(spec, priv, tw, lane);
@@ -524,17 +560,18 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   const uint32_t pm = (MODE != 2 && a.planemask) ? a.planemask[im] : 0xffffffffu;
   __shared__ uint32_t rmk[C::NROW];
   tile_row_masks<C>(rmk, rm);
-  float k2v[C::V];
+  real_of<T> k2v[C::V];
   column_k2<N, C, F::P::last()>(k2v, a, colw, lane);
   for (int r = 0; r < a.nterm; ++r) {
     if (r < 32 && !((pm >> r) & 1u)) continue;  // reference absent from the plane: W_r is never read
     float2 srv = make_float2(0.f, 0.f);  // nref <= 255
     if constexpr (MODE != 2) srv = STAGE ? srl[r] : a.sref[r * a.NW + iw];
     const TermMult<MODE> tm(a, r, w2, srv);
-    float2 x[C::V];
+    T x[C::V];
     auto mult = [&](auto real) {
 #pragma unroll
-      for (int e = 0; e < C::V; ++e) x[e] = cscale(cmul(spec[e], tm.template at<decltype(real)::value>(k2v[e])), s2);
+      for (int e = 0; e < C::V; ++e)
+        x[e] = cscale(cmul(spec[e], term_at<T, decltype(real)::value>(tm, k2v[e])), s2);
     };
     if (tm.real) mult(std::true_type{});
     else mult(std::false_type{});
@@ -550,25 +587,26 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
 }
 
 // cols_fanin: K = F_y^-1[sum_r M_r * F_y(W_r)] / NY (M_r: term_mult<MODE>).
-template <int N, int MODE, int WPE>
+// (T: arithmetic type of the transforms, float2 or double2)
+template <int N, int MODE, int WPE, class T = float2>
 __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_cols_fanin(KArgs a) {
   using C = ColCfg<N>;
-  using F = WaveFft<N, C::NB, false, false, PadLds<N>>;
-  using Fi = WaveFft<N, C::NB, false, true, PadLds<N>>;
+  using F = WaveFft<N, C::NB, false, false, PadLds<N>, T>;
+  using Fi = WaveFft<N, C::NB, false, true, PadLds<N>, T>;
   static_assert(F::P::last() == Fi::P::first(), "register hand-off needs mirrored plans");
-  __shared__ float2 smem[C::SMEM];
+  __shared__ __align__(16) float2 smem[col_smem<N, T>()];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int p, c0;
   col_tile<C>(p, c0);
   const size_t pb = (size_t)p * a.NX * N;
-  float2* priv = smem + wave * C::PRIV;
+  T* priv = (T*)smem + wave * C::PRIV;
   const TileOffs<C> to(a.NX, c0);
   const int iw = plane_iw(a, p);
   const float w2 = a.w2[iw];
   const int colw = c0 + wave * C::NB;
-  float2 acc[C::V];
+  T acc[C::V];
 #pragma unroll
-  for (int e = 0; e < C::V; ++e) acc[e] = czero();
+  for (int e = 0; e < C::V; ++e) acc[e] = cz<T>();
   const int im = model_iw_any(a, iw);
   const uint32_t* rm = (MODE != 2 && a.rowmask) ? a.rowmask + im * a.NY : nullptr;
   const uint32_t pm = (MODE != 2 && a.planemask) ? a.planemask[im] : 0xffffffffu;
@@ -580,13 +618,13 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   // (STAGE): a global load in the loop would wait (one counter for all vector
   // memory operations) for the next term's tile, which is fetched into
   // registers while this term is transformed
-  constexpr bool STAGE = col_stage_lds<N, WPE>();
-  __shared__ float2 twl[STAGE ? N : 1];
+  constexpr bool STAGE = col_stage_lds<N, WPE, T>();
+  __shared__ T twl[STAGE ? N : 1];
   __shared__ float2 srl[STAGE ? 256 : 1];
   tile_row_masks<C>(rmk, rm);
-  const float2* tw = a.twy;
+  const T* tw = twy_of(a, (T*)nullptr);
   if constexpr (STAGE) {
-    for (int i = threadIdx.x; i < N; i += C::THREADS) twl[i] = a.twy[i];
+    for (int i = threadIdx.x; i < N; i += C::THREADS) twl[i] = tw[i];
     if constexpr (MODE != 2) {
       const int nt = a.nterm < 256 ? a.nterm : 256;
       for (int i = threadIdx.x; i < nt; i += C::THREADS) srl[i] = a.sref[i * a.NW + iw];
@@ -594,7 +632,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
     tw = twl;
   }
   __syncthreads();
-  float k2v[C::V];
+  real_of<T> k2v[C::V];
   column_k2<N, C, F::P::last()>(k2v, a, colw, lane);
   auto fetch = [&](TileRegs<C>& t, int rr) {
     const PlaneBuf wb(a.W + (size_t)rr * a.wstride + pb, plane_bytes(a));
@@ -604,7 +642,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   // one term: transform its tile (already in LDS), multiply, accumulate;
   // after_read runs once the tile is in registers
   auto term = [&](int rc, auto&& after_read) {
-    float2 v[C::V];
+    T v[C::V];
     tile_read<N, C, F::P::first()>(v, smem, wave, lane);
     after_read();
     __syncthreads();
@@ -614,7 +652,8 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
     const TermMult<MODE> tm(a, rc, w2, srv);
     auto mult = [&](auto real) {
 #pragma unroll
-      for (int e = 0; e < C::V; ++e) acc[e] = cadd(acc[e], cmul(v[e], tm.template at<decltype(real)::value>(k2v[e])));
+      for (int e = 0; e < C::V; ++e)
+        acc[e] = cadd(acc[e], cmul(v[e], term_at<T, decltype(real)::value>(tm, k2v[e])));
     };
     if (tm.real) mult(std::true_type{});
     else mult(std::false_type{});
@@ -761,8 +800,14 @@ hipError_t launch_rows(int kind, const KArgs& a, hipStream_t st) {
 hipError_t launch_rows_fanout(const KArgs& a, hipStream_t st) {
   return dispatch_n(a.NX, [&](auto nc) -> hipError_t {
     constexpr int N = decltype(nc)::value;
-    if (a.mplane) hipLaunchKernelGGL((k_rows_fanout<N, true>), row_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
-    else hipLaunchKernelGGL((k_rows_fanout<N, false>), row_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    const dim3 g = row_grid(N, a.np * a.NY);
+    if (a.f64) {
+      if (a.mplane) hipLaunchKernelGGL((k_rows_fanout<N, true, double2>), g, dim3(kThreads), 0, st, a);
+      else hipLaunchKernelGGL((k_rows_fanout<N, false, double2>), g, dim3(kThreads), 0, st, a);
+    } else {
+      if (a.mplane) hipLaunchKernelGGL((k_rows_fanout<N, true>), g, dim3(kThreads), 0, st, a);
+      else hipLaunchKernelGGL((k_rows_fanout<N, false>), g, dim3(kThreads), 0, st, a);
+    }
     return hipGetLastError();
   });
 }
@@ -834,10 +879,17 @@ static hipError_t dispatch_wpe(int wpe, Fn&& fn) {
   }
 }
 
+// float64 arithmetic (a.f64): one register budget, 2 waves per SIMD (256 VGPRs)
 hipError_t launch_cols_fanout(int mode, const KArgs& a, hipStream_t st) {
   return dispatch_n(a.NY, [&](auto nc) -> hipError_t {
     constexpr int N = decltype(nc)::value;
     const dim3 g = col_grid(N, a.NX, a.np);
+    if (a.f64) {
+      if (mode == 0) hipLaunchKernelGGL((k_cols_fanout<N, 0, 2, double2>), g, dim3(kColThreads), 0, st, a);
+      else if (mode == 1) hipLaunchKernelGGL((k_cols_fanout<N, 1, 2, double2>), g, dim3(kColThreads), 0, st, a);
+      else hipLaunchKernelGGL((k_cols_fanout<N, 2, 2, double2>), g, dim3(kColThreads), 0, st, a);
+      return hipGetLastError();
+    }
     return dispatch_wpe(col_wpe_get(0, mode, N), [&](auto wc) -> hipError_t {
       constexpr int W = decltype(wc)::value;
       if (mode == 0) hipLaunchKernelGGL((k_cols_fanout<N, 0, W>), g, dim3(kColThreads), 0, st, a);
@@ -852,6 +904,12 @@ hipError_t launch_cols_fanin(int mode, const KArgs& a, hipStream_t st) {
   return dispatch_n(a.NY, [&](auto nc) -> hipError_t {
     constexpr int N = decltype(nc)::value;
     const dim3 g = col_grid(N, a.NX, a.np);
+    if (a.f64) {
+      if (mode == 0) hipLaunchKernelGGL((k_cols_fanin<N, 0, 2, double2>), g, dim3(kColThreads), 0, st, a);
+      else if (mode == 1) hipLaunchKernelGGL((k_cols_fanin<N, 1, 2, double2>), g, dim3(kColThreads), 0, st, a);
+      else hipLaunchKernelGGL((k_cols_fanin<N, 2, 2, double2>), g, dim3(kColThreads), 0, st, a);
+      return hipGetLastError();
+    }
     return dispatch_wpe(col_wpe_get(1, mode, N), [&](auto wc) -> hipError_t {
       constexpr int W = decltype(wc)::value;
       if (mode == 0) hipLaunchKernelGGL((k_cols_fanin<N, 0, W>), g, dim3(kColThreads), 0, st, a);

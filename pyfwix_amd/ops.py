@@ -14,11 +14,13 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import FWIX_NSPS, FWIX_PSPI, FWIX_PSPI_PLAIN, Handle, call, create, default_params, make_dims
+from ._lib import (FWIX_NSPS, FWIX_PREC_F32, FWIX_PREC_F64_SCATTER, FWIX_PSPI, FWIX_PSPI_PLAIN, Handle, call, create,
+                   default_params, make_dims)
 
 __all__ = ["FFT2d", "PhaseShift", "Selector", "Taper", "OneStep", "Injection", "Scatter",
            "ImagingCondition", "OneWay", "Born", "reflect_coef", "bfp16_pack", "bfp16_unpack", "ptr", "stream",
-           "make_dims", "default_params", "FWIX_PSPI", "FWIX_NSPS", "FWIX_PSPI_PLAIN"]
+           "make_dims", "default_params", "FWIX_PSPI", "FWIX_NSPS", "FWIX_PSPI_PLAIN", "FWIX_PREC_F32",
+           "FWIX_PREC_F64_SCATTER"]
 
 
 def ptr(t: torch.Tensor | None) -> C.c_void_p:
@@ -431,6 +433,12 @@ class Born:
     def set_codec(self, codec: int) -> None:
         """Checkpoint codec: 0 lossless, 1 16-bit block floating point (set before model())."""
         call("fwix_born_set_codec", self._h.h, int(codec))
+
+    def set_precision(self, precision: int) -> None:
+        """FWIX_PREC_F32 (default) or FWIX_PREC_F64_SCATTER: the Born steps' scattered-field
+        transforms in float64 arithmetic (complex64 storage), for low frequencies on fine
+        grids where the first Taylor term's k^2 / (w^2 s) weight amplifies float32 rounding."""
+        call("fwix_born_set_precision", self._h.h, int(precision))
 
     def set_density(self, den):
         d = self.dims
