@@ -181,6 +181,12 @@ class ShotBatch:
                            ckpt_every=int(ckpt_every), planes_per_chunk=int(planes_per_chunk))
         self.op.set_shot_planes(self.counts)
         self.op.set_frequencies(freqs)
+        self.freqs = freqs
+        self._set_geometry()
+        self.slow = self.dslow = self.grad = None
+
+    def _set_geometry(self) -> None:
+        import torch
         ids = lambda attr: np.concatenate([np.full(len(getattr(s, attr)), k, np.int32)  # noqa: E731
                                            for k, s in enumerate(self.shots)])
         src = np.concatenate([np.asarray(s.src, np.float32).reshape(-1, 3) for s in self.shots])
@@ -194,7 +200,20 @@ class ShotBatch:
         nrec = np.array([len(s.rec) for s in self.shots], np.int64)
         self.rec_off = np.concatenate([[0], np.cumsum(nrec * self.counts)[:-1]])
         self.nsamp = int((nrec * self.counts).sum())
-        self.slow = self.dslow = self.grad = None
+
+    def set_shots(self, shots) -> None:
+        """Rebind the batch to other shots of the same layout (aperture, frequencies
+        and plane count per shot): new sources, receivers and wavelets; slowness,
+        references and scratch are kept.  The background must be modeled again."""
+        shots = list(shots)
+        if len(shots) != len(self.shots):
+            raise ValueError(f"expected {len(self.shots)} shots, got {len(shots)}")
+        for a, b in zip(shots, self.shots):
+            if (a.nx, a.ny) != (b.nx, b.ny) or a.nw != b.nw or not np.array_equal(
+                    np.asarray(a.freqs, np.float32), np.asarray(b.freqs, np.float32)):
+                raise ValueError("set_shots keeps the batch layout: same apertures and frequencies per shot")
+        self.shots = shots
+        self._set_geometry()
 
     # -------------------------------------------------------------- model inputs
     def _embed(self, arrs, pad: str):
