@@ -26,7 +26,13 @@ constexpr int col_wpe_default(int kind, int mode, int N) {
   if (kind == 0) return (pow2 && N <= 512) ? 6 : (N == 384 || N == 768) ? 4 : 2;
   return (pow2 && N <= 512) || N == 768 ? 4 : 2;
 }
+// 1024-point columns use 256-thread workgroups (4 waves) on 4-column tiles:
+// a 1024 x 8 tile takes 74 KB of LDS and, with the 256-VGPR budget these
+// transforms need, one 8-wave workgroup per CU, so nothing overlapped one
+// tile's loads with another's transforms; two 4-wave workgroups share a CU.
+constexpr int col_threads(int N) { return N == 1024 ? 256 : 512; }
 constexpr int col_cw(int N) {
+  if (N == 1024) return 4;
   const int cw = (size_has3(N) ? 6144 : 4096) / N;
   return cw < 8 ? 8 : cw;
 }

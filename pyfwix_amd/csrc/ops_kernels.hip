@@ -1,7 +1,10 @@
 // Plain per-operator kernels: the reference's operator granularity
 // (PhaseShift, Selector, Taper, add/scale), used by the standalone operator
 // entry points and by the plain per-shot comparator path.  One thread per
-// complex sample, 64-bit flat indices, x fastest.
+// complex sample, x fastest; the point-wise kernels run on a (plane samples,
+// frequency, shot) grid so a thread finds its (x, y) with one 32-bit
+// division (a plane holds at most 1024 x 1024 samples) and its frequency
+// from the block index.
 #include "fwix_kernels.h"
 #include "fwix_launch.h"
 
@@ -13,44 +16,48 @@ static inline dim3 flat_grid(size_t n) {
   return dim3((unsigned)g);
 }
 
+// grid (plane samples / kThreads, NW, NS): j = sample within the plane
+static inline dim3 plane_grid(const PwArgs& a) {
+  return dim3((unsigned)(((size_t)a.NX * a.NY + kThreads - 1) / kThreads), (unsigned)a.NW, (unsigned)a.NS);
+}
+__device__ __forceinline__ size_t plane_base(const PwArgs& a) {
+  return ((size_t)blockIdx.z * a.NW + blockIdx.y) * ((size_t)a.NX * a.NY);
+}
+
 // d = (add ? d : 0) + Phi * m  (propagator/src/phase_shift.cu:7-61,125-184,248-305)
 template <int MODE>
 __global__ __launch_bounds__(kThreads) void k_phase_shift(const float2* __restrict__ m, float2* d, PwArgs a) {
-  const size_t n = (size_t)a.NX * a.NY * a.NW * a.NS;
-  for (size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x; idx < n; idx += (size_t)gridDim.x * kThreads) {
-    const int ix = (int)(idx % a.NX);
-    const size_t t = idx / a.NX;
-    const int iy = (int)(t % a.NY);
-    const int iw = (int)((t / a.NY) % a.NW);
-    const float2 s = a.sref[iw];
-    const float k2 = fmaf(a.kx[ix], a.kx[ix], a.ky[iy] * a.ky[iy]);
-    float2 y = cmul(m[idx], ps_mult<MODE>(a.w2[iw], s.x, s.y, k2, a.dz, a.eps));
-    if (a.add) y = cadd(y, d[idx]);
-    d[idx] = y;
-  }
+  const unsigned j = blockIdx.x * kThreads + threadIdx.x;
+  if (j >= (unsigned)(a.NX * a.NY)) return;
+  const unsigned iy = j / (unsigned)a.NX, ix = j - iy * (unsigned)a.NX;
+  const int iw = blockIdx.y;
+  const size_t idx = plane_base(a) + j;
+  const float2 s = a.sref[iw];
+  const float k2 = fmaf(a.kx[ix], a.kx[ix], a.ky[iy] * a.ky[iy]);
+  float2 y = cmul(m[idx], ps_mult<MODE>(a.w2[iw], s.x, s.y, k2, a.dz, a.eps));
+  if (a.add) y = cadd(y, d[idx]);
+  d[idx] = y;
 }
 
 // d = (add ? d : 0) + [labels == value] * m  (propagator/src/selector.cu:7-37)
 __global__ __launch_bounds__(kThreads) void k_select(const float2* __restrict__ m, float2* d, PwArgs a) {
-  const size_t n = (size_t)a.NX * a.NY * a.NW * a.NS;
-  const size_t plane3 = (size_t)a.NX * a.NY * a.NW;
-  for (size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x; idx < n; idx += (size_t)gridDim.x * kThreads) {
-    float2 y = a.labels[idx % plane3] == a.value ? m[idx] : czero();
-    if (a.add) y = cadd(y, d[idx]);
-    d[idx] = y;
-  }
+  const unsigned j = blockIdx.x * kThreads + threadIdx.x;
+  if (j >= (unsigned)(a.NX * a.NY)) return;
+  const size_t idx = plane_base(a) + j;
+  float2 y = a.labels[(size_t)blockIdx.y * a.NX * a.NY + j] == a.value ? m[idx] : czero();
+  if (a.add) y = cadd(y, d[idx]);
+  d[idx] = y;
 }
 
 // d = (add ? d : 0) + wx(ix) wy(iy) m  (propagator/src/taper.cu:7-77)
 __global__ __launch_bounds__(kThreads) void k_taper(const float2* __restrict__ m, float2* d, PwArgs a) {
-  const size_t n = (size_t)a.NX * a.NY * a.NW * a.NS;
-  for (size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x; idx < n; idx += (size_t)gridDim.x * kThreads) {
-    const int ix = (int)(idx % a.NX);
-    const int iy = (int)((idx / a.NX) % a.NY);
-    float2 y = cscale(m[idx], a.tapx[ix] * a.tapy[iy]);
-    if (a.add) y = cadd(y, d[idx]);
-    d[idx] = y;
-  }
+  const unsigned j = blockIdx.x * kThreads + threadIdx.x;
+  if (j >= (unsigned)(a.NX * a.NY)) return;
+  const unsigned iy = j / (unsigned)a.NX, ix = j - iy * (unsigned)a.NX;
+  const size_t idx = plane_base(a) + j;
+  float2 y = cscale(m[idx], a.tapx[ix] * a.tapy[iy]);
+  if (a.add) y = cadd(y, d[idx]);
+  d[idx] = y;
 }
 
 // d = (add ? d : 0) + alpha * m  (operator/src/complex_vector.cu:4-29 add / scale)
@@ -127,18 +134,21 @@ hipError_t launch_bfp16_unpack(const void* q, const float* scale, float2* out, s
 }
 
 hipError_t launch_phase_shift(int mode, const float2* m, float2* d, const PwArgs& a, hipStream_t st) {
-  const dim3 g = flat_grid((size_t)a.NX * a.NY * a.NW * a.NS);
+  if (a.NW > 65535 || a.NS > 65535) return hipErrorInvalidValue;
+  const dim3 g = plane_grid(a);
   if (mode == 0) hipLaunchKernelGGL(k_phase_shift<0>, g, dim3(kThreads), 0, st, m, d, a);
   else if (mode == 1) hipLaunchKernelGGL(k_phase_shift<1>, g, dim3(kThreads), 0, st, m, d, a);
   else hipLaunchKernelGGL(k_phase_shift<2>, g, dim3(kThreads), 0, st, m, d, a);
   return hipGetLastError();
 }
 hipError_t launch_select(const float2* m, float2* d, const PwArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_select, flat_grid((size_t)a.NX * a.NY * a.NW * a.NS), dim3(kThreads), 0, st, m, d, a);
+  if (a.NW > 65535 || a.NS > 65535) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_select, plane_grid(a), dim3(kThreads), 0, st, m, d, a);
   return hipGetLastError();
 }
 hipError_t launch_taper(const float2* m, float2* d, const PwArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_taper, flat_grid((size_t)a.NX * a.NY * a.NW * a.NS), dim3(kThreads), 0, st, m, d, a);
+  if (a.NW > 65535 || a.NS > 65535) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_taper, plane_grid(a), dim3(kThreads), 0, st, m, d, a);
   return hipGetLastError();
 }
 hipError_t launch_copy16(const void* in, void* out, size_t n16, hipStream_t st) {

@@ -34,7 +34,7 @@ struct RowCfg {
 // private padded exchange regions (one per wave) for the Stockham stages.
 template <int N>
 struct ColCfg {
-  static constexpr int THREADS = kColThreads;
+  static constexpr int THREADS = col_threads(N);
   static constexpr int WAVES = THREADS / 64;
   static constexpr int CW = col_cw(N);
   static constexpr int NB = CW / WAVES;
@@ -49,20 +49,22 @@ struct ColCfg {
 
 // Workgroups are handed to the 8 XCDs round-robin by linear id, and each
 // XCD has its own L2.  A column tile reads CW * 8 bytes of every row; with
-// CW = 8 that is half of a 128-byte line, so two neighbouring tiles share
-// every line they read.  Linear ids L and L + 8 run on the same XCD back to
-// back, so they take the two halves: within each group of 16 ids,
-// 16g + 8h + q gets the logical tile 16g + 2q + h (a trailing partial group
-// keeps the identity).  Logical tile J is column tile J % gridDim.x of plane
-// J / gridDim.x.
+// CW = 8 (4) that is a half (quarter) of a 128-byte line, so G = 2 (4)
+// neighbouring tiles share every line they read.  Linear ids L, L + 8, ...
+// run on the same XCD back to back, so they take the G parts: within each
+// group of 8G ids, 8G g + 8h + q gets the logical tile 8G g + G q + h (a
+// trailing partial group keeps the identity).  Logical tile J is column
+// tile J % gridDim.x of plane J / gridDim.x.
 template <class C>
 __device__ __forceinline__ void col_tile(int& plane, int& c0) {
   const int nb = gridDim.x;
   const int L = blockIdx.y * nb + blockIdx.x;
   int J = L;
-  if constexpr (C::CW * 8 < 128) {
+  constexpr int G = 128 / (C::CW * 8);
+  if constexpr (G > 1) {
     const int total = nb * gridDim.y;
-    if ((L & ~15) + 16 <= total) J = (L & ~15) + 2 * (L & 7) + ((L >> 3) & 1);
+    const int r = L % (8 * G);
+    if (L - r + 8 * G <= total) J = L - r + G * (r % 8) + r / 8;
   }
   plane = J / nb;
   c0 = (J - plane * nb) * C::CW;
@@ -501,13 +503,14 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
 
 // Whether the twiddle table (N values) and the reference slownesses (up to
 // 256) fit in LDS beside the tile and the row masks without lowering the
-// workgroups a CU holds (at most WPE / 2 by registers; 160 KB of LDS).
+// workgroups a CU holds (at most WPE * 4 / waves per workgroup by
+// registers; 160 KB of LDS).
 template <int N, int WPE, class T = float2>
 constexpr bool col_stage_lds() {
   using C = ColCfg<N>;
   constexpr int base = col_smem<N, T>() * 8 + C::NROW * 4;
   constexpr int extra = N * (int)sizeof(T) + 256 * 8;
-  constexpr int by_regs = WPE / 2;
+  constexpr int by_regs = WPE * 4 / C::WAVES;
   constexpr int without = (160 * 1024) / base < by_regs ? (160 * 1024) / base : by_regs;
   constexpr int with = (160 * 1024) / (base + extra) < by_regs ? (160 * 1024) / (base + extra) : by_regs;
   return with >= without;
@@ -825,8 +828,8 @@ hipError_t launch_cols(bool inv, const KArgs& a, hipStream_t st) {
   return dispatch_n(a.NY, [&](auto nc) -> hipError_t {
     constexpr int N = decltype(nc)::value;
     const dim3 g = col_grid(N, a.NX, a.np);
-    if (inv) hipLaunchKernelGGL((k_cols<N, true>), g, dim3(kColThreads), 0, st, a);
-    else hipLaunchKernelGGL((k_cols<N, false>), g, dim3(kColThreads), 0, st, a);
+    if (inv) hipLaunchKernelGGL((k_cols<N, true>), g, dim3(col_threads(N)), 0, st, a);
+    else hipLaunchKernelGGL((k_cols<N, false>), g, dim3(col_threads(N)), 0, st, a);
     return hipGetLastError();
   });
 }
@@ -885,16 +888,16 @@ hipError_t launch_cols_fanout(int mode, const KArgs& a, hipStream_t st) {
     constexpr int N = decltype(nc)::value;
     const dim3 g = col_grid(N, a.NX, a.np);
     if (a.f64) {
-      if (mode == 0) hipLaunchKernelGGL((k_cols_fanout<N, 0, 2, double2>), g, dim3(kColThreads), 0, st, a);
-      else if (mode == 1) hipLaunchKernelGGL((k_cols_fanout<N, 1, 2, double2>), g, dim3(kColThreads), 0, st, a);
-      else hipLaunchKernelGGL((k_cols_fanout<N, 2, 2, double2>), g, dim3(kColThreads), 0, st, a);
+      if (mode == 0) hipLaunchKernelGGL((k_cols_fanout<N, 0, 2, double2>), g, dim3(col_threads(N)), 0, st, a);
+      else if (mode == 1) hipLaunchKernelGGL((k_cols_fanout<N, 1, 2, double2>), g, dim3(col_threads(N)), 0, st, a);
+      else hipLaunchKernelGGL((k_cols_fanout<N, 2, 2, double2>), g, dim3(col_threads(N)), 0, st, a);
       return hipGetLastError();
     }
     return dispatch_wpe(col_wpe_get(0, mode, N), [&](auto wc) -> hipError_t {
       constexpr int W = decltype(wc)::value;
-      if (mode == 0) hipLaunchKernelGGL((k_cols_fanout<N, 0, W>), g, dim3(kColThreads), 0, st, a);
-      else if (mode == 1) hipLaunchKernelGGL((k_cols_fanout<N, 1, W>), g, dim3(kColThreads), 0, st, a);
-      else hipLaunchKernelGGL((k_cols_fanout<N, 2, W>), g, dim3(kColThreads), 0, st, a);
+      if (mode == 0) hipLaunchKernelGGL((k_cols_fanout<N, 0, W>), g, dim3(col_threads(N)), 0, st, a);
+      else if (mode == 1) hipLaunchKernelGGL((k_cols_fanout<N, 1, W>), g, dim3(col_threads(N)), 0, st, a);
+      else hipLaunchKernelGGL((k_cols_fanout<N, 2, W>), g, dim3(col_threads(N)), 0, st, a);
       return hipGetLastError();
     });
   });
@@ -905,16 +908,16 @@ hipError_t launch_cols_fanin(int mode, const KArgs& a, hipStream_t st) {
     constexpr int N = decltype(nc)::value;
     const dim3 g = col_grid(N, a.NX, a.np);
     if (a.f64) {
-      if (mode == 0) hipLaunchKernelGGL((k_cols_fanin<N, 0, 2, double2>), g, dim3(kColThreads), 0, st, a);
-      else if (mode == 1) hipLaunchKernelGGL((k_cols_fanin<N, 1, 2, double2>), g, dim3(kColThreads), 0, st, a);
-      else hipLaunchKernelGGL((k_cols_fanin<N, 2, 2, double2>), g, dim3(kColThreads), 0, st, a);
+      if (mode == 0) hipLaunchKernelGGL((k_cols_fanin<N, 0, 2, double2>), g, dim3(col_threads(N)), 0, st, a);
+      else if (mode == 1) hipLaunchKernelGGL((k_cols_fanin<N, 1, 2, double2>), g, dim3(col_threads(N)), 0, st, a);
+      else hipLaunchKernelGGL((k_cols_fanin<N, 2, 2, double2>), g, dim3(col_threads(N)), 0, st, a);
       return hipGetLastError();
     }
     return dispatch_wpe(col_wpe_get(1, mode, N), [&](auto wc) -> hipError_t {
       constexpr int W = decltype(wc)::value;
-      if (mode == 0) hipLaunchKernelGGL((k_cols_fanin<N, 0, W>), g, dim3(kColThreads), 0, st, a);
-      else if (mode == 1) hipLaunchKernelGGL((k_cols_fanin<N, 1, W>), g, dim3(kColThreads), 0, st, a);
-      else hipLaunchKernelGGL((k_cols_fanin<N, 2, W>), g, dim3(kColThreads), 0, st, a);
+      if (mode == 0) hipLaunchKernelGGL((k_cols_fanin<N, 0, W>), g, dim3(col_threads(N)), 0, st, a);
+      else if (mode == 1) hipLaunchKernelGGL((k_cols_fanin<N, 1, W>), g, dim3(col_threads(N)), 0, st, a);
+      else hipLaunchKernelGGL((k_cols_fanin<N, 2, W>), g, dim3(col_threads(N)), 0, st, a);
       return hipGetLastError();
     });
   });
