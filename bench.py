@@ -1,30 +1,34 @@
 #!/usr/bin/env python3
-"""Benchmark: Born forward + adjoint (migration) on one MI355X.
+"""Benchmark: the FWI slowness gradient on one MI355X (the largest workload
+that fits one GPU), plus the other workloads of the path.
 
-One step = background down-going modeling (every U_z kept in HBM) + Born
-forward (linearized modeling of a smoothed random slowness perturbation) +
-Born adjoint (migration of the Born data into the extended model), on the
-256^2 x 200 grid padded to 512^2, 64 frequencies over 3-40 Hz, 4 reference
-slownesses, 20-node taper.  The metric counts every executed wavefield depth
-step (NX*NY*NW*NS points each) per second.
+Default (--config c5): 512^2 x 400 model padded to 1024^2, 128 frequencies
+over 2-40 Hz, 4 reference slownesses, 20-node taper, 16 shots on a 4x4
+surface grid.  One step = the gradient of one shot: background modeling on
+the start model with checkpoints every 16 depths, the L2 residual against
+data modeled on the true model (before the timed region), and the Born
+adjoint of the residual with segment recomputation.  Steps cycle through the
+shots.  The metric counts the background and adjoint depth steps of every
+timed shot slice (NX*NY*NW points each) per second; seconds per shot are
+reported beside it (`grad_s_per_shot`).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c5] [--no-cpu]
 
 Prints one JSON line.  Before the timed region the column kernels' register
 budget is tuned for the padded size (`--no-autotune` keeps the defaults) and
 a stream copy measures the achievable HBM bandwidth (`roofline.peak_measured`).
-`--graph` also times the step replayed as one HIP graph.
 
 Other workloads (same JSON shape, their own metric where it differs):
 
+    --config c2        Born forward + adjoint (migration) with background
+                       modeling, 256^2 x 200 padded to 512^2, 64 frequencies
+                       (`--graph` also times the step as one HIP graph)
     --config c1        the c1 grid through the same Born step
     --config c3        NSPS Downward forward + adjoint in the salt model,
                        beside the same recursion with PSPI steps
     --config c4        64 shots of uneven aperture / frequency count as shot
                        batches (`pyfwix_amd/batching.py`), vs one shot per
                        operator, plus a PSPI step sample vs the plain path
-    --config c5        FWI gradient, 16 shots, checkpoints every 16 depths:
-                       seconds per shot (use --budget-gb 200)
 
 Profiling helpers (the summaries under profiles/ are made this way):
 
@@ -82,11 +86,12 @@ def _short(name):
     return name[5:] if name.startswith("void ") else name
 
 
-def summarize_stats(csv_path, out_path, command, top=16):
+def summarize_stats(csv_path, out_path, command, top=16,
+                    title="c2 Born forward + adjoint (512x512 padded, 64 frequencies, 200 depths)"):
     rows = _csv_rows(csv_path)
     rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
     tot = sum(float(r["TotalDurationNs"]) for r in rows)
-    lines = ["# Kernel time, c2 Born forward + adjoint (512x512 padded, 64 frequencies, 200 depths)", "",
+    lines = [f"# Kernel time, {title}", "",
              "Command (from the repository root):", "", "```", command, "```", "",
              f"Total kernel time {tot / 1e6:.1f} ms over {sum(int(r['Calls']) for r in rows)} dispatches.", "",
              "| kernel | calls | total ms | avg us | min us | max us | % |", "|---|---|---|---|---|---|---|"]
@@ -108,7 +113,8 @@ def _counter_means(csv_path, counter):
     return {k: (n, v / n) for k, (n, v) in acc.items()}
 
 
-def summarize_traffic(fetch_csv, write_csv, out_path, command_fetch, command_write):
+def summarize_traffic(fetch_csv, write_csv, out_path, command_fetch, command_write,
+                      title="c2 Born forward + adjoint"):
     fetch = _counter_means(fetch_csv, "FETCH_SIZE")
     write = _counter_means(write_csv, "WRITE_SIZE")
     probe = [k for k in fetch if COPY_PROBE_TAG in k]
@@ -117,7 +123,7 @@ def summarize_traffic(fetch_csv, write_csv, out_path, command_fetch, command_wri
     kib = COPY_PROBE_BYTES / 1024.0
     ff = kib / fetch[probe[0]][1]
     wf = kib / write[[k for k in write if COPY_PROBE_TAG in k][0]][1]
-    lines = ["# HBM traffic per dispatch, c2 Born forward + adjoint", "",
+    lines = [f"# HBM traffic per dispatch, {title}", "",
              "Two counter runs (from the repository root):", "", "```", command_fetch, command_write, "```", "",
              f"Calibration: a plain copy of {COPY_PROBE_BYTES} bytes (`{COPY_PROBE_TAG}` kernel, 1.0 * x) reads "
              f"{fetch[probe[0]][1]:.0f} KiB (FETCH_SIZE) and writes {write[[k for k in write if COPY_PROBE_TAG in k][0]][1]:.0f} KiB "
@@ -211,11 +217,13 @@ def cpu_baseline(w, budget_s=30.0):
                        f"{dt:.1f} s")}
 
 
-def plain_vs_fused(w, slow_dims, sref, labels, u, reps=5):
-    """PSPI forward + adjoint depth step: fused path vs the plain per-operator path."""
+def plain_vs_fused(w, slow_dims, sref, labels, u, reps=5, copy_gbs=None):
+    """PSPI forward + adjoint depth step: fused path vs the plain per-operator path
+    (with the plain kernels' per-pass bandwidth when copy_gbs is given)."""
     import torch
     from pyfwix_amd.ops import FWIX_PSPI, FWIX_PSPI_PLAIN, OneStep
     out = {}
+    table = None
     for name, method in (("fused", FWIX_PSPI), ("plain", FWIX_PSPI_PLAIN)):
         op = OneStep(method, slow_dims, nref=w.nref, taperx=w.taper, tapery=w.taper)
         op.set_refs(sref, labels)
@@ -230,9 +238,15 @@ def plain_vs_fused(w, slow_dims, sref, labels, u, reps=5):
             op.adjoint(False, u, u)
         torch.cuda.synchronize()
         out[name] = 2 * reps * w.points / (time.perf_counter() - t0) / 1e9
+        if method == FWIX_PSPI_PLAIN and copy_gbs:
+            def _once():
+                op.forward(False, u, u)
+                op.adjoint(False, u, u)
+                torch.cuda.synchronize()
+            table = plain_pass_table(copy_gbs, _once)
         del op
     return {"fused_gpts": round(out["fused"], 3), "plain_gpts": round(out["plain"], 3),
-            "speedup": round(out["fused"] / out["plain"], 3)}
+            "speedup": round(out["fused"] / out["plain"], 3), "plain_passes": table}
 
 
 def hbm_copy_gbs(dev, nbytes=2 << 30, reps=10):
@@ -550,84 +564,245 @@ def bench_nsps(args):
     print(json.dumps(res), flush=True)
 
 
+def _roofline(passes, config, kernel_of, copy_gbs, ev_us):
+    """Roofline of the dominant pass (largest estimated total time) from the
+    sampled per-pass event times; traffic / VALU from the committed counter
+    summaries of the same config when they exist."""
+    if not passes:
+        return None
+    est = {k: v["ms"] / v["launches"] * v["total_launches"] for k, v in passes.items()}
+    dom_name = max(est, key=est.get)
+    dom = passes[dom_name]
+    avg_us = dom["ms"] * 1e3 / dom["launches"]
+    per_launch = dom["bytes"] / dom["launches"]
+    achieved = per_launch / (avg_us * 1e-6) / 1e9
+    kern = kernel_of(dom_name)
+    vi = valu_for(config, kern)
+    return {"bound": "hbm", "kernel": dom_name, "kernel_symbol": kern, "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "peak_measured": round(copy_gbs, 1), "frac_measured": round(achieved / copy_gbs, 4),
+            "traffic": traffic_for(config, kern),
+            "traffic_source": f"profiles/traffic_{config}.md (two rocprofv3 --pmc runs of this config)",
+            "valu_issue": None if vi is None else {
+                "instr_per_launch": vi, "achieved": round(vi / avg_us / 1e3, 1), "peak": VALU_PEAK_GINSTR,
+                "unit": "G wave-instr/s", "frac": round(vi / avg_us / 1e3 / VALU_PEAK_GINSTR, 4)},
+            "alg_bytes_per_launch": round(per_launch), "avg_us": round(avg_us, 2),
+            "launches_timed": dom["launches"], "launches": dom["total_launches"],
+            "empty_event_pair_us": None if ev_us is None else round(ev_us, 2)}
+
+
+def _passes_table(passes):
+    return {k: {"ms_est": round(v["ms"] / v["launches"] * v["total_launches"], 3),
+                "launches": v["total_launches"], "timed": v["launches"],
+                "avg_us": round(v["ms"] * 1e3 / v["launches"], 2),
+                "GBs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)} for k, v in
+            sorted(passes.items(), key=lambda kv: -kv[1]["ms"] / kv[1]["launches"] * kv[1]["total_launches"])}
+
+
+def plain_pass_table(copy_gbs, timed):
+    """Per-pass bandwidth of the plain per-operator kernels recorded while
+    `timed` ran (every launch timed with HIP events), against the copy rate."""
+    from pyfwix_amd import _lib
+    _lib.timing_reset()
+    _lib.timing_select(None, 1)
+    _lib.timing_enable(1 << 14)
+    try:
+        timed()
+    finally:
+        _lib.timing_disable()
+    passes = _lib.timing_collect()
+    _lib.timing_reset()
+    out = {}
+    for k, v in sorted(passes.items()):
+        if not k.startswith("plain_") or not v["launches"]:
+            continue
+        gbs = v["bytes"] / (v["ms"] * 1e-3) / 1e9
+        out[k] = {"avg_us": round(v["ms"] * 1e3 / v["launches"], 2), "GBs": round(gbs, 1),
+                  "frac_of_copy": round(gbs / copy_gbs, 3), "launches": v["launches"]}
+    return out
+
+
+def cpu_baseline_fwi(N, nw, nx, taper, slow3, refs, d, freqs, budget_planes=16):
+    """FP32 numpy/scipy restatement (the oracle in complex64, threaded FFTs) of
+    one background PSPI step and one Born adjoint step of the FWI slice at its
+    full lateral size on `budget_planes` of its frequencies; Gpts*freq/s."""
+    import oracle as O
+    cores = min(16, os.cpu_count() or 1)
+    O.set_fft_workers(cores)
+    nwp = min(nw, budget_planes)
+    g = O.Grid(NX=N, NY=N, NW=nwp, NS=1, nx=nx, ny=nx, dx=d, dy=d, ow=float(freqs[0]),
+               dw=float(freqs[1] - freqs[0]), dz=d, nz=slow3.shape[0])
+    z = slow3.shape[0] // 2
+    rng = np.random.default_rng(0)
+    shape = (1, nwp, N, N)
+    U = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    lam = U[:, ::-1].copy()
+    sref = np.repeat(refs[0][z][:, None], nwp, axis=1).astype(np.complex64)
+    lab = np.repeat(refs[1][z][None], nwp, axis=0)
+    slow_z = np.broadcast_to(slow3[z][None], (nwp, nx, nx)).astype(np.complex64)
+    t0 = time.perf_counter()
+    O.pspi_forward(U, sref, lab, g, d, 0.0, taper, taper)                              # background step
+    mu = O.pspi_adjoint(lam, sref, lab, g, d, 0.0, taper, taper)                       # Born adjoint step
+    O.ic_adjoint(O.scatter_adjoint(mu, slow_z, g, d, 0.04, 1), U, nx, nx)
+    dt = time.perf_counter() - t0
+    return {"value": round(2 * nwp * N * N / dt / 1e9, 4), "unit": "Gpts*freq/s", "cores": cores,
+            "host_cpus": os.cpu_count(), "kind": "port",
+            "sample": (f"one background PSPI step + one Born adjoint step at depth {z} of one shot slice "
+                       f"({N}x{N} padded, {nwp} of its {nw} frequencies), numpy/scipy.fft complex64, "
+                       f"{dt:.1f} s")}
+
+
 def bench_fwi(args):
-    """FWI-style slowness gradient: per launch (a group of shots that fits the
-    HBM budget), background modeling on the start model with checkpoints every
-    `ckpt` depths, the L2 data residual against data modeled on the true model,
-    and the Born adjoint (migration) of the residual with segment recomputation;
-    the gradient is summed over shots.  Reports seconds per shot."""
+    """FWI-style slowness gradient, one group of shots per step (default one
+    shot): background modeling on the start model with checkpoints every
+    `ckpt` depths, the L2 data residual against data modeled on the true model
+    (before the timed region), and the Born adjoint of the residual with
+    segment recomputation; the gradient is summed over shots.  Steps cycle
+    through the shots.  The value counts the background and adjoint depth
+    steps of every timed shot slice (recomputed steps not counted); the
+    seconds per shot are reported beside it."""
     import torch
-    from pyfwix_amd.batching import ShotBatch, launch_bytes, padded_size, shot_refs
+    from pyfwix_amd import _lib
+    from pyfwix_amd.batching import ShotBatch, padded_size, shot_refs
     from pyfwix_amd.ops import FWIX_PSPI, FWIX_PSPI_PLAIN, OneStep, make_dims
     from pyfwix_amd.workloads import FWI_CONFIGS, fwi_models, grid_shots
 
     cfg = FWI_CONFIGS[args.config]
     dev = torch.device("cuda:0")
     nz, d, nref, taper, ck = cfg["nz"], cfg["d"], cfg["nref"], cfg["taper"], cfg["ckpt"]
+    nx = cfg["nx"]
     params = {"nref": nref, "taper": taper, "eps": 0.0, "eps_scat": 0.04, "taylor": 1}
     t_prep = time.perf_counter()
     shots = grid_shots(**cfg)
     if args.shots:
         shots = shots[:args.shots]
     s_true, s_start = fwi_models(**cfg)
-    N = padded_size(cfg["nx"])
+    N = padded_size(nx)
     refs_true = shot_refs(s_true, nref, N, N)
     refs_start = shot_refs(s_start, nref, N, N)
     t_prep = time.perf_counter() - t_prep
-    budget = int(args.budget_gb * (1 << 30))
+    per = max(1, min(args.shots_per_step, len(shots)))
+    groups = [shots[i:i + per] for i in range(0, len(shots) - per + 1, per)]
     nw = shots[0].nw
-    per = 1
-    while per < len(shots) and launch_bytes(nw * (per + 1), per + 1, N, N, cfg["nx"], cfg["nx"], nz, ck) <= budget:
-        per += 1
-    groups = [list(range(i, min(i + per, len(shots)))) for i in range(0, len(shots), per)]
-    print(f"[fwi] {len(shots)} shots in {len(groups)} launches of <= {per}, host models {t_prep:.1f} s",
+    print(f"[fwi] {len(shots)} shots, {per} per step, {len(groups)} groups, host models {t_prep:.1f} s",
           file=sys.stderr, flush=True)
-    tuned = None if args.no_autotune else autotune_sizes([N])
-
-    def batch(idx, s3, refs):
-        b = ShotBatch([shots[i] for i in idx], nz, d, d, d, params, ckpt_every=ck, device=dev, NX=N, NY=N)
-        b.set_slowness([s3] * len(idx), [refs] * len(idx))
-        return b
-
+    copy_gbs = hbm_copy_gbs(dev)
+    zt = nz // 2
+    # tune on the middle depth's references over every plane of one step
+    npl = nw * per
+    tuned = None if args.no_autotune else autotune_sizes(
+        [N], refs=(torch.from_numpy(np.ascontiguousarray(np.repeat(refs_start[0][zt:zt + 1, :, None], npl, 2))).to(dev),
+                   torch.from_numpy(refs_start[1][zt:zt + 1, None]).to(dev).expand(1, npl, N, N).contiguous()),
+        freqs=np.tile(shots[0].freqs, per))
+    b = ShotBatch(groups[0], nz, d, d, d, params, ckpt_every=ck, device=dev, NX=N, NY=N,
+                  planes_per_chunk=args.chunk)
     # observed data on the true model (outside the timed region)
+    t_obs = time.perf_counter()
+    b.set_slowness([s_true] * per, [refs_true] * per)
     observed = []
-    for idx in groups:
-        b = batch(idx, s_true, refs_true)
+    for grp in groups:
+        b.set_shots(grp)
         dobs = torch.zeros(b.nsamp, dtype=torch.complex64, device=dev)
         b.op.model(dobs)
-        torch.cuda.synchronize()
         observed.append(dobs)
-        del b
-        torch.cuda.empty_cache()
-    grad = torch.zeros((nz, cfg["nx"], cfg["nx"]), dtype=torch.complex64, device=dev)
-    wall, steps, misfit = 0.0, 0, 0.0
-    for idx, dobs in zip(groups, observed):
-        b = batch(idx, s_start, refs_start)
-        data = torch.zeros(b.nsamp, dtype=torch.complex64, device=dev)
-        g = torch.zeros((nz, len(idx), b.ny, b.nx), dtype=torch.complex64, device=dev)
-        torch.cuda.synchronize()
-        s0 = b.op.steps
-        t0 = time.perf_counter()
+    torch.cuda.synchronize()
+    t_obs = time.perf_counter() - t_obs
+    b.set_slowness([s_start] * per, [refs_start] * per)
+    grad = torch.zeros((nz, nx, nx), dtype=torch.complex64, device=dev)
+    data = torch.zeros(b.nsamp, dtype=torch.complex64, device=dev)
+    g = torch.zeros((nz, per, b.ny, b.nx), dtype=torch.complex64, device=dev)
+    misfit = torch.zeros((), dtype=torch.float32, device=dev)
+    k_step = [0]
+
+    def step():
+        k = k_step[0] % len(groups)
+        k_step[0] += 1
+        b.set_shots(groups[k])
         b.op.model(data)                          # background on the start model, data recorded
-        data.sub_(dobs)                           # L2 residual
+        data.sub_(observed[k])                    # L2 residual
+        misfit.add_(0.5 * torch.linalg.vector_norm(data) ** 2)
         b.op.adjoint(False, g, data)              # gradient of 0.5 |d - d_obs|^2 (Born adjoint)
         grad.add_(g.sum(1))
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        wall += dt
-        steps += b.op.steps - s0
-        misfit += 0.5 * float(torch.linalg.vector_norm(data) ** 2)
-        print(f"[fwi] launch of {len(idx)} shots ({b.planes} planes, ckpt {ck}): {dt:.3f} s", file=sys.stderr,
-              flush=True)
-        del b, data, g
-        torch.cuda.empty_cache()
-    # the gradient of one shot, fused vs reference-granularity operators, on the
-    # regular operator layout (frequency-replicated slowness and labels: 107 GB
-    # and 54 GB at the full depth, so the first `zw` depths are sampled)
-    one = None
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    grad.zero_()
+    misfit.zero_()
+    s0 = b.op.steps
+    timed = not args.no_pass_timing
+    if timed:
+        _lib.timing_reset()
+        _lib.timing_select(None, max(1, args.pass_sample))
+        _lib.timing_enable(1 << 15)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    passes, ev_us = {}, None
+    if timed:
+        _lib.timing_disable()
+        passes = _lib.timing_collect()
+        _lib.timing_select(None, 1)
+        ev_us = _lib.timing_overhead(torch.cuda.current_stream().cuda_stream, 200)
+    steps_exec = b.op.steps - s0
+    counted = 2 * (nz - 1) * args.steps           # background + adjoint steps of each timed shot slice
+    P = b.points
+    value = counted * P / wall / 1e9
+
+    class _W:  # pass -> kernel symbol: a shot batch maps planes to shots (row kernels' MAP flag)
+        NX = NY = N
+    kern = lambda name: (pass_kernel(name, _W) or "").replace(", false>", ", true>") or None  # noqa: E731
+    roof = _roofline(passes, args.config, kern, copy_gbs, ev_us)
+    lab = N * N * b.planes
+    m = nx * nx * b.planes
+    alg = (nz - 1) * args.steps * ((16 * P + lab) + (24 * P + 16 * m + lab)) + (nz // ck) * 8 * P * args.steps
+    res = {
+        "metric": "wavefield Gpts*freq/s per depth step (fwd+adj)",
+        "value": round(value, 4),
+        "unit": "Gpts*freq/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex64",
+        "data": "synthetic (smooth background + seeded Gaussian anomalies; start model smoothed in closed form; "
+                "observed data modeled on the true model)",
+        "config": {"workload": f"{args.config}: FWI slowness gradient (background modeling on the start model "
+                               f"with checkpoints every {ck} depths, L2 residual, Born adjoint with segment "
+                               f"recomputation), {per} shot(s) per step cycling over {len(groups) * per} shots",
+                   "nx": nx, "ny": nx, "nz": nz, "NX": N, "NY": N, "nw": nw, "band_hz": [cfg["fmin"], cfg["fmax"]],
+                   "nref": nref, "taper": taper, "ckpt_every": ck, "shots_per_step": per,
+                   "receivers": f"every {cfg['rec_step']}th node on the deepest plane",
+                   "planes_per_chunk": args.chunk or "auto"},
+        "grad_s_per_shot": round(wall / args.steps / per, 4),
+        "depth_steps": {"counted_per_step": 2 * (nz - 1), "executed_per_step": steps_exec // args.steps,
+                        "note": "background and adjoint steps counted; recomputed background segments not"},
+        "roofline": roof,
+        "step_roofline": {"achieved": round(alg / wall / 1e9, 1), "unit": "GB/s",
+                          "frac": round(alg / wall / 1e9 / HBM_PEAK_GBS, 4),
+                          "bytes": "16P+labels per background step, 24P+16m+labels per adjoint step, "
+                                   "8P per checkpoint write"},
+        "passes": _passes_table(passes),
+        "misfit": float(misfit),
+        "grad_norm": float(torch.linalg.vector_norm(grad)),
+        "column_budgets": tuned,
+        "host_model_prep_s": round(t_prep, 1),
+        "observed_data_s": round(t_obs, 1),
+    }
+    del b, data, g
+    torch.cuda.empty_cache()
     if not args.no_plain:
-        from pyfwix_amd.ops import FWIX_PSPI as _FUSED, FWIX_PSPI_PLAIN as _PLAIN, Born
-        sh, nx = shots[0], cfg["nx"]
+        # the gradient of one shot, fused vs reference-granularity operators, on the
+        # regular operator layout (frequency-replicated slowness and labels: 107 GB
+        # and 54 GB at the full depth, so the first `zw` depths are sampled)
+        from pyfwix_amd.ops import Born
+        sh = shots[0]
         zw = min(nz, 3 * ck)
         dims = make_dims(N, N, nw, 1, nx, nx, 0.0, d, 0.0, d, float(sh.freqs[0]), 1.0, 0.0, d, zw)
         born = Born(dims, nref=nref, taperx=taper, tapery=taper, eps_scat=0.04, taylor=1, ckpt_every=ck)
@@ -643,7 +818,7 @@ def bench_fwi(args):
         data1 = torch.zeros((len(sh.rec), nw), dtype=torch.complex64, device=dev)
         g1 = torch.zeros_like(slow)
         t_one = {}
-        for name, method in (("fused", _FUSED), ("plain", _PLAIN)):
+        for name, method in (("fused", FWIX_PSPI), ("plain", FWIX_PSPI_PLAIN)):
             born.set_method(method)
             born.set_frequencies(sh.freqs)
             born.set_refs(sref, labels)
@@ -651,6 +826,7 @@ def bench_fwi(args):
             born.set_sources(sh.src[:, 0], sh.src[:, 1], sh.src[:, 2], np.zeros(1, np.int32))
             born.set_receivers(sh_rec[:, 0], sh_rec[:, 1], sh_rec[:, 2], np.zeros(len(sh_rec), np.int32))
             born.set_wavelet(wav1)
+            born.model(data1)                          # warm
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             born.model(data1)
@@ -658,28 +834,26 @@ def bench_fwi(args):
             torch.cuda.synchronize()
             t_one[name] = time.perf_counter() - t0
             print(f"[fwi] one shot, {name}: {t_one[name]:.3f} s", file=sys.stderr, flush=True)
-        one = {"fused_s": round(t_one["fused"], 3), "plain_s": round(t_one["plain"], 3),
-               "speedup": round(t_one["plain"] / t_one["fused"], 3), "depths": zw,
-               "plain_s_per_shot_est": round(wall / len(shots) * t_one["plain"] / t_one["fused"], 3),
-               "sample": f"background modeling + Born adjoint of shot 0 alone over the first {zw} depths on the "
-                         f"regular operator layout, fused steps vs imaging condition / scattering / PSPI as separate "
-                         f"operators; plain_s_per_shot_est scales the measured s/shot by that ratio"}
+        res["vs_plain_shot"] = {
+            "fused_s": round(t_one["fused"], 3), "plain_s": round(t_one["plain"], 3),
+            "speedup": round(t_one["plain"] / t_one["fused"], 3), "depths": zw,
+            "plain_s_per_shot_est": round(res["grad_s_per_shot"] * t_one["plain"] / t_one["fused"], 3),
+            "sample": f"background modeling + Born adjoint of shot 0 alone over the first {zw} depths on the "
+                      f"regular operator layout, fused steps vs imaging condition / scattering / PSPI as separate "
+                      f"operators; plain_s_per_shot_est scales the measured s/shot by that ratio"}
         del born, slow, sref, labels, wav1, data1, g1
         torch.cuda.empty_cache()
-    pts = sum(s.nw for s in shots) * N * N
-    nominal = 2 * (nz - 1)
-    # PSPI forward + adjoint step, fused vs the plain per-operator kernels, on one shot's planes
-    sample = None
-    if not args.no_plain:
+        # PSPI forward + adjoint step, fused vs the plain per-operator kernels, on one shot's planes
         rates = {}
-        dims = make_dims(N, N, nw, 1, cfg["nx"], cfg["nx"], 0.0, d, 0.0, d, 1.0, 0.0, 0.0, d, 2)
+        dims = make_dims(N, N, nw, 1, nx, nx, 0.0, d, 0.0, d, 1.0, 0.0, 0.0, d, 2)
         sref = torch.from_numpy(np.ascontiguousarray(np.repeat(refs_start[0][:2, :, None], nw, 2))).to(dev)
-        lab = torch.from_numpy(np.ascontiguousarray(np.repeat(refs_start[1][:2, None], nw, 1))).to(dev)
+        lab_t = torch.from_numpy(np.ascontiguousarray(np.repeat(refs_start[1][:2, None], nw, 1))).to(dev)
         u = torch.randn((1, nw, N, N), dtype=torch.complex64, device=dev)
+        plain_table = None
         for name, method in (("fused", FWIX_PSPI), ("plain", FWIX_PSPI_PLAIN)):
             op = OneStep(method, dims, nref=nref, taperx=taper, tapery=taper)
             op.set_frequencies(shots[0].freqs)
-            op.set_refs(sref, lab)
+            op.set_refs(sref, lab_t)
             op.set_depth(1)
             op.forward(False, u, u)
             op.adjoint(False, u, u)
@@ -690,39 +864,21 @@ def bench_fwi(args):
                 op.adjoint(False, u, u)
             torch.cuda.synchronize()
             rates[name] = 6 * nw * N * N / (time.perf_counter() - t0) / 1e9
+            if method == FWIX_PSPI_PLAIN:
+                def _once():
+                    op.forward(False, u, u)
+                    op.adjoint(False, u, u)
+                    torch.cuda.synchronize()
+                plain_table = plain_pass_table(copy_gbs, _once)
             del op
-        sample = {"fused_gpts": round(rates["fused"], 3), "plain_gpts": round(rates["plain"], 3),
-                  "speedup": round(rates["fused"] / rates["plain"], 3),
-                  "sample": f"PSPI forward + adjoint at one depth of one shot ({N}x{N}x{nw}), 3 repetitions"}
-    res = {
-        "metric": "FWI grad s/shot",
-        "value": round(wall / len(shots), 4),
-        "unit": "s/shot",
-        "n_gpus": 1,
-        "steps": 1,
-        "warmup": 0,
-        "ms_per_step": round(wall * 1e3, 1),
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "complex64",
-        "data": "synthetic (smooth background + seeded Gaussian anomalies; start model smoothed in closed form)",
-        "config": {"workload": f"{args.config}: FWI slowness gradient (background modeling with checkpoints, L2 "
-                               f"residual, Born adjoint with recomputation), {len(shots)} shots",
-                   "nx": cfg["nx"], "ny": cfg["nx"], "nz": nz, "NX": N, "NY": N, "nw": nw,
-                   "band_hz": [cfg["fmin"], cfg["fmax"]], "nref": nref, "taper": taper, "ckpt_every": ck,
-                   "receivers": f"every {cfg['rec_step']}th node on the deepest plane", "launches": len(groups),
-                   "shots_per_launch": per, "hbm_budget_gb": args.budget_gb},
-        "gpts_per_s": round(nominal * pts / wall / 1e9, 4),
-        "depth_steps": {"counted": f"{nominal} per shot slice (background, adjoint)", "executed": steps},
-        "misfit": misfit,
-        "grad_norm": float(torch.linalg.vector_norm(grad)),
-        "vs_plain_step": sample,
-        "vs_plain_shot": one,
-        "column_budgets": tuned,
-        "host_model_prep_s": round(t_prep, 1),
-        "cpu_baseline": None,
-    }
+        res["vs_plain"] = {"fused_gpts": round(rates["fused"], 3), "plain_gpts": round(rates["plain"], 3),
+                           "speedup": round(rates["fused"] / rates["plain"], 3),
+                           "sample": f"PSPI forward + adjoint at one depth of one shot ({N}x{N}x{nw}), 3 repetitions",
+                           "plain_passes": plain_table}
+        del u, sref, lab_t
+        torch.cuda.empty_cache()
+    res["cpu_baseline"] = None if args.no_cpu else cpu_baseline_fwi(
+        N, nw, nx, taper, s_start, refs_start, d, shots[0].freqs)
     print(json.dumps(res), flush=True)
 
 
@@ -731,7 +887,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2")
+    ap.add_argument("--config", default="c5")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
     ap.add_argument("--copy-probe", action="store_true", help="run the counter calibration copy first")
@@ -748,9 +904,11 @@ def main():
     ap.add_argument("--shots", type=int, default=0, help="shot-batch configs: first n shots only (0: all)")
     ap.add_argument("--budget-gb", type=float, default=160.0, help="shot-batch configs: HBM per launch")
     ap.add_argument("--no-per-shot", action="store_true", help="shot-batch configs: skip the one-shot-per-launch leg")
+    ap.add_argument("--shots-per-step", type=int, default=1, help="c5: shots per gradient step (one launch)")
     ap.add_argument("--summarize-stats", nargs=2, metavar=("CSV", "OUT"))
     ap.add_argument("--summarize-traffic", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT"))
     ap.add_argument("--command", action="append", default=[], help="profiling command recorded in a summary")
+    ap.add_argument("--title", default=None, help="workload named in a summary's title")
     args = ap.parse_args()
     if os.environ.get("ROCPROF_COUNTER_COLLECTION", "").strip().lower() in ("1", "true", "yes", "on"):
         # Under rocprofv3 --pmc the profiler intercepts the dispatch queue and
@@ -760,11 +918,13 @@ def main():
         # HSA_STATUS_ERROR_INVALID_PACKET_FORMAT), so counter runs record no events.
         args.no_pass_timing = True
     if args.summarize_stats:
-        summarize_stats(*args.summarize_stats, command=" ".join(args.command))
+        kw = {"title": args.title} if args.title else {}
+        summarize_stats(*args.summarize_stats, command=" ".join(args.command), **kw)
         return
     if args.summarize_traffic:
         cmds = args.command + ["", ""]
-        summarize_traffic(*args.summarize_traffic, command_fetch=cmds[0], command_write=cmds[1])
+        kw = {"title": args.title} if args.title else {}
+        summarize_traffic(*args.summarize_traffic, command_fetch=cmds[0], command_write=cmds[1], **kw)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); shots and "
@@ -772,6 +932,15 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe:
+        # a plain 1.0 * x copy of known size: counter runs scale FETCH_SIZE / WRITE_SIZE by it
+        import torch
+        x = torch.ones(COPY_PROBE_BYTES // 4, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(2):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        del x, y
     from pyfwix_amd.workloads import FWI_CONFIGS, NSPS_CONFIGS, SHOT_BATCH_CONFIGS
     if args.config in NSPS_CONFIGS:
         bench_nsps(args)
@@ -789,13 +958,6 @@ def main():
     from pyfwix_amd.workloads import CONFIGS, born_workload, device_inputs, workload_dims
 
     dev = torch.device("cuda:0")
-    if args.copy_probe:
-        x = torch.ones(COPY_PROBE_BYTES // 4, dtype=torch.float32, device=dev)
-        y = torch.empty_like(x)
-        for _ in range(2):
-            torch.mul(x, 1.0, out=y)
-        torch.cuda.synchronize()
-        del x, y
     cfg = CONFIGS[args.config]
     w = born_workload(args.config, **cfg)
     copy_gbs = hbm_copy_gbs(dev)
@@ -936,7 +1098,7 @@ def main():
     del born, grad, data
     if not args.no_plain:
         u = torch.randn((w.NS, w.NW, w.NY, w.NX), dtype=torch.complex64, device=dev)
-        res["vs_plain"] = plain_vs_fused(w, dims, sref, labels, u)
+        res["vs_plain"] = plain_vs_fused(w, dims, sref, labels, u, copy_gbs=copy_gbs)
         del u
     del slow, ds
     torch.cuda.empty_cache()
