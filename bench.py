@@ -63,11 +63,13 @@ def pass_kernel(name, w):
     (column kernels carry their register budget, row kernels the shot-map flag)."""
     from pyfwix_amd import autotune
     nx, ny = w.NX, w.NY
-    b = lambda kind, mode: autotune.get(kind, mode, ny)  # noqa: E731
-    return {"cols_fanout": f"fwix::k_cols_fanout<{ny}, 0, {b(0, 0)}>",
-            "cols_fanin": f"fwix::k_cols_fanin<{ny}, 1, {b(1, 1)}>",
-            "born_cols_kfac": f"fwix::k_cols_fanout<{ny}, 2, {b(0, 2)}>",
-            "born_cols_kfac_adj": f"fwix::k_cols_fanin<{ny}, 2, {b(1, 2)}>",
+    def col(kind, mode):
+        stem, wpe = ("k_cols_fanout", "k_cols_fanin")[kind], autotune.get(kind, mode, ny)
+        if autotune.get_layout(kind, mode, ny) == 1:
+            return f"fwix::{stem}_split<{mode}, {wpe}>"
+        return f"fwix::{stem}<{ny}, {mode}, {wpe}>"
+    return {"cols_fanout": col(0, 0), "cols_fanin": col(1, 1),
+            "born_cols_kfac": col(0, 2), "born_cols_kfac_adj": col(1, 2),
             "rows_select": f"fwix::k_rows_select<{nx}, false>", "rows_fanout": f"fwix::k_rows_fanout<{nx}, false>",
             "born_rows_scatter": f"fwix::k_rows_scatter<{nx}, true, false>",
             "born_rows_inv_scatter_adj": f"fwix::k_rows_scatter_adj<{nx}, true, false>",

@@ -350,6 +350,14 @@ int fwix_timing_report(char* buf, int len);
  * process-wide; pyfwix_amd/autotune.py sets it from timed launches. */
 int fwix_tune_columns_set(int kind, int mode, int n, int waves_per_simd);
 int fwix_tune_columns_get(int kind, int mode, int n);
+/* Layout of the 1024-point column transforms (kind, mode as above; no
+ * counterpart in the reference): 0 = one wave per column, 1 = two waves per
+ * column (a radix-2 split into two 512-point transforms, twice the waves per
+ * tile).  Any other n accepts only layout 0.  Results are identical up to
+ * float rounding.  The float64 scattering mode always uses layout 0.
+ * _set returns 0 or -1 for an invalid argument; _get the layout or -1. */
+int fwix_tune_columns_layout_set(int kind, int mode, int n, int layout);
+int fwix_tune_columns_layout_get(int kind, int mode, int n);
 
 /* Device copy of `bytes` (a multiple of 16, 16-byte aligned buffers) with
  * 16-byte loads and stores: the stream-copy kernel that measures the

@@ -23,7 +23,7 @@ import numpy as np
 
 from . import _lib
 
-__all__ = ["CANDIDATES", "get", "set_budget", "table", "tune_size", "save", "load"]
+__all__ = ["CANDIDATES", "get", "set_budget", "get_layout", "set_layout", "table", "tune_size", "save", "load"]
 
 CANDIDATES = (2, 4, 6)
 FANOUT, FANIN = 0, 1
@@ -41,10 +41,26 @@ def set_budget(kind: int, mode: int, n: int, wpe: int) -> None:
     _lib.call("fwix_tune_columns_set", int(kind), int(mode), int(n), int(wpe))
 
 
+def get_layout(kind: int, mode: int, n: int) -> int:
+    """Layout of the column transforms for (kind, mode, column length n): 0 one
+    wave per column, 1 two waves per column (n = 1024 only)."""
+    v = _lib.lib.fwix_tune_columns_layout_get(int(kind), int(mode), int(n))
+    if v < 0:
+        raise ValueError(f"no column kernel for kind={kind} mode={mode} n={n}")
+    return v
+
+
+def set_layout(kind: int, mode: int, n: int, layout: int) -> None:
+    _lib.call("fwix_tune_columns_layout_set", int(kind), int(mode), int(n), int(layout))
+
+
 def table(sizes=(96, 128, 192, 256, 384, 512, 768, 1024)) -> dict:
-    """{"n": {"fanout": [m0, m1, m2], "fanin": [m0, m1, m2]}} for the given sizes."""
+    """{"n": {"fanout": [m0, m1, m2], "fanin": [m0, m1, m2], "fanout_layout": [...],
+    "fanin_layout": [...]}} for the given sizes."""
     return {str(n): {"fanout": [get(FANOUT, m, n) for m in range(3)],
-                     "fanin": [get(FANIN, m, n) for m in range(3)]} for n in sizes}
+                     "fanin": [get(FANIN, m, n) for m in range(3)],
+                     "fanout_layout": [get_layout(FANOUT, m, n) for m in range(3)],
+                     "fanin_layout": [get_layout(FANIN, m, n) for m in range(3)]} for n in sizes}
 
 
 def save(path: str, sizes=(96, 128, 192, 256, 384, 512, 768, 1024)) -> None:
@@ -59,6 +75,8 @@ def load(path: str) -> None:
         for m in range(3):
             set_budget(FANOUT, m, int(n), v["fanout"][m])
             set_budget(FANIN, m, int(n), v["fanin"][m])
+            set_layout(FANOUT, m, int(n), v.get("fanout_layout", [0, 0, 0])[m])
+            set_layout(FANIN, m, int(n), v.get("fanin_layout", [0, 0, 0])[m])
 
 
 def _time_passes(run, names, reps):
@@ -88,8 +106,10 @@ def tune_size(n: int, planes: int | None = None, nref: int = 4, taylor: int = 1,
     labels (1, planes, n, n)) device tensors and ``freqs`` (planes values, Hz)
     tune on a real depth of a workload (which references are present in which
     rows changes what the fan-out / fan-in transforms do); by default the
-    labels are random.  Returns {"fanout": {wpe: us}, "fanin": {...},
-    "scatter_fanout": {...}, "scatter_fanin": {...}, "chosen": {...}}."""
+    labels are random.  At n = 1024 the two-wave layout is timed beside the
+    one-wave one.  Returns {"fanout": {config: us}, "fanin": {...},
+    "scatter_fanout": {...}, "scatter_fanin": {...}, "chosen": {...}}, config
+    "4" for one wave per column at 4 waves per SIMD, "split4" for two."""
     import torch
     from .ops import FWIX_PSPI, OneStep, Scatter, make_dims
     dev = torch.device(device or "cuda:0")
@@ -127,43 +147,54 @@ def tune_size(n: int, planes: int | None = None, nref: int = 4, taylor: int = 1,
         scat.forward(False, u, v)
         scat.adjoint(False, w, v)
 
-    before = {(k, m): get(k, m, n) for k in (FANOUT, FANIN) for m in range(3)}
+    # configurations: register budget, and at n = 1024 also the two-wave layout
+    # (keys "4" = one wave per column at 4 waves per SIMD, "split4" = two waves)
+    configs = [(0, w) for w in candidates]
+    if n == 1024:
+        configs += [(1, w) for w in candidates if w <= 4]
+    key = lambda lw: (f"split{lw[1]}" if lw[0] else str(lw[1]))  # noqa: E731
+    before = {(k, m): (get_layout(k, m, n), get(k, m, n)) for k in (FANOUT, FANIN) for m in range(3)}
+
+    def apply(k, m, lw):
+        set_layout(k, m, n, lw[0])
+        set_budget(k, m, n, lw[1])
+
     out = {"fanout": {}, "fanin": {}, "scatter_fanout": {}, "scatter_fanin": {}}
     try:
-        for wpe in candidates:
+        for lw in configs:
             for k in (FANOUT, FANIN):
                 for m in range(3):
-                    set_budget(k, m, n, wpe)
+                    apply(k, m, lw)
             t = _time_passes(pspi, ("cols_fanout", "cols_fanin"), reps)
-            out["fanout"][wpe] = t.get("cols_fanout")
-            out["fanin"][wpe] = t.get("cols_fanin")
+            out["fanout"][key(lw)] = t.get("cols_fanout")
+            out["fanin"][key(lw)] = t.get("cols_fanin")
             if taylor >= 1:
                 t = _time_passes(scatter, ("scatter_cols", "scatter_cols_adj"), reps)
-                out["scatter_fanout"][wpe] = t.get("scatter_cols")
-                out["scatter_fanin"][wpe] = t.get("scatter_cols_adj")
+                out["scatter_fanout"][key(lw)] = t.get("scatter_cols")
+                out["scatter_fanin"][key(lw)] = t.get("scatter_cols_adj")
     except Exception:
-        for (k, m), wpe in before.items():
-            set_budget(k, m, n, wpe)
+        for (k, m), lw in before.items():
+            apply(k, m, lw)
         raise
+
     def best(d, k, m):
-        """The budget in use unless another is faster by more than ``margin``."""
+        """The configuration in use unless another is faster by more than ``margin``."""
         cur = before[(k, m)]
-        if not d or any(x is None for x in d.values()) or cur not in d:
+        if not d or any(x is None for x in d.values()) or key(cur) not in d:
             return cur
-        w = min(d, key=d.get)
-        return w if d[w] < (1.0 - margin) * d[cur] else cur
+        w = min(configs, key=lambda lw: d[key(lw)])
+        return w if d[key(w)] < (1.0 - margin) * d[key(cur)] else cur
 
     chosen = {"fanout": best(out["fanout"], FANOUT, 0), "fanin": best(out["fanin"], FANIN, 1),
               "scatter_fanout": best(out["scatter_fanout"], FANOUT, 2),
               "scatter_fanin": best(out["scatter_fanin"], FANIN, 2)}
     for m in (0, 1):
-        set_budget(FANOUT, m, n, chosen["fanout"])
-        set_budget(FANIN, m, n, chosen["fanin"])
-    set_budget(FANOUT, 2, n, chosen["scatter_fanout"])
-    set_budget(FANIN, 2, n, chosen["scatter_fanin"])
-    out["chosen"] = chosen
-    out = {k: ({str(b): (None if t is None else round(t, 2)) for b, t in v.items()} if k != "chosen" else v)
-           for k, v in out.items()}
+        apply(FANOUT, m, chosen["fanout"])
+        apply(FANIN, m, chosen["fanin"])
+    apply(FANOUT, 2, chosen["scatter_fanout"])
+    apply(FANIN, 2, chosen["scatter_fanin"])
+    out = {k: {b: (None if t is None else round(t, 2)) for b, t in v.items()} for k, v in out.items()}
+    out["chosen"] = {k: key(v) for k, v in chosen.items()}
     del step, scat, u, v, w, slow
     torch.cuda.empty_cache()
     return out

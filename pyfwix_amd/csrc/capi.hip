@@ -750,6 +750,13 @@ int fwix_tune_columns_set(int kind, int mode, int n, int waves_per_simd) {
   });
 }
 int fwix_tune_columns_get(int kind, int mode, int n) { return col_wpe_get(kind, mode, n); }
+int fwix_tune_columns_layout_set(int kind, int mode, int n, int layout) {
+  return guard([&] {
+    if (col_layout_set(kind, mode, n, layout) != 0)
+      throw Error("tune_columns_layout: kind must be 0/1, mode 0..2, n a supported size, layout 0 (or 1 for n = 1024)");
+  });
+}
+int fwix_tune_columns_layout_get(int kind, int mode, int n) { return col_layout_get(kind, mode, n); }
 int fwix_copy(const void* in, void* out, size_t bytes, fwix_stream s) {
   return guard([&] {
     if (bytes % 16) throw Error("fwix_copy: bytes must be a multiple of 16");

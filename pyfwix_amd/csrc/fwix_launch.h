@@ -79,6 +79,10 @@ hipError_t launch_ic_adj(float2* img, const float2* D, const float2* U, int NX, 
 // fan-out / 1 fan-in, per mode (0, 1, 2) and column length; -1 if invalid.
 int col_wpe_get(int kind, int mode, int n);
 int col_wpe_set(int kind, int mode, int n, int wpe);
+// Column layout of the n-point transforms (kind, mode as above): 0 one wave
+// per column, 1 two waves per column (n = 1024 only); -1 if invalid.
+int col_layout_get(int kind, int mode, int n);
+int col_layout_set(int kind, int mode, int n, int layout);
 // out[k] += sum of g over the planes of shot k (first[k], count[k]); m per plane
 hipError_t launch_plane_sum(const float2* g, float2* out, const int* first, const int* count, int nshot, long long m,
                             hipStream_t st);

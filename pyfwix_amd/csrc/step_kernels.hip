@@ -44,8 +44,34 @@ struct ColCfg {
   static constexpr int TILE = N * CWP;
   static constexpr int PRIV = NB * padded(N);
   static constexpr int SMEM = TILE > WAVES * PRIV ? TILE : WAVES * PRIV;
+  static constexpr bool SPLIT = false;  // tile rows in natural order
   static_assert(NB >= 1 && V >= 4, "column tile too narrow");
 };
+// 1024-point columns on two waves each (k_cols_fanout_split / _fanin_split):
+// 512-thread workgroups, 4 columns, wave 2c + h holding rows 2i + h of
+// column c as a 512-point transform (V = 8 values per lane).  A loaded tile
+// keeps the even rows first and the odd rows after them (SPLIT), so each wave
+// reads its 512 rows with the stride of a 512-point column; results leave
+// through tile_store_r2.
+struct SplitCfg {
+  static constexpr int THREADS = 512;
+  static constexpr int WAVES = 8;
+  static constexpr int CW = 4;
+  static constexpr int NB = 1;
+  static constexpr int V = 8;
+  static constexpr int CWP = CW + 1;
+  static constexpr int NROW = 1024;
+  static constexpr int TILE = NROW * CWP;
+  static constexpr int PRIV = padded(512);
+  static constexpr int SMEM = TILE > WAVES * PRIV ? TILE : WAVES * PRIV;
+  static constexpr bool SPLIT = true;
+};
+// LDS row slot of tile row r in a loaded tile
+template <class C>
+__device__ __forceinline__ int tslot(int r) {
+  if constexpr (C::SPLIT) return (r & 1) * (C::NROW / 2) + (r >> 1);
+  else return r;
+}
 
 // Workgroups are handed to the 8 XCDs round-robin by linear id, and each
 // XCD has its own L2.  A column tile reads CW * 8 bytes of every row; with
@@ -409,7 +435,7 @@ struct TileRegs {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int e = threadIdx.x + k * C::THREADS;
-      tile[(e / C::CW) * C::CWP + e % C::CW] = t[k];
+      tile[tslot<C>(e / C::CW) * C::CWP + e % C::CW] = t[k];
     }
   }
 };
@@ -423,6 +449,7 @@ __device__ __forceinline__ void tile_load(float2* tile, const PlaneBuf& b, const
 template <class C>
 __device__ __forceinline__ void tile_store_masked(const float2* tile, const PlaneBuf& b, const TileOffs<C>& o,
                                                   const uint32_t* rmk, uint32_t bit) {
+  static_assert(!C::SPLIT, "split tiles are stored by tile_store_r2");
 #pragma unroll
   for (int k = 0; k < TileOffs<C>::PER; ++k) {
     const int e = threadIdx.x + k * C::THREADS;
@@ -432,6 +459,7 @@ __device__ __forceinline__ void tile_store_masked(const float2* tile, const Plan
 template <class C>
 __device__ __forceinline__ void tile_store(const float2* tile, const PlaneBuf& b, const TileOffs<C>& o, int add) {
   constexpr int PER = TileOffs<C>::PER;
+  static_assert(!C::SPLIT, "split tiles are stored by tile_store_r2");
   float2 t[PER];
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
@@ -698,6 +726,255 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   tile_store<C>(smem, PlaneBuf(a.K + pb, plane_bytes(a)), to, 0);
 }
 
+// ------------------------------------------------------------------ 1024-point columns, two waves each
+// Radix-2 across a wave pair.  Forward (decimation in time): wave h of column
+// c transforms x[2i + h] (512 points: E for h = 0, O for h = 1); then X[n] =
+// E[k] + W^k O[k] for n = k < 512 and E[k] - W^k O[k] for n = k + 512, W^k =
+// exp(-2 pi i k / 1024).  Each wave keeps the spectrum samples of one parity,
+// X[2j + h] at the positions j of a 512-point transform, so the multipliers
+// are evaluated once per sample.  Inverse: wave h transforms its samples
+// Y[2j + h] back (A for h = 0, B for h = 1, 512 points each) and the tile
+// store combines x[m] = A[m] + W^-m B[m], x[m + 512] = A[m] - W^-m B[m].
+struct SplitTw {
+  float2 h[512];  // exp(-2 pi i j / 512): the 512-point transforms
+  float2 c[512];  // exp(-2 pi i k / 1024), k < 512: the radix-2 combinations
+};
+__device__ __forceinline__ void split_tw_stage(SplitTw& tw, const float2* __restrict__ twy) {
+  for (int i = threadIdx.x; i < 512; i += SplitCfg::THREADS) {
+    tw.h[i] = twy[2 * i];
+    tw.c[i] = twy[i];
+  }
+}
+// Reference slownesses of the plane staged in LDS: the first kSplitSref
+// (more are read from global memory).  The bound keeps a workgroup's LDS
+// under a third of the CU's, so three workgroups fit where registers allow.
+constexpr int kSplitSref = 32;
+__device__ __forceinline__ void split_sref_stage(float2* srl, const KArgs& a, int iw) {
+  const int nt = a.nterm < kSplitSref ? a.nterm : kSplitSref;
+  for (int i = threadIdx.x; i < nt; i += SplitCfg::THREADS) srl[i] = a.sref[i * a.NW + iw];
+}
+// this wave's 512 samples (rows 2i + h of column cl) at the first-stage positions
+template <int R>
+__device__ __forceinline__ void split_read(float2* v, const float2* tile, int cl, int h, int lane) {
+#pragma unroll
+  for (int e = 0; e < SplitCfg::V; ++e) {
+    int b, i;
+    edge_pos<512, 1, R, false>(lane, e, b, i);
+    v[e] = tile[(h * 512 + i) * SplitCfg::CWP + cl];
+  }
+}
+// this wave's 512 values at positions R to tile slots 512 h + i of column cl
+template <int R>
+__device__ __forceinline__ void split_write(float2* tile, const float2* v, int cl, int h, int lane) {
+#pragma unroll
+  for (int e = 0; e < SplitCfg::V; ++e) {
+    int b, i;
+    edge_pos<512, 1, R, false>(lane, e, b, i);
+    tile[(h * 512 + i) * SplitCfg::CWP + cl] = v[e];
+  }
+}
+// spectrum index of value e of this wave at positions R
+template <int R>
+__device__ __forceinline__ int split_n(int lane, int e, int h) {
+  int b, i;
+  edge_pos<512, 1, R, false>(lane, e, b, i);
+  return 2 * i + h;
+}
+// X[n] from E (tile slots k) and O (slots 512 + k), k = n mod 512
+__device__ __forceinline__ float2 split_spec(const float2* tile, const SplitTw& tw, int cl, int n) {
+  const int k = n & 511;
+  const float2 E = tile[k * SplitCfg::CWP + cl];
+  const float2 O = cmul(tile[(512 + k) * SplitCfg::CWP + cl], tw.c[k]);
+  return n < 512 ? cadd(E, O) : csub(E, O);
+}
+// Store the tile's inverse transforms: A at slots m, B at slots 512 + m of
+// each column.  Element k of this thread sits in row m = threadIdx.x / CW +
+// k * (THREADS / CW): elements k and k + 4 are rows m and m + 512.  With rmk,
+// only rows whose mask has `bit` set are stored.
+__device__ __forceinline__ void tile_store_r2(const float2* tile, const SplitTw& tw, const PlaneBuf& b,
+                                              const TileOffs<SplitCfg>& o, const uint32_t* rmk, uint32_t bit) {
+  using C = SplitCfg;
+  static_assert(TileOffs<C>::PER == 8 && 4 * (C::THREADS / C::CW) == 512, "rows k and k + 4 must be m and m + 512");
+  const int c = (int)threadIdx.x % C::CW;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int m = (int)threadIdx.x / C::CW + k * (C::THREADS / C::CW);
+    const float2 A = tile[m * C::CWP + c];
+    const float2 B = cmulc(tile[(512 + m) * C::CWP + c], tw.c[m]);
+    unsigned olo = o.at(k), ohi = o.at(k + 4);
+    if (rmk) {
+      if (!(row_mask_of<C>(rmk, k) & bit)) olo = kOff;
+      if (!(row_mask_of<C>(rmk, k + 4) & bit)) ohi = kOff;
+    }
+    b.store(cadd(A, B), olo);
+    b.store(csub(A, B), ohi);
+  }
+}
+// k^2 at spectrum indices n of column colw
+template <int R>
+__device__ __forceinline__ void split_k2(float* k2v, const KArgs& a, int colw, int h, int lane) {
+  const float kxv = a.kx[colw < a.NX ? colw : 0];
+#pragma unroll
+  for (int e = 0; e < SplitCfg::V; ++e) {
+    const float kyv = a.ky[split_n<R>(lane, e, h)];
+    k2v[e] = fmaf(kxv, kxv, kyv * kyv);
+  }
+}
+
+// cols_fanout for 1024-point columns: W_r = F_y^-1[M_r * F_y(K)] / NY
+template <int MODE, int WPE>
+__global__ __launch_bounds__(SplitCfg::THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_cols_fanout_split(KArgs a) {
+  using C = SplitCfg;
+  using F = WaveFft<512, 1, false, false, PadLds<512>>;
+  using Fi = WaveFft<512, 1, false, true, PadLds<512>>;
+  static_assert(F::P::last() == Fi::P::first(), "register hand-off needs mirrored plans");
+  constexpr int RS = F::P::last();
+  __shared__ float2 smem[C::SMEM];
+  __shared__ SplitTw tw;
+  __shared__ float2 srl[kSplitSref];
+  __shared__ uint32_t rmk[C::NROW];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cl = wave >> 1, h = wave & 1;
+  int p, c0;
+  col_tile<C>(p, c0);
+  const size_t pb = (size_t)p * a.NX * C::NROW;
+  float2* priv = smem + wave * C::PRIV;
+  const TileOffs<C> to(a.NX, c0);
+  {
+    TileRegs<C> t;
+    t.fetch(PlaneBuf(a.K + pb, plane_bytes(a)), to);
+    t.commit(smem);
+  }
+  const int iw = plane_iw(a, p);
+  split_tw_stage(tw, a.twy);
+  if constexpr (MODE != 2) split_sref_stage(srl, a, iw);
+  const int im = model_iw_any(a, iw);
+  const uint32_t* rm = (MODE != 2 && a.rowmask) ? a.rowmask + im * a.NY : nullptr;
+  const uint32_t pm = (MODE != 2 && a.planemask) ? a.planemask[im] : 0xffffffffu;
+  tile_row_masks<C>(rmk, rm);
+  __syncthreads();
+  float2 xs[C::V];
+  split_read<F::P::first()>(xs, smem, cl, h, lane);
+  __syncthreads();
+  F::run(xs, priv, tw.h, lane);
+  __syncthreads();
+  split_write<RS>(smem, xs, cl, h, lane);
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) xs[e] = split_spec(smem, tw, cl, split_n<RS>(lane, e, h));
+  float k2v[C::V];
+  split_k2<RS>(k2v, a, c0 + cl, h, lane);
+  __syncthreads();  // the transforms' exchange regions overlap the tile
+  const float w2 = a.w2[iw];
+  const float s2 = a.scale;  // 1/NY: forward and inverse column normalisation together
+  for (int r = 0; r < a.nterm; ++r) {
+    if (r < 32 && !((pm >> r) & 1u)) continue;  // reference absent from the plane: W_r is never read
+    float2 srv = make_float2(0.f, 0.f);
+    if constexpr (MODE != 2) srv = r < kSplitSref ? srl[r] : a.sref[r * a.NW + iw];
+    const TermMult<MODE> tm(a, r, w2, srv);
+    float2 y[C::V];
+    auto mult = [&](auto real) {
+#pragma unroll
+      for (int e = 0; e < C::V; ++e) y[e] = cscale(cmul(xs[e], tm.template at<decltype(real)::value>(k2v[e])), s2);
+    };
+    if (tm.real) mult(std::true_type{});
+    else mult(std::false_type{});
+    Fi::run(y, priv, tw.h, lane);
+    __syncthreads();
+    split_write<Fi::P::last()>(smem, y, cl, h, lane);
+    __syncthreads();
+    const PlaneBuf wb(a.W + (size_t)r * a.wstride + pb, plane_bytes(a));
+    tile_store_r2(smem, tw, wb, to, rm ? rmk : nullptr, r < 32 ? 1u << r : 0xffffffffu);
+    __syncthreads();
+  }
+}
+
+// cols_fanin for 1024-point columns: K = F_y^-1[sum_r M_r * F_y(W_r)] / NY
+template <int MODE, int WPE>
+__global__ __launch_bounds__(SplitCfg::THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_cols_fanin_split(KArgs a) {
+  using C = SplitCfg;
+  using F = WaveFft<512, 1, false, false, PadLds<512>>;
+  using Fi = WaveFft<512, 1, false, true, PadLds<512>>;
+  static_assert(F::P::last() == Fi::P::first(), "register hand-off needs mirrored plans");
+  constexpr int RS = F::P::last();
+  __shared__ float2 smem[C::SMEM];
+  __shared__ SplitTw tw;
+  __shared__ float2 srl[kSplitSref];
+  __shared__ uint32_t rmk[C::NROW];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cl = wave >> 1, h = wave & 1;
+  int p, c0;
+  col_tile<C>(p, c0);
+  const size_t pb = (size_t)p * a.NX * C::NROW;
+  float2* priv = smem + wave * C::PRIV;
+  const TileOffs<C> to(a.NX, c0);
+  const int iw = plane_iw(a, p);
+  const float w2 = a.w2[iw];
+  const int im = model_iw_any(a, iw);
+  const uint32_t* rm = (MODE != 2 && a.rowmask) ? a.rowmask + im * a.NY : nullptr;
+  const uint32_t pm = (MODE != 2 && a.planemask) ? a.planemask[im] : 0xffffffffu;
+  const uint32_t live = a.nterm >= 32 ? pm : (pm & ((1u << a.nterm) - 1u));
+  const bool masked = rm != nullptr;
+  const int rend = masked ? 32 : a.nterm;
+  tile_row_masks<C>(rmk, rm);
+  split_tw_stage(tw, a.twy);
+  if constexpr (MODE != 2) split_sref_stage(srl, a, iw);
+  __syncthreads();
+  float k2v[C::V];
+  split_k2<RS>(k2v, a, c0 + cl, h, lane);
+  float2 acc[C::V];  // Y[2j + h]
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) acc[e] = czero();
+  auto fetch = [&](TileRegs<C>& t, int rr) {
+    const PlaneBuf wb(a.W + (size_t)rr * a.wstride + pb, plane_bytes(a));
+    if (masked) t.fetch_masked(wb, to, rmk, 1u << rr);
+    else t.fetch(wb, to);
+  };
+  // the next term's tile is fetched into registers during this term's
+  // transform where the budget holds it (2 waves per SIMD); at 4 the other
+  // workgroup on the CU hides the load instead
+  constexpr bool PREFETCH = WPE <= 2;
+  int r = masked ? next_bit(live, -1) : 0;
+  TileRegs<C> tile;
+  if (PREFETCH && r < rend) fetch(tile, r);
+  while (r < rend) {
+    if constexpr (!PREFETCH) fetch(tile, r);
+    tile.commit(smem);
+    __syncthreads();
+    float2 v[C::V];
+    split_read<F::P::first()>(v, smem, cl, h, lane);
+    const int rn = masked ? next_bit(live, r) : r + 1;
+    if (PREFETCH && rn < rend) fetch(tile, rn);
+    __syncthreads();
+    F::run(v, priv, tw.h, lane);
+    __syncthreads();
+    split_write<RS>(smem, v, cl, h, lane);
+    __syncthreads();
+    float2 srv = make_float2(0.f, 0.f);
+    if constexpr (MODE != 2) srv = r < kSplitSref ? srl[r] : a.sref[r * a.NW + iw];
+    const TermMult<MODE> tm(a, r, w2, srv);
+    auto mult = [&](auto real) {
+#pragma unroll
+      for (int e = 0; e < C::V; ++e) {
+        const float2 X = split_spec(smem, tw, cl, split_n<RS>(lane, e, h));
+        acc[e] = cadd(acc[e], cmul(X, tm.template at<decltype(real)::value>(k2v[e])));
+      }
+    };
+    if (tm.real) mult(std::true_type{});
+    else mult(std::false_type{});
+    __syncthreads();
+    r = rn;
+  }
+  const float s2 = a.scale;
+#pragma unroll
+  for (int e = 0; e < C::V; ++e) acc[e] = cscale(acc[e], s2);
+  Fi::run(acc, priv, tw.h, lane);
+  __syncthreads();
+  split_write<Fi::P::last()>(smem, acc, cl, h, lane);
+  __syncthreads();
+  tile_store_r2(smem, tw, PlaneBuf(a.K + pb, plane_bytes(a)), to, nullptr, 0);
+}
+
 // ------------------------------------------------------------------ launchers
 #define FWIX_SIZES(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
 
@@ -863,6 +1140,20 @@ int col_wpe_get(int kind, int mode, int n) {
   wpe_init();
   return g_wpe[kind][mode][i];
 }
+// Layout of the 1024-point column transforms, per kind (0 fan-out, 1 fan-in)
+// and mode: 0 one wave per column (256-thread workgroups, 4 columns), 1 two
+// waves per column (512-thread workgroups, 4 columns, radix-2 split).
+static int g_layout[2][3] = {{1, 1, 1}, {1, 1, 1}};
+int col_layout_get(int kind, int mode, int n) {
+  if (kind < 0 || kind > 1 || mode < 0 || mode > 2 || size_slot(n) < 0) return -1;
+  return n == 1024 ? g_layout[kind][mode] : 0;
+}
+int col_layout_set(int kind, int mode, int n, int layout) {
+  if (kind < 0 || kind > 1 || mode < 0 || mode > 2 || size_slot(n) < 0) return -1;
+  if (layout != 0 && !(layout == 1 && n == 1024)) return -1;
+  if (n == 1024) g_layout[kind][mode] = layout;
+  return 0;
+}
 int col_wpe_set(int kind, int mode, int n, int wpe) {
   const int i = size_slot(n);
   if (kind < 0 || kind > 1 || mode < 0 || mode > 2 || i < 0) return -1;
@@ -887,6 +1178,16 @@ hipError_t launch_cols_fanout(int mode, const KArgs& a, hipStream_t st) {
   return dispatch_n(a.NY, [&](auto nc) -> hipError_t {
     constexpr int N = decltype(nc)::value;
     const dim3 g = col_grid(N, a.NX, a.np);
+    if constexpr (N == 1024) {
+      if (!a.f64 && col_layout_get(0, mode, N) == 1)
+        return dispatch_wpe(col_wpe_get(0, mode, N), [&](auto wc) -> hipError_t {
+          constexpr int W = decltype(wc)::value;
+          if (mode == 0) hipLaunchKernelGGL((k_cols_fanout_split<0, W>), g, dim3(SplitCfg::THREADS), 0, st, a);
+          else if (mode == 1) hipLaunchKernelGGL((k_cols_fanout_split<1, W>), g, dim3(SplitCfg::THREADS), 0, st, a);
+          else hipLaunchKernelGGL((k_cols_fanout_split<2, W>), g, dim3(SplitCfg::THREADS), 0, st, a);
+          return hipGetLastError();
+        });
+    }
     if (a.f64) {
       if (mode == 0) hipLaunchKernelGGL((k_cols_fanout<N, 0, 2, double2>), g, dim3(col_threads(N)), 0, st, a);
       else if (mode == 1) hipLaunchKernelGGL((k_cols_fanout<N, 1, 2, double2>), g, dim3(col_threads(N)), 0, st, a);
@@ -907,6 +1208,16 @@ hipError_t launch_cols_fanin(int mode, const KArgs& a, hipStream_t st) {
   return dispatch_n(a.NY, [&](auto nc) -> hipError_t {
     constexpr int N = decltype(nc)::value;
     const dim3 g = col_grid(N, a.NX, a.np);
+    if constexpr (N == 1024) {
+      if (!a.f64 && col_layout_get(1, mode, N) == 1)
+        return dispatch_wpe(col_wpe_get(1, mode, N), [&](auto wc) -> hipError_t {
+          constexpr int W = decltype(wc)::value;
+          if (mode == 0) hipLaunchKernelGGL((k_cols_fanin_split<0, W>), g, dim3(SplitCfg::THREADS), 0, st, a);
+          else if (mode == 1) hipLaunchKernelGGL((k_cols_fanin_split<1, W>), g, dim3(SplitCfg::THREADS), 0, st, a);
+          else hipLaunchKernelGGL((k_cols_fanin_split<2, W>), g, dim3(SplitCfg::THREADS), 0, st, a);
+          return hipGetLastError();
+        });
+    }
     if (a.f64) {
       if (mode == 0) hipLaunchKernelGGL((k_cols_fanin<N, 0, 2, double2>), g, dim3(col_threads(N)), 0, st, a);
       else if (mode == 1) hipLaunchKernelGGL((k_cols_fanin<N, 1, 2, double2>), g, dim3(col_threads(N)), 0, st, a);

@@ -127,3 +127,14 @@ def test_column_budget_table():
             autotune.set_budget(*bad)
     with pytest.raises(ValueError):
         autotune.get(0, 0, 500)
+    # column layouts: two waves per column only for 1024-point columns
+    assert autotune.get_layout(0, 0, 512) == 0 and autotune.get_layout(1, 2, 768) == 0
+    old = autotune.get_layout(1, 1, 1024)
+    try:
+        autotune.set_layout(1, 1, 1024, 1 - old)
+        assert autotune.get_layout(1, 1, 1024) == 1 - old and autotune.table((1024,))["1024"]["fanin_layout"][1] == 1 - old
+    finally:
+        autotune.set_layout(1, 1, 1024, old)
+    for bad in ((0, 0, 512, 1), (2, 0, 1024, 1), (0, 3, 1024, 0), (0, 0, 1024, 2), (0, 0, 500, 0)):
+        with pytest.raises(RuntimeError, match="tune_columns_layout"):
+            autotune.set_layout(*bad)

@@ -66,11 +66,84 @@ def test_tune_size_sets_a_candidate(dev):
         for key in ("fanout", "fanin", "scatter_fanout", "scatter_fanin"):
             assert set(r[key]) == {str(w) for w in autotune.CANDIDATES}
             assert all(t is not None and t > 0 for t in r[key].values())
-            assert r["chosen"][key] in autotune.CANDIDATES
+            assert r["chosen"][key] in {str(w) for w in autotune.CANDIDATES}
             # the fastest, or the budget in use when nothing beats it by the margin
-            assert r[key][str(r["chosen"][key])] == min(r[key].values()) or r["chosen"][key] == before[
-                (0 if "fanout" in key else 1, 2 if "scatter" in key else (0 if key == "fanout" else 1))]
-        assert autotune.get(0, 0, n) == r["chosen"]["fanout"] and autotune.get(1, 2, n) == r["chosen"]["scatter_fanin"]
+            assert r[key][r["chosen"][key]] == min(r[key].values()) or r["chosen"][key] == str(before[
+                (0 if "fanout" in key else 1, 2 if "scatter" in key else (0 if key == "fanout" else 1))])
+        assert autotune.get(0, 0, n) == int(r["chosen"]["fanout"])
+        assert autotune.get(1, 2, n) == int(r["chosen"]["scatter_fanin"])
     finally:
         for (k, m), wpe in before.items():
             autotune.set_budget(k, m, n, wpe)
+
+
+@pytest.mark.parametrize("nref", [4, 40])
+def test_1024_column_layouts_match_oracle(dev, nref):
+    """1024-point columns: one wave per column and the two-wave radix-2 split
+    give the oracle's steps and scattering terms, and each other's to float
+    rounding (references absent from rows and planes, more references than
+    the split kernels stage in LDS, every budget the split is built for)."""
+    import torch
+    import oracle as O
+    from fwix_testutil import TOL_WFLD, dims_of, grid, smooth_slowness
+    from pyfwix_amd import autotune
+    from pyfwix_amd.ops import FWIX_NSPS, FWIX_PSPI, OneStep, Scatter
+    g = grid(48, 1024, 3, 1, nx=40, ny=1000, nz=2)
+    rng = np.random.default_rng(1024)
+    sref = (1.0 / rng.uniform(1500, 3000, (2, nref, g.NW)) ** 2).astype(np.complex64)
+    labels = np.zeros((2, g.NW, g.NY, g.NX), np.uint8)
+    labels[1, 0] = rng.integers(0, nref, (g.NY, g.NX))
+    labels[1, 1, :400] = nref - 1                           # rows with the last reference alone
+    labels[1, 2] = rng.choice([0, nref // 2], (g.NY, g.NX))  # most references absent
+    slow = smooth_slowness(rng, g, imag=0.01)
+    shape = (g.NS, g.NW, g.NY, g.NX)
+    x, y = crand(rng, shape), crand(rng, shape)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a.astype(np.complex64))).to(dev)  # noqa: E731
+    s64, lab = sref[1].astype(np.complex128), labels[1].astype(np.int64)
+    refs = {}
+    for name, method in (("pspi", FWIX_PSPI), ("nsps", FWIX_NSPS)):
+        tap = 4 if name == "pspi" else 0
+        fwd = O.pspi_forward if name == "pspi" else O.nsps_forward
+        adj = O.pspi_adjoint if name == "pspi" else O.nsps_adjoint
+        kw = dict(tx=tap, ty=tap) if name == "pspi" else {}
+        refs[name] = (fwd(x, s64, lab, g, g.dz, 0.0, **kw), adj(y, s64, lab, g, g.dz, 0.0, **kw))
+    sl2 = slow[1].astype(np.complex64).astype(np.complex128)
+    refs["scatter"] = (O.scatter_forward(x, sl2, g, g.dz, 0.04, 2), O.scatter_adjoint(y, sl2, g, g.dz, 0.04, 2))
+    before = {(k, m): (autotune.get_layout(k, m, 1024), autotune.get(k, m, 1024)) for k in (0, 1) for m in range(3)}
+    outs = {}
+    try:
+        for lw in [(0, 2), (1, 2), (1, 4)]:
+            for k in (0, 1):
+                for m in range(3):
+                    autotune.set_layout(k, m, 1024, lw[0])
+                    autotune.set_budget(k, m, 1024, lw[1])
+            res = {}
+            for name, method in (("pspi", FWIX_PSPI), ("nsps", FWIX_NSPS)):
+                op = OneStep(method, dims_of(g), nref=nref, taperx=4 if name == "pspi" else 0,
+                             tapery=4 if name == "pspi" else 0)
+                op.set_refs(t(sref), torch.from_numpy(labels).to(dev))
+                op.set_depth(1)
+                f, a = torch.zeros(shape, dtype=torch.complex64, device=dev), torch.zeros(shape, dtype=torch.complex64,
+                                                                                        device=dev)
+                op.forward(False, t(x), f)
+                op.adjoint(False, a, t(y))
+                res[name] = (f.cpu().numpy(), a.cpu().numpy())
+            sc = Scatter(dims_of(g), taylor=2, eps_scat=0.04)
+            sc.set_slow(t(slow))
+            sc.set_depth(1)
+            f, a = torch.zeros(shape, dtype=torch.complex64, device=dev), torch.zeros(shape, dtype=torch.complex64,
+                                                                                    device=dev)
+            sc.forward(False, t(x), f)
+            sc.adjoint(False, a, t(y))
+            res["scatter"] = (f.cpu().numpy(), a.cpu().numpy())
+            outs[lw] = res
+    finally:
+        for (k, m), lw in before.items():
+            autotune.set_layout(k, m, 1024, lw[0])
+            autotune.set_budget(k, m, 1024, lw[1])
+    for lw, res in outs.items():
+        for name, (f, a) in res.items():
+            assert rel(f, refs[name][0]) < TOL_WFLD, (lw, name, "forward")
+            assert rel(a, refs[name][1]) < TOL_WFLD, (lw, name, "adjoint")
+            assert rel(f, outs[(0, 2)][name][0]) < 2e-6, (lw, name, "forward vs one wave")
+            assert rel(a, outs[(0, 2)][name][1]) < 2e-6, (lw, name, "adjoint vs one wave")
