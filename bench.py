@@ -39,6 +39,10 @@ Profiling helpers (the summaries under profiles/ are made this way):
     --summarize-traffic FETCH WRITE OUT   counter_collection.csv of two
                                      `rocprofv3 --pmc` runs -> markdown OUT,
                                      read back by the bench for `traffic`
+    --summarize-valu SQ STATS OUT    counter_collection.csv of a `rocprofv3
+                                     --pmc SQ_...` run + kernel_stats.csv of
+                                     the same workload -> markdown OUT, read
+                                     back by the bench for `valu_issue`
 """
 from __future__ import annotations
 
@@ -140,16 +144,61 @@ def summarize_traffic(fetch_csv, write_csv, out_path, command_fetch, command_wri
     open(out_path, "w").write("\n".join(lines) + "\n")
 
 
-def traffic_for(config, kernel):
-    """Scaled HBM bytes per dispatch of `kernel` from profiles/traffic_<config>.md, or None."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{config}.md")
+def summarize_valu(sq_csv, stats_csv, out_path, command, command_stats, title, top=12):
+    """VALU issue table from a `rocprofv3 --pmc SQ_INSTS_VALU SQ_WAVES SQ_INSTS_VMEM_RD
+    SQ_INSTS_VMEM_WR SQ_INSTS_LDS SQ_INSTS_SALU` run and the kernel times of a
+    `--kernel-trace --stats` run of the same workload (read back by the bench
+    for `valu_issue`)."""
+    names = ("SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR")
+    cnt = {c: _counter_means(sq_csv, c) for c in names}
+    times = {_short(r["Name"]): (float(r["AverageNs"]) / 1e3, float(r["TotalDurationNs"]))
+             for r in _csv_rows(stats_csv)}
+    lines = [f"# VALU issue, {title}", "", "Counter run and kernel-time run (from the repository root):", "",
+             "```", command, command_stats, "```", "",
+             f"Peak VALU issue of the code as built (non-packed FP32; packed FP32 is switched off): one wave64 "
+             f"instruction per cycle per CU (4 SIMD16 units, 4 cycles each), 256 CUs x 2.4 GHz = "
+             f"{VALU_PEAK_GINSTR} G wave-instructions/s. `issue frac` = VALU wave-instructions per dispatch / "
+             f"(avg kernel time x {VALU_PEAK_GINSTR} G/s); transcendental instructions (sin, cos, sqrt, exp, rcp) "
+             f"issue at a quarter of that rate, so cycle occupancy is somewhat higher than the fraction shown.", "",
+             "| kernel | dispatches | VALU wave-instr / dispatch | LDS instr / dispatch | VMEM rd / wr per dispatch | "
+             "avg us | issue frac |", "|---|---|---|---|---|---|---|"]
+    rows = [k for k in cnt["SQ_INSTS_VALU"] if k in times]
+    rows.sort(key=lambda k: -times[k][1])
+    for k in rows[:top]:
+        n, valu = cnt["SQ_INSTS_VALU"][k]
+        avg_us = times[k][0]
+        lds = cnt["SQ_INSTS_LDS"].get(k, (0, 0.0))[1]
+        rd = cnt["SQ_INSTS_VMEM_RD"].get(k, (0, 0.0))[1]
+        wr = cnt["SQ_INSTS_VMEM_WR"].get(k, (0, 0.0))[1]
+        frac = valu / (avg_us * 1e-6) / (VALU_PEAK_GINSTR * 1e9)
+        lines.append(f"| `{k}` | {n} | {valu:.0f} | {lds:.0f} | {rd:.0f} / {wr:.0f} | {avg_us:.2f} | {frac:.3f} |")
+    open(out_path, "w").write("\n".join(lines) + "\n")
+
+
+def _profile_row(path, kernel, ncells, col):
+    """Cell `col` of the row of `kernel` in a profile table, or None.  A column
+    kernel whose register budget differs from the profiled one matches the
+    row of the same kernel at another budget."""
     if kernel is None or not os.path.exists(path):
         return None
+    rows = {}
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0] == f"`{kernel}`":
-            return float(cells[4])
+        if len(cells) == ncells and cells[0].startswith("`"):
+            rows[cells[0].strip("`")] = cells[col]
+    if kernel in rows:
+        return float(rows[kernel].split("/")[0])
+    if "k_cols_" in kernel:
+        stem = kernel.rsplit(",", 1)[0]
+        for k, v in rows.items():
+            if k.rsplit(",", 1)[0] == stem:
+                return float(v.split("/")[0])
     return None
+
+
+def traffic_for(config, kernel):
+    """Scaled HBM bytes per dispatch of `kernel` from profiles/traffic_<config>.md, or None."""
+    return _profile_row(os.path.join(ROOT, "profiles", f"traffic_{config}.md"), kernel, 5, 4)
 
 
 VALU_PEAK_GINSTR = 614.4   # wave64 VALU instructions per second (1e9): 256 CUs x 1 per cycle x 2.4 GHz
@@ -157,14 +206,7 @@ VALU_PEAK_GINSTR = 614.4   # wave64 VALU instructions per second (1e9): 256 CUs 
 
 def valu_for(config, kernel):
     """VALU wave-instructions per dispatch of `kernel` from profiles/valu_<config>.md, or None."""
-    path = os.path.join(ROOT, "profiles", f"valu_{config}.md")
-    if kernel is None or not os.path.exists(path):
-        return None
-    for line in open(path):
-        cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 7 and cells[0] == f"`{kernel}`":
-            return float(cells[2])
-    return None
+    return _profile_row(os.path.join(ROOT, "profiles", f"valu_{config}.md"), kernel, 7, 2)
 
 
 def _step_alg_bytes(w, steps_bg, steps_fwd, steps_adj):
@@ -909,6 +951,7 @@ def main():
     ap.add_argument("--shots-per-step", type=int, default=1, help="c5: shots per gradient step (one launch)")
     ap.add_argument("--summarize-stats", nargs=2, metavar=("CSV", "OUT"))
     ap.add_argument("--summarize-traffic", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT"))
+    ap.add_argument("--summarize-valu", nargs=3, metavar=("SQ_CSV", "STATS_CSV", "OUT"))
     ap.add_argument("--command", action="append", default=[], help="profiling command recorded in a summary")
     ap.add_argument("--title", default=None, help="workload named in a summary's title")
     args = ap.parse_args()
@@ -927,6 +970,11 @@ def main():
         cmds = args.command + ["", ""]
         kw = {"title": args.title} if args.title else {}
         summarize_traffic(*args.summarize_traffic, command_fetch=cmds[0], command_write=cmds[1], **kw)
+        return
+    if args.summarize_valu:
+        cmds = args.command + ["", ""]
+        summarize_valu(*args.summarize_valu, command=cmds[0], command_stats=cmds[1],
+                       title=args.title or args.config)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); shots and "

@@ -745,13 +745,15 @@ __device__ __forceinline__ void split_tw_stage(SplitTw& tw, const float2* __rest
     tw.c[i] = twy[i];
   }
 }
-// Reference slownesses of the plane staged in LDS: the first kSplitSref
-// (more are read from global memory).  The bound keeps a workgroup's LDS
-// under a third of the CU's, so three workgroups fit where registers allow.
-constexpr int kSplitSref = 32;
-__device__ __forceinline__ void split_sref_stage(float2* srl, const KArgs& a, int iw) {
-  const int nt = a.nterm < kSplitSref ? a.nterm : kSplitSref;
-  for (int i = threadIdx.x; i < nt; i += SplitCfg::THREADS) srl[i] = a.sref[i * a.NW + iw];
+// Reference slowness r of the plane, by a scalar load (through a
+// constant-address-space pointer): that load waits on the scalar counter
+// only, never on the vector-memory counter that the tile stores and the
+// prefetched tile hold, and every workgroup of the plane reads the same few
+// values (scalar-cache hits).
+__device__ __forceinline__ float2 split_sref(const KArgs& a, int r, int iw) {
+  using CP = const __attribute__((address_space(4))) float*;
+  const CP p = (CP)(const void*)a.sref + 2 * (r * a.NW + iw);
+  return make_float2(p[0], p[1]);
 }
 // this wave's 512 samples (rows 2i + h of column cl) at the first-stage positions
 template <int R>
@@ -831,7 +833,6 @@ __global__ __launch_bounds__(SplitCfg::THREADS) __attribute__((amdgpu_waves_per_
   constexpr int RS = F::P::last();
   __shared__ float2 smem[C::SMEM];
   __shared__ SplitTw tw;
-  __shared__ float2 srl[kSplitSref];
   __shared__ uint32_t rmk[C::NROW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int cl = wave >> 1, h = wave & 1;
@@ -847,7 +848,6 @@ __global__ __launch_bounds__(SplitCfg::THREADS) __attribute__((amdgpu_waves_per_
   }
   const int iw = plane_iw(a, p);
   split_tw_stage(tw, a.twy);
-  if constexpr (MODE != 2) split_sref_stage(srl, a, iw);
   const int im = model_iw_any(a, iw);
   const uint32_t* rm = (MODE != 2 && a.rowmask) ? a.rowmask + im * a.NY : nullptr;
   const uint32_t pm = (MODE != 2 && a.planemask) ? a.planemask[im] : 0xffffffffu;
@@ -870,7 +870,7 @@ __global__ __launch_bounds__(SplitCfg::THREADS) __attribute__((amdgpu_waves_per_
   for (int r = 0; r < a.nterm; ++r) {
     if (r < 32 && !((pm >> r) & 1u)) continue;  // reference absent from the plane: W_r is never read
     float2 srv = make_float2(0.f, 0.f);
-    if constexpr (MODE != 2) srv = r < kSplitSref ? srl[r] : a.sref[r * a.NW + iw];
+    if constexpr (MODE != 2) srv = split_sref(a, r, iw);
     const TermMult<MODE> tm(a, r, w2, srv);
     float2 y[C::V];
     auto mult = [&](auto real) {
@@ -899,7 +899,6 @@ __global__ __launch_bounds__(SplitCfg::THREADS) __attribute__((amdgpu_waves_per_
   constexpr int RS = F::P::last();
   __shared__ float2 smem[C::SMEM];
   __shared__ SplitTw tw;
-  __shared__ float2 srl[kSplitSref];
   __shared__ uint32_t rmk[C::NROW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int cl = wave >> 1, h = wave & 1;
@@ -918,7 +917,6 @@ __global__ __launch_bounds__(SplitCfg::THREADS) __attribute__((amdgpu_waves_per_
   const int rend = masked ? 32 : a.nterm;
   tile_row_masks<C>(rmk, rm);
   split_tw_stage(tw, a.twy);
-  if constexpr (MODE != 2) split_sref_stage(srl, a, iw);
   __syncthreads();
   float k2v[C::V];
   split_k2<RS>(k2v, a, c0 + cl, h, lane);
@@ -951,7 +949,7 @@ __global__ __launch_bounds__(SplitCfg::THREADS) __attribute__((amdgpu_waves_per_
     split_write<RS>(smem, v, cl, h, lane);
     __syncthreads();
     float2 srv = make_float2(0.f, 0.f);
-    if constexpr (MODE != 2) srv = r < kSplitSref ? srl[r] : a.sref[r * a.NW + iw];
+    if constexpr (MODE != 2) srv = split_sref(a, r, iw);
     const TermMult<MODE> tm(a, r, w2, srv);
     auto mult = [&](auto real) {
 #pragma unroll
