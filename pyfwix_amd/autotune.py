@@ -151,7 +151,7 @@ def tune_size(n: int, planes: int | None = None, nref: int = 4, taylor: int = 1,
     # (keys "4" = one wave per column at 4 waves per SIMD, "split4" = two waves)
     configs = [(0, w) for w in candidates]
     if n == 1024:
-        configs += [(1, w) for w in candidates if w <= 4]
+        configs += [(1, w) for w in candidates]
     key = lambda lw: (f"split{lw[1]}" if lw[0] else str(lw[1]))  # noqa: E731
     before = {(k, m): (get_layout(k, m, n), get(k, m, n)) for k in (FANOUT, FANIN) for m in range(3)}
 

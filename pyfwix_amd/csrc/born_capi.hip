@@ -192,9 +192,9 @@ struct ScatterOp : fwix_op_s {
       const double pb = 8.0 * np * plane;
       if (!adjoint) {  // Scatter.cpp:90-102
         if (T >= 1) {
-          a.in = cin; a.out = a.K; a.add = 0; a.scale = sx;
+          a.in = cin; a.out = a.K; a.qout = 1; a.add = 0; a.scale = sx;  // K in the quad-block layout
           hip_check(launch_rows(0, a, st), "scatter rows");
-          a.nterm = T; a.scale = sy2;
+          a.qout = 0; a.nterm = T; a.scale = sy2;
           PassTimer t("scatter_cols", (1 + T) * pb, st);
           hip_check(launch_cols_fanout(2, a, st), "scatter columns");
         }
@@ -312,9 +312,9 @@ struct BornOp : fwix_op_s {
     const int T = par.taylor;
     if (!adjoint) {
       if (T >= 1) {
-        a.in = in; a.out = a.K; a.add = 0; a.scale = sx;
+        a.in = in; a.out = a.K; a.qout = 1; a.add = 0; a.scale = sx;  // K in the quad-block layout
         { PassTimer t("plain_scatter_rows", 2 * sb, st); hip_check(launch_rows(0, a, st), "plain scatter rows"); }
-        a.nterm = T; a.scale = sy2;
+        a.qout = 0; a.nterm = T; a.scale = sy2;
         { PassTimer t("plain_scatter_cols", (1 + T) * sb, st); hip_check(launch_cols_fanout(2, a, st), "plain scatter columns"); }
       }
       a.in = in; a.out = out; a.add = 0; a.nterm = T + 1; a.scale = sx;

@@ -393,9 +393,9 @@ void step_passes(int method, bool adjoint, const fwix_dims& d, const fwix_params
     float2* cout = out + (size_t)c0 * plane;
     // "shared spectrum" shape: PSPI forward and NSPS adjoint
     if ((pspi && !adjoint) || (!pspi && adjoint)) {
-      a.in = cin; a.out = a.K; a.add = 0; a.scale = sx;
+      a.in = cin; a.out = a.K; a.qout = 1; a.add = 0; a.scale = sx;  // K in the quad-block layout
       { auto t = timer("rows_fwd", 2 * pb); hip_check(launch_rows(taper ? 2 : 0, a, st), "rows pass"); }
-      a.scale = sy2;
+      a.qout = 0; a.scale = sy2;
       { auto t = timer("cols_fanout", pb + wb); hip_check(launch_cols_fanout(adjoint ? 1 : 0, a, st), "column fan-out pass"); }
       a.out = cout; a.add = add; a.scale = sx;
       { auto t = timer("rows_select", wb + (1 + (add ? 1 : 0)) * pb + pb / 8); hip_check(launch_rows_select(a, st), "rows select pass"); }
@@ -404,7 +404,7 @@ void step_passes(int method, bool adjoint, const fwix_dims& d, const fwix_params
       { auto t = timer("rows_fanout", pb + pb / 8 + wb); hip_check(launch_rows_fanout(a, st), "rows fan-out pass"); }
       a.scale = sy2;
       { auto t = timer("cols_fanin", wb + pb); hip_check(launch_cols_fanin(adjoint ? 1 : 0, a, st), "column fan-in pass"); }
-      a.in = a.K; a.out = cout; a.add = add; a.scale = sx;
+      a.in = a.K; a.qin = 1; a.out = cout; a.add = add; a.scale = sx;
       { auto t = timer("rows_inv", (2 + (add ? 1 : 0)) * pb); hip_check(launch_rows(taper ? 3 : 1, a, st), "rows inverse pass"); }
     }
   }

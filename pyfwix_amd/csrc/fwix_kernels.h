@@ -85,7 +85,17 @@ struct KArgs {
   float scale;             // normalisation applied by the kernel (set per launch)
   int add;
   int f64;                 // the launch runs its transforms in float64 arithmetic
+  int qin, qout;           // k_rows: in / out in the quad-block layout (they are K)
 };
+
+// Quad-block layout of the column-pass scratch (K and the W terms): a 128-byte
+// line holds a 4 x 4 block (rows 4q..4q+3 x columns 4p..4p+3, row-major inside),
+// blocks row-major.  A column tile of 4 or 8 columns then loads and stores
+// whole lines, and the 4 rows of a row-pass workgroup share every line they
+// touch.  y counts rows over all planes of the chunk (NY is a multiple of 4).
+__device__ __forceinline__ unsigned qidx(int y, int x, int NX) {
+  return (unsigned)(((((y >> 2) * (NX >> 2)) + (x >> 2)) << 4) + ((y & 3) << 2) + (x & 3));
+}
 
 // twiddle tables of the arithmetic type T (float2 or double2)
 __device__ __forceinline__ const float2* twx_of(const KArgs& a, float2*) { return a.twx; }

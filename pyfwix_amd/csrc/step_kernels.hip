@@ -97,7 +97,58 @@ __device__ __forceinline__ void col_tile(int& plane, int& c0) {
 }
 
 // ------------------------------------------------------------------ row passes
+// A row workgroup covers rows R0 .. R0 + 4 NB - 1 (R0 = 4 NB blockIdx.x), wave
+// w rows R0 + w NB .. + NB - 1, and its waves' LDS regions hold those rows
+// in order: row R0 + rel at rel * padded(N) (+ the PadLds padding).  In the
+// quad-block layout the same rows are one contiguous run of 4 NB N elements
+// from R0 N, so the workgroup moves them between LDS and memory with
+// consecutive lanes on consecutive elements (whole 128-byte lines).
+static_assert(kWaves == 4, "quad-block row staging assumes 4 waves of rows");
+template <int N>
+__device__ __forceinline__ int rows_stage_addr(int j) {  // LDS slot of element j of the run
+  const int qr = j / (4 * N), rem = j - qr * 4 * N;
+  const int w = rem & 15, rel = 4 * qr + (w >> 2), col = ((rem >> 4) << 2) + (w & 3);
+  return rel * padded(N) + col + (col >> 5);
+}
+template <int N>
+__device__ __forceinline__ int rows_stage_row(int j) {  // row of element j relative to R0
+  const int qr = j / (4 * N), rem = j - qr * 4 * N;
+  return 4 * qr + ((rem & 15) >> 2);
+}
+template <int N>
+__device__ __forceinline__ int rows_stage_col(int j) {
+  const int rem = j % (4 * N);
+  return ((rem >> 4) << 2) + (rem & 3);
+}
+// run (quad-block memory) -> LDS rows; rows at or beyond nrows read as zero
+template <int N, int NB>
+__device__ __forceinline__ void rows_quad_load(float2* lds, const float2* __restrict__ src, int R0, int nrows) {
+  constexpr int TOT = 4 * NB * N, PER = TOT / kThreads;
+  static_assert(PER * kThreads == TOT, "run must divide evenly");
+  float2 t[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int j = threadIdx.x + k * kThreads;
+    t[k] = R0 + rows_stage_row<N>(j) < nrows ? src[(size_t)R0 * N + j] : czero();
+  }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) lds[rows_stage_addr<N>(threadIdx.x + k * kThreads)] = t[k];
+}
+// LDS rows -> run (quad-block memory): dst[j] = f(value, row, col) for rows below nrows
+template <int N, int NB, class Fn>
+__device__ __forceinline__ void rows_quad_store(const float2* lds, float2* __restrict__ dst, int R0, int nrows, Fn&& f) {
+  constexpr int TOT = 4 * NB * N, PER = TOT / kThreads;
+  static_assert(PER * kThreads == TOT, "run must divide evenly");
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int j = threadIdx.x + k * kThreads;
+    const int row = R0 + rows_stage_row<N>(j);
+    if (row < nrows) dst[(size_t)R0 * N + j] = f(lds[rows_stage_addr<N>(j)], row, rows_stage_col<N>(j), (size_t)R0 * N + j);
+  }
+}
+
 // out = (add ? out : 0) + post(scale * F_x(pre * in))   along x, one row per wave
+// (in / out in the quad-block layout when a.qin / a.qout: staged through LDS)
 template <int N, bool INV, bool PRE_TAPER, bool POST_TAPER>
 __global__ __launch_bounds__(kThreads) void k_rows(KArgs a) {
   using C = RowCfg<N>;
@@ -105,24 +156,54 @@ __global__ __launch_bounds__(kThreads) void k_rows(KArgs a) {
   __shared__ float2 lds[kWaves * C::LDSW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nrows = a.np * a.NY;
-  const int rb = (blockIdx.x * kWaves + wave) * C::NB;
-  if (rb >= nrows) return;
+  const int R0 = blockIdx.x * kWaves * C::NB;
+  const int rb = R0 + wave * C::NB;
+  const bool staged = a.qin || a.qout;  // grid-uniform: every wave reaches the barriers
+  if (rb >= nrows && !staged) return;
   float2* wl = lds + wave * C::LDSW;
-  float2 v[C::V];
-#pragma unroll
-  for (int e = 0; e < C::V; ++e) {
-    int b, i;
-    edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
-    const int lr = rb + b;
-    float2 x = czero();
-    if (lr < nrows) {
-      x = a.in[(unsigned)(lr * N + i)];
-      if (PRE_TAPER) x = cscale(x, a.tapx[i] * a.tapy[lr % a.NY]);
-    }
-    v[e] = x;
-  }
-  F::run(v, wl, a.twx, lane);
   const float s = a.scale;  // row normalisation 1/sqrt(NX)
+  float2 v[C::V];
+  if (a.qin) {
+    rows_quad_load<N, C::NB>(lds, a.in, R0, nrows);
+    __syncthreads();
+  }
+  if (rb < nrows) {
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      int b, i;
+      edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
+      const int lr = rb + b;
+      float2 x = czero();
+      if (lr < nrows) {
+        x = a.qin ? wl[PadLds<N>::addr(b, i)] : a.in[(unsigned)(lr * N + i)];
+        if (PRE_TAPER) x = cscale(x, a.tapx[i] * a.tapy[lr % a.NY]);
+      }
+      v[e] = x;
+    }
+    if (a.qin) wave_sync();  // the transform reuses the region it was read from
+    F::run(v, wl, a.twx, lane);
+  }
+  auto post = [&](float2 y, int lr, int i, unsigned idx) {
+    y = cscale(y, s);
+    if (POST_TAPER) y = cscale(y, a.tapx[i] * a.tapy[lr % a.NY]);
+    if (a.add) y = cadd(y, a.out[idx]);
+    return y;
+  };
+  if (a.qout) {
+    if (rb < nrows) {
+      wave_sync();
+#pragma unroll
+      for (int e = 0; e < C::V; ++e) {
+        int b, i;
+        edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
+        wl[PadLds<N>::addr(b, i)] = v[e];
+      }
+    }
+    __syncthreads();
+    rows_quad_store<N, C::NB>(lds, a.out, R0, nrows, post);
+    return;
+  }
+  if (rb >= nrows) return;
 #pragma unroll
   for (int e = 0; e < C::V; ++e) {
     int b, i;
@@ -130,10 +211,7 @@ __global__ __launch_bounds__(kThreads) void k_rows(KArgs a) {
     const int lr = rb + b;
     if (lr < nrows) {
       const unsigned idx = lr * N + i;
-      float2 y = cscale(v[e], s);
-      if (POST_TAPER) y = cscale(y, a.tapx[i] * a.tapy[lr % a.NY]);
-      if (a.add) y = cadd(y, a.out[idx]);
-      a.out[idx] = y;
+      a.out[idx] = post(v[e], lr, i, idx);
     }
   }
 }
@@ -197,7 +275,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
       int b, i;
       edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
       const int lr = rb + b;
-      if (lr < nrows) W[(unsigned)(lr * N + i)] = to_f2(cscale(x[e], s));
+      if (lr < nrows) W[qidx(lr, i, N)] = to_f2(cscale(x[e], s));
     }
   }
 }
@@ -255,7 +333,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
       int b, i;
       edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
       const int lr = rb + b;
-      nxt[e] = lr < nrows ? W[(unsigned)(lr * N + i)] : czero();
+      nxt[e] = lr < nrows ? W[qidx(lr, i, N)] : czero();
     }
   };
   // absent references contribute nothing to these rows: skip their loads and transforms
@@ -392,18 +470,39 @@ struct PlaneBuf {
   }
 };
 // Byte offsets of this thread's tile elements: element k sits in row
-// threadIdx.x / CW + k * (THREADS / CW), column c0 + threadIdx.x % CW.
+// threadIdx.x / CW + k * RS (RS = THREADS / CW), column c0 + threadIdx.x % CW,
+// of a row-major plane or (quad) of one in the quad-block layout.  When RS is
+// a multiple of 4, a step of RS rows moves the offset by RS rows of NX
+// elements in both layouts; the narrow tiles of short columns (RS = 1, 2)
+// add the quad-block row part per element.  Columns at or beyond NX get
+// offsets at or beyond the plane (kOff + at most a plane).
 template <class C>
 struct TileOffs {
   static constexpr int PER = C::NROW * C::CW / C::THREADS;
+  static constexpr int RS = C::THREADS / C::CW;
   static_assert(PER * C::THREADS == C::NROW * C::CW && C::THREADS % C::CW == 0, "tile must divide evenly");
   unsigned o0, step;
-  __device__ __forceinline__ TileOffs(int NX, int c0) {
+  int row0, nx4;
+  bool quad;
+  __device__ __forceinline__ TileOffs(int NX, int c0, bool quad_) : quad(quad_) {
+    row0 = (int)threadIdx.x / C::CW;
+    nx4 = NX >> 2;
     const int col = c0 + (int)threadIdx.x % C::CW;
-    o0 = col < NX ? (unsigned)(((int)threadIdx.x / C::CW) * NX + col) * 8u : kOff;
-    step = (unsigned)(C::THREADS / C::CW * NX) * 8u;
+    step = (unsigned)(RS * NX) * 8u;
+    if (col >= NX) o0 = kOff;
+    else if (!quad) o0 = (unsigned)(row0 * NX + col) * 8u;
+    else if constexpr (RS % 4 == 0) o0 = qidx(row0, col, NX) * 8u;
+    else o0 = (unsigned)(((col >> 2) << 4) + (col & 3)) * 8u;  // column part; rows added per element
   }
-  __device__ __forceinline__ unsigned at(int k) const { return o0 + (unsigned)k * step; }
+  __device__ __forceinline__ unsigned at(int k) const {
+    if constexpr (RS % 4 != 0) {
+      if (quad) {
+        const int row = row0 + k * RS;
+        return o0 + (unsigned)((((row >> 2) * nx4) << 4) + ((row & 3) << 2)) * 8u;
+      }
+    }
+    return o0 + (unsigned)k * step;
+  }
 };
 __device__ __forceinline__ unsigned plane_bytes(const KArgs& a) { return (unsigned)(a.NX * a.NY) * 8u; }
 
@@ -513,7 +612,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   col_tile<C>(p, c0);
   const size_t pb = (size_t)p * a.NX * N;
   float2* priv = smem + wave * C::PRIV;
-  const TileOffs<C> to(a.NX, c0);
+  const TileOffs<C> to(a.NX, c0, false);
   tile_load<C>(smem, PlaneBuf(a.in + pb, plane_bytes(a)), to);
   __syncthreads();
   float2 v[C::V];
@@ -566,7 +665,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   col_tile<C>(p, c0);
   const size_t pb = (size_t)p * a.NX * N;
   T* priv = (T*)smem + wave * C::PRIV;
-  const TileOffs<C> to(a.NX, c0);
+  const TileOffs<C> to(a.NX, c0, true);
   tile_load<C>(smem, PlaneBuf(a.K + pb, plane_bytes(a)), to);
   const int iw = plane_iw(a, p);
   const T* tw = twy_of(a, (T*)nullptr);
@@ -631,7 +730,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   col_tile<C>(p, c0);
   const size_t pb = (size_t)p * a.NX * N;
   T* priv = (T*)smem + wave * C::PRIV;
-  const TileOffs<C> to(a.NX, c0);
+  const TileOffs<C> to(a.NX, c0, true);
   const int iw = plane_iw(a, p);
   const float w2 = a.w2[iw];
   const int colw = c0 + wave * C::NB;
@@ -823,6 +922,32 @@ __device__ __forceinline__ void split_k2(float* k2v, const KArgs& a, int colw, i
   }
 }
 
+// The phase-shift multipliers M_r(k^2) at this wave's 8 spectrum samples,
+// evaluated at 4 of them.  Indices n and 1024 - n have the same k^2 (the ky
+// table is odd-symmetric, bit for bit), and both lie in the same wave: wave h
+// holds n = 2j + h at j = lane + 64 e (the positions of a last radix-8 stage),
+// so value e >= 4 mirrors value 7 - e of lane 63 - lane (h = 1) or 64 - lane
+// (h = 0).  Lane 0 of wave 0 holds n = 0 and n = 512, its own mirrors: its
+// values 5..7 mirror its values 3..1, and value 4 (n = 512) is evaluated.
+// msrc: byte address of the mirror lane for ds_bpermute.
+template <bool REAL, class TM>
+__device__ __forceinline__ void split_mults(float2* m, const TM& tm, const float* k2v, int msrc, bool lane0h0) {
+  static_assert(SplitCfg::V == 8, "mirror map assumes 8 values per lane");
+#pragma unroll
+  for (int e = 0; e < 4; ++e) m[e] = tm.template at<REAL>(k2v[e]);
+#pragma unroll
+  for (int e = 4; e < 8; ++e) {
+    m[e].x = __int_as_float(__builtin_amdgcn_ds_bpermute(msrc, __float_as_int(m[7 - e].x)));
+    m[e].y = __int_as_float(__builtin_amdgcn_ds_bpermute(msrc, __float_as_int(m[7 - e].y)));
+  }
+  if (lane0h0) {
+    m[4] = tm.template at<REAL>(k2v[4]);
+    m[5] = m[3];
+    m[6] = m[2];
+    m[7] = m[1];
+  }
+}
+
 // cols_fanout for 1024-point columns: W_r = F_y^-1[M_r * F_y(K)] / NY
 template <int MODE, int WPE>
 __global__ __launch_bounds__(SplitCfg::THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_cols_fanout_split(KArgs a) {
@@ -836,11 +961,14 @@ __global__ __launch_bounds__(SplitCfg::THREADS) __attribute__((amdgpu_waves_per_
   __shared__ uint32_t rmk[C::NROW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int cl = wave >> 1, h = wave & 1;
+  static_assert(RS == 8, "split_mults needs the positions of a last radix-8 stage");
+  const int msrc = ((h ? 63 - lane : 64 - lane) & 63) * 4;
+  const bool lane0h0 = h == 0 && lane == 0;
   int p, c0;
   col_tile<C>(p, c0);
   const size_t pb = (size_t)p * a.NX * C::NROW;
   float2* priv = smem + wave * C::PRIV;
-  const TileOffs<C> to(a.NX, c0);
+  const TileOffs<C> to(a.NX, c0, true);
   {
     TileRegs<C> t;
     t.fetch(PlaneBuf(a.K + pb, plane_bytes(a)), to);
@@ -874,8 +1002,15 @@ __global__ __launch_bounds__(SplitCfg::THREADS) __attribute__((amdgpu_waves_per_
     const TermMult<MODE> tm(a, r, w2, srv);
     float2 y[C::V];
     auto mult = [&](auto real) {
+      float2 m[C::V];
+      if constexpr (MODE == 2) {
 #pragma unroll
-      for (int e = 0; e < C::V; ++e) y[e] = cscale(cmul(xs[e], tm.template at<decltype(real)::value>(k2v[e])), s2);
+        for (int e = 0; e < C::V; ++e) m[e] = tm.template at<decltype(real)::value>(k2v[e]);
+      } else {
+        split_mults<decltype(real)::value>(m, tm, k2v, msrc, lane0h0);
+      }
+#pragma unroll
+      for (int e = 0; e < C::V; ++e) y[e] = cscale(cmul(xs[e], m[e]), s2);
     };
     if (tm.real) mult(std::true_type{});
     else mult(std::false_type{});
@@ -902,11 +1037,14 @@ __global__ __launch_bounds__(SplitCfg::THREADS) __attribute__((amdgpu_waves_per_
   __shared__ uint32_t rmk[C::NROW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int cl = wave >> 1, h = wave & 1;
+  static_assert(RS == 8, "split_mults needs the positions of a last radix-8 stage");
+  const int msrc = ((h ? 63 - lane : 64 - lane) & 63) * 4;
+  const bool lane0h0 = h == 0 && lane == 0;
   int p, c0;
   col_tile<C>(p, c0);
   const size_t pb = (size_t)p * a.NX * C::NROW;
   float2* priv = smem + wave * C::PRIV;
-  const TileOffs<C> to(a.NX, c0);
+  const TileOffs<C> to(a.NX, c0, true);
   const int iw = plane_iw(a, p);
   const float w2 = a.w2[iw];
   const int im = model_iw_any(a, iw);
@@ -952,11 +1090,16 @@ __global__ __launch_bounds__(SplitCfg::THREADS) __attribute__((amdgpu_waves_per_
     if constexpr (MODE != 2) srv = split_sref(a, r, iw);
     const TermMult<MODE> tm(a, r, w2, srv);
     auto mult = [&](auto real) {
+      float2 m[C::V];
+      if constexpr (MODE == 2) {
 #pragma unroll
-      for (int e = 0; e < C::V; ++e) {
-        const float2 X = split_spec(smem, tw, cl, split_n<RS>(lane, e, h));
-        acc[e] = cadd(acc[e], cmul(X, tm.template at<decltype(real)::value>(k2v[e])));
+        for (int e = 0; e < C::V; ++e) m[e] = tm.template at<decltype(real)::value>(k2v[e]);
+      } else {
+        split_mults<decltype(real)::value>(m, tm, k2v, msrc, lane0h0);
       }
+#pragma unroll
+      for (int e = 0; e < C::V; ++e)
+        acc[e] = cadd(acc[e], cmul(split_spec(smem, tw, cl, split_n<RS>(lane, e, h)), m[e]));
     };
     if (tm.real) mult(std::true_type{});
     else mult(std::false_type{});
