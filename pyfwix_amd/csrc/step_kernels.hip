@@ -336,16 +336,21 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
       nxt[e] = lr < nrows ? W[qidx(lr, i, N)] : czero();
     }
   };
-  // absent references contribute nothing to these rows: skip their loads and transforms
+  // absent references contribute nothing to these rows: skip their loads and
+  // transforms.  The next reference's row is fetched into registers during
+  // this one's transform, except at 16 values per lane, where holding it
+  // would cost a wave of occupancy
+  constexpr bool PREFETCH = C::V < 16;
   int r = a.rowmask ? next_bit(live, -1) : 0;
   const int rend = a.rowmask ? 32 : a.nref;
-  if (r < rend) fetch(r);
+  if (PREFETCH && r < rend) fetch(r);
   while (r < rend) {
+    if constexpr (!PREFETCH) fetch(r);
     float2 x[C::V];
 #pragma unroll
     for (int e = 0; e < C::V; ++e) x[e] = nxt[e];
     const int rn = a.rowmask ? next_bit(live, r) : r + 1;
-    if (rn < rend) fetch(rn);
+    if (PREFETCH && rn < rend) fetch(rn);
     F::run(x, wl, twl, lane);
 #pragma unroll
     for (int e = 0; e < C::V; ++e)
