@@ -96,9 +96,8 @@ __global__ __launch_bounds__(kThreads) void k_rows_ic_fwd(KArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nrows = a.np * a.NY;
   const int rb = (blockIdx.x * kWaves + wave) * C::NB;
-  // twiddles in LDS: inside the term loops a global load would wait (one
-  // counter for all vector memory operations) for the term stores / the next
-  // term's loads still in flight
+  // twiddles in LDS: a global twiddle load would wait (one counter for all
+  // vector memory operations) for every load still in flight
   __shared__ float2 twl[N];
   for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = a.twx[i];
   __syncthreads();
@@ -251,7 +250,9 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
 // FUSED = true : mu = taper * scale * F_x^-1(K) is the end of the PSPI adjoint
 //                step; it is written to out (the adjoint wavefield) as well.
 // FUSED = false: mu = in (standalone Scatter adjoint).
-template <int N, bool FUSED, bool MAP, class T = float2>
+// ONE: the launch has exactly one Taylor order (nterm == 2), a separate
+// instantiation so that the general form's per-order factors cost it no registers.
+template <int N, bool FUSED, bool MAP, class T = float2, bool ONE = false>
 __global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
   using C = BRowCfg<N>;
   using S = real_of<T>;
@@ -301,38 +302,67 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
     }
   }
   if (a.nterm < 2) return;
-  T pw[C::V], inv[C::V];
-#pragma unroll
-  for (int e = 0; e < C::V; ++e) {
-    int b, i;
-    edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
-    const int lr = rb + b;
-    pw[e] = cz<T>();
-    inv[e] = cz<T>();
-    if (lr < nrows) {
-      const int p = lr / a.NY;
-      const S h = (S)(0.5f * a.w[plane_iw(a, p)] * a.dz);   // scatter.cu:197-238
-      u[e] = CX<T>::mk(-h * u[e].y, h * u[e].x);
-      scatter_factors(a, model_iw<MAP>(a, plane_iw(a, p)), lr - p * a.NY, i, pw[e], inv[e]);
-      pw[e] = cmul(pw[e], inv[e]);                          // start at order 1
-    }
-  }
-  for (int it = 1; it < a.nterm; ++it) {
-    T x[C::V];
-    const S c = (S)taylor_coef(it);
+  if constexpr (ONE) {  // one Taylor order: x = conj(c_1 s~^(-3/2)) * u, nothing carried to a next order
+    const S c = (S)taylor_coef(1);
 #pragma unroll
     for (int e = 0; e < C::V; ++e) {
-      x[e] = cscale(cmulc(u[e], pw[e]), c);                // conj(c_it s~^(-1/2-it)) * u
-      pw[e] = cmul(pw[e], inv[e]);
+      int b, i;
+      edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
+      const int lr = rb + b;
+      T x = cz<T>();
+      if (lr < nrows) {
+        const int p = lr / a.NY;
+        const S h = (S)(0.5f * a.w[plane_iw(a, p)] * a.dz);   // scatter.cu:197-238
+        const T v = CX<T>::mk(-h * u[e].y, h * u[e].x);
+        T rsq, inv;
+        scatter_factors(a, model_iw<MAP>(a, plane_iw(a, p)), lr - p * a.NY, i, rsq, inv);
+        x = cscale(cmulc(v, cmul(rsq, inv)), c);
+      }
+      u[e] = x;
     }
-    F::run(x, wl, twl, lane);
-    float2* W = a.W + (size_t)(it - 1) * a.wstride;
+    F::run(u, wl, twl, lane);
 #pragma unroll
     for (int e = 0; e < C::V; ++e) {
       int b, i;
       edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
       const int lr = rb + b;
-      if (lr < nrows) W[qidx(lr, i, N)] = to_f2(cscale(x[e], a.scale));
+      if (lr < nrows) a.W[qidx(lr, i, N)] = to_f2(cscale(u[e], a.scale));
+    }
+    return;
+  } else {
+    T pw[C::V], inv[C::V];
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      int b, i;
+      edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
+      const int lr = rb + b;
+      pw[e] = cz<T>();
+      inv[e] = cz<T>();
+      if (lr < nrows) {
+        const int p = lr / a.NY;
+        const S h = (S)(0.5f * a.w[plane_iw(a, p)] * a.dz);   // scatter.cu:197-238
+        u[e] = CX<T>::mk(-h * u[e].y, h * u[e].x);
+        scatter_factors(a, model_iw<MAP>(a, plane_iw(a, p)), lr - p * a.NY, i, pw[e], inv[e]);
+        pw[e] = cmul(pw[e], inv[e]);                          // start at order 1
+      }
+    }
+    for (int it = 1; it < a.nterm; ++it) {
+      T x[C::V];
+      const S c = (S)taylor_coef(it);
+#pragma unroll
+      for (int e = 0; e < C::V; ++e) {
+        x[e] = cscale(cmulc(u[e], pw[e]), c);                // conj(c_it s~^(-1/2-it)) * u
+        pw[e] = cmul(pw[e], inv[e]);
+      }
+      F::run(x, wl, twl, lane);
+      float2* W = a.W + (size_t)(it - 1) * a.wstride;
+#pragma unroll
+      for (int e = 0; e < C::V; ++e) {
+        int b, i;
+        edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
+        const int lr = rb + b;
+        if (lr < nrows) W[qidx(lr, i, N)] = to_f2(cscale(x[e], a.scale));
+      }
     }
   }
 }
@@ -353,9 +383,8 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj_out(KArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nrows = a.np * a.NY;
   const int rb = (blockIdx.x * kWaves + wave) * C::NB;
-  // twiddles in LDS: inside the term loops a global load would wait (one
-  // counter for all vector memory operations) for the term stores / the next
-  // term's loads still in flight
+  // twiddles in LDS: a global twiddle load would wait (one counter for all
+  // vector memory operations) for every load still in flight
   __shared__ T twl[N];
   const T* twg = twx_of(a, (T*)nullptr);
   for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = twg[i];
@@ -658,6 +687,9 @@ static hipError_t launch_mapped(const KArgs& a, hipStream_t st) {
 template <int N, bool F, bool M, class T> struct KIcFwd { static constexpr auto fn = k_rows_ic_fwd<N, M>; };
 template <int N, bool F, bool M, class T> struct KScatter { static constexpr auto fn = k_rows_scatter<N, F, M, T>; };
 template <int N, bool F, bool M, class T> struct KScatterAdj { static constexpr auto fn = k_rows_scatter_adj<N, F, M, T>; };
+template <int N, bool F, bool M, class T> struct KScatterAdj1 {
+  static constexpr auto fn = k_rows_scatter_adj<N, F, M, T, true>;
+};
 template <int N, bool F, bool M, class T> struct KAdjOut { static constexpr auto fn = k_rows_adj_out<N, F, M, T>; };
 
 hipError_t launch_rows_ic_fwd(const KArgs& a, hipStream_t st) {
@@ -669,6 +701,8 @@ hipError_t launch_rows_scatter(bool born, const KArgs& a, hipStream_t st) {
   return born ? launch_mapped<KScatter, true, true>(a, st) : launch_mapped<KScatter, false>(a, st);
 }
 hipError_t launch_rows_scatter_adj(bool fused, const KArgs& a, hipStream_t st) {
+  if (a.nterm == 2)
+    return fused ? launch_mapped<KScatterAdj1, true, true>(a, st) : launch_mapped<KScatterAdj1, false>(a, st);
   return fused ? launch_mapped<KScatterAdj, true, true>(a, st) : launch_mapped<KScatterAdj, false>(a, st);
 }
 hipError_t launch_rows_adj_out(bool ic, const KArgs& a, hipStream_t st) {
