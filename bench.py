@@ -58,8 +58,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-COPY_PROBE_BYTES = 1 << 28          # bytes read (and written) by the copy probe
-COPY_PROBE_TAG = "MulFunctor"       # the probe is y = 1.0 * x over float32
+COPY_PROBE_BYTES = 2 << 30          # bytes read (and written) by the copy probe
+COPY_PROBE_TAG = "fwix::k_copy16"   # the probe is the library's 16-byte stream copy (fwix_copy),
+                                    # the same kernel and size as the bandwidth probe hbm_copy_gbs
 
 
 def pass_kernel(name, w):
@@ -131,7 +132,8 @@ def summarize_traffic(fetch_csv, write_csv, out_path, command_fetch, command_wri
     wf = kib / write[[k for k in write if COPY_PROBE_TAG in k][0]][1]
     lines = [f"# HBM traffic per dispatch, {title}", "",
              "Two counter runs (from the repository root):", "", "```", command_fetch, command_write, "```", "",
-             f"Calibration: a plain copy of {COPY_PROBE_BYTES} bytes (`{COPY_PROBE_TAG}` kernel, 1.0 * x) reads "
+             f"Calibration: a plain copy of {COPY_PROBE_BYTES} bytes (`{COPY_PROBE_TAG}`, every dispatch of it in the "
+             f"run copies that many) reads "
              f"{fetch[probe[0]][1]:.0f} KiB (FETCH_SIZE) and writes {write[[k for k in write if COPY_PROBE_TAG in k][0]][1]:.0f} KiB "
              f"(WRITE_SIZE); scale factors fetch {ff:.4f}, write {wf:.4f} turn the counters into bytes.", "",
              "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | traffic bytes (scaled) |",
@@ -983,14 +985,17 @@ def main():
         sys.exit(2)
 
     if args.copy_probe:
-        # a plain 1.0 * x copy of known size: counter runs scale FETCH_SIZE / WRITE_SIZE by it
+        # a plain copy of known size with ordinary loads and stores, like the
+        # pass kernels': counter runs scale FETCH_SIZE / WRITE_SIZE by it
         import torch
+        from pyfwix_amd import _lib
         x = torch.ones(COPY_PROBE_BYTES // 4, dtype=torch.float32, device="cuda:0")
         y = torch.empty_like(x)
         for _ in range(2):
-            torch.mul(x, 1.0, out=y)
+            _lib.call("fwix_copy", x.data_ptr(), y.data_ptr(), COPY_PROBE_BYTES, torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
         del x, y
+        torch.cuda.empty_cache()
     from pyfwix_amd.workloads import FWI_CONFIGS, NSPS_CONFIGS, SHOT_BATCH_CONFIGS
     if args.config in NSPS_CONFIGS:
         bench_nsps(args)
