@@ -275,6 +275,14 @@ int fwix_born_set_codec(fwix_op op, int codec);
  * counterpart in the reference (float32 only). */
 enum { FWIX_PREC_F32 = 0, FWIX_PREC_F64_SCATTER = 1 };
 int fwix_born_set_precision(fwix_op op, int precision);
+/* Background recursions (fwix_born_model and the recomputed checkpoint
+ * segments) chain consecutive PSPI steps through the row spectrum: a step's
+ * select pass also leaves F_x(taper * U_{z+1}) for the next step, which then
+ * skips its first row pass (one pass and one read of U fewer per step), except
+ * where a source injects in between.  on = 1 (default) chains, 0 runs every
+ * step whole.  Results agree to float32 rounding.  Costs one wavefield-sized
+ * device buffer.  No counterpart in the reference. */
+int fwix_born_set_step_chaining(fwix_op op, int on);
 /* The codec on any device array of n complex samples: q holds n short2,
  * scales (n + 63) / 64 floats. */
 int fwix_bfp16_pack(const fwix_c64* in, void* q, float* scales, size_t n, fwix_stream s);

@@ -115,6 +115,7 @@ _SIGS = {
     "fwix_born_set_density": [_P, _P],
     "fwix_born_set_codec": [_P, _I],
     "fwix_born_set_precision": [_P, _I],
+    "fwix_born_set_step_chaining": [_P, _I],
     "fwix_bfp16_pack": [_P, _P, _P, C.c_size_t, _P],
     "fwix_bfp16_unpack": [_P, _P, _P, C.c_size_t, _P],
     "fwix_born_model_up": [_P, _I, _P, _P],

@@ -422,10 +422,13 @@ struct BornOp : fwix_op_s {
       const int s1 = std::min(s0 + ckpt, d.nz);
       float2* sg = (float2*)seg.p;
       restore(s0 / ckpt, sg, st);
+      bool kr = false;  // the row spectrum of prev is already in the engine (chained steps)
       for (int zz = s0 + 1; zz < s1; ++zz) {
         float2* prev = sg + (size_t)(zz - 1 - s0) * n;
         float2* cur = sg + (size_t)(zz - s0) * n;
-        eng.apply(false, zz - 1, 0, prev, cur, st);
+        const bool kn = zz + 1 < s1 && !src.injects(zz);  // cur goes on unchanged into the next step
+        eng.apply_chained(zz - 1, prev, cur, kr, kn, st);
+        kr = kn && eng.chains();
         src.forward(zz, wavelet, cur, st);
         ++steps;
       }
@@ -444,13 +447,21 @@ struct BornOp : fwix_op_s {
     const bool direct = ckpt == 1 && !codec;
     float2* u = direct ? (float2*)store.p : bufU();
     hip_check(launch_zero(u, n, st), "zero wavefield");
+    // consecutive steps chain through the row spectrum while no source
+    // injects in between (the receivers and checkpoints only read U)
+    bool kr = false;
+    auto step = [&](int z, const float2* in, float2* out) {
+      const bool kn = z + 2 < d.nz && !src.injects(z + 1);
+      eng.apply_chained(z, in, out, kr, kn, st);
+      kr = kn && eng.chains();
+      ++steps;
+    };
     for (int z = 0; z < d.nz; ++z) {
       src.forward(z, wavelet, u, st);
       if (data) rec.adjoint(z, data, u, st);
       if (direct) {
         if (z + 1 < d.nz) {
-          eng.apply(false, z, 0, u, u + n, st);
-          ++steps;
+          step(z, u, u + n);
           u += n;
         }
         continue;
@@ -463,10 +474,7 @@ struct BornOp : fwix_op_s {
         else
           hip_check(launch_axpy(u, (float2*)store.p + k * n, n, 1.f, 0, st), "store");
       }
-      if (z + 1 < d.nz) {
-        eng.apply(false, z, 0, u, u, st);
-        ++steps;
-      }
+      if (z + 1 < d.nz) step(z, u, u);
     }
     seg_first = -1;
     have_bg = true;
@@ -1001,6 +1009,13 @@ int fwix_born_set_precision(fwix_op op, int precision) {
     if (precision != FWIX_PREC_F32 && precision != FWIX_PREC_F64_SCATTER)
       throw Error("precision must be FWIX_PREC_F32 or FWIX_PREC_F64_SCATTER");
     p->f64 = precision == FWIX_PREC_F64_SCATTER;
+  });
+}
+int fwix_born_set_step_chaining(fwix_op op, int on) {
+  return guard([&] {
+    auto* p = as<BornOp>(op, KIND_BORN);
+    p->eng.chain = on != 0;
+    if (!on) p->eng.Kall.release();
   });
 }
 int fwix_bfp16_pack(const fwix_c64* in, void* q, float* scales, size_t n, fwix_stream s) {

@@ -373,7 +373,7 @@ KArgs StepEngine::base(int iz, int c0, int np) const {
 // timers (null: no timers).
 void step_passes(int method, bool adjoint, const fwix_dims& d, const fwix_params& p, const KArgs& proto, int chunk,
                  int add, const float2* in, float2* out, hipStream_t st,
-                 const std::function<double(int, int)>* rows) {
+                 const std::function<double(int, int)>* rows, float2* kall, bool k_ready, bool k_next) {
   const size_t plane = (size_t)d.NX * d.NY;
   const int planes = d.NW * d.NS;
   const float sx = (float)(1.0 / std::sqrt((double)d.NX));
@@ -389,16 +389,21 @@ void step_passes(int method, bool adjoint, const fwix_dims& d, const fwix_params
     a.p0 = c0;
     a.np = np;
     a.wstride = (long long)np * plane;
+    if (kall) a.K = kall + (size_t)c0 * plane;
     const float2* cin = in + (size_t)c0 * plane;
     float2* cout = out + (size_t)c0 * plane;
     // "shared spectrum" shape: PSPI forward and NSPS adjoint
     if ((pspi && !adjoint) || (!pspi && adjoint)) {
-      a.in = cin; a.out = a.K; a.qout = 1; a.add = 0; a.scale = sx;  // K in the quad-block layout
-      { auto t = timer("rows_fwd", 2 * pb); hip_check(launch_rows(taper ? 2 : 0, a, st), "rows pass"); }
-      a.qout = 0; a.scale = sy2;
+      if (!k_ready) {
+        a.in = cin; a.out = a.K; a.qout = 1; a.add = 0; a.scale = sx;  // K in the quad-block layout
+        { auto t = timer("rows_fwd", 2 * pb); hip_check(launch_rows(taper ? 2 : 0, a, st), "rows pass"); }
+        a.qout = 0;
+      }
+      a.scale = sy2;
       { auto t = timer("cols_fanout", pb + wb); hip_check(launch_cols_fanout(adjoint ? 1 : 0, a, st), "column fan-out pass"); }
-      a.out = cout; a.add = add; a.scale = sx;
-      { auto t = timer("rows_select", wb + (1 + (add ? 1 : 0)) * pb + pb / 8); hip_check(launch_rows_select(a, st), "rows select pass"); }
+      a.out = cout; a.add = add; a.scale = sx; a.knext = k_next ? 1 : 0;
+      { auto t = timer("rows_select", wb + (1 + (add ? 1 : 0) + (k_next ? 1 : 0)) * pb + pb / 8); hip_check(launch_rows_select(a, st), "rows select pass"); }
+      a.knext = 0;
     } else {  // "masked inputs" shape: PSPI adjoint and NSPS forward
       a.in = cin; a.scale = sx;
       { auto t = timer("rows_fanout", pb + pb / 8 + wb); hip_check(launch_rows_fanout(a, st), "rows fan-out pass"); }
@@ -416,6 +421,20 @@ void StepEngine::apply(bool adjoint, int iz, int add, const float2* in, float2* 
   if (plain) return apply_plain(adjoint, iz, add, in, out, st);
   const std::function<double(int, int)> rows = [&](int c0, int np) { return present_rows(iz, c0, np); };
   step_passes(method, adjoint, tables.dims, params, base(iz, 0, chunk), chunk, add, in, out, st, &rows);
+}
+
+void StepEngine::apply_chained(int iz, const float2* in, float2* out, bool k_ready, bool k_next, hipStream_t st) {
+  if (!chains()) return apply(false, iz, 0, in, out, st);
+  check_ready(iz);
+  const fwix_dims& d = tables.dims;
+  const size_t bytes = (size_t)d.NX * d.NY * d.NW * d.NS * sizeof(float2);
+  if (Kall.n != bytes) {
+    if (k_ready) throw Error("chained step: no row spectrum left by a previous step");
+    Kall.alloc(bytes);
+  }
+  const std::function<double(int, int)> rows = [&](int c0, int np) { return present_rows(iz, c0, np); };
+  step_passes(method, false, d, params, base(iz, 0, chunk), chunk, 0, in, out, st, &rows, (float2*)Kall.p, k_ready,
+              k_next);
 }
 
 // The PSPI step at the reference's operator granularity, one shot per launch

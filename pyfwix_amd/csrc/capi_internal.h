@@ -93,9 +93,15 @@ struct PassTimer {
 };
 void validate_params(const fwix_params* p);
 int auto_chunk(const fwix_dims& d, int nterm, int requested);
+// Step chaining (forward PSPI / adjoint NSPS only, the "shared spectrum"
+// shape): with kall the chunks' row spectra live in kall (all planes) instead
+// of the proto's K; k_ready skips the first row pass (kall already holds
+// scale * F_x(taper * in), left by the previous step), and k_next makes the
+// select pass leave scale * F_x(taper * out) there for the next step.
 void step_passes(int method, bool adjoint, const fwix_dims& d, const fwix_params& p, const KArgs& proto, int chunk,
                  int add, const float2* in, float2* out, hipStream_t st,
-                 const std::function<double(int, int)>* rows);
+                 const std::function<double(int, int)>* rows, float2* kall = nullptr, bool k_ready = false,
+                 bool k_next = false);
 
 // One depth step (PSPI or NSPS) over a whole wavefield, forward or adjoint,
 // run as chunks of planes through the three fused passes.
@@ -130,6 +136,15 @@ struct StepEngine {
   KArgs base(int iz, int c0, int np) const;
   void apply(bool adjoint, int iz, int add, const float2* in, float2* out, hipStream_t st);
   void apply_plain(bool adjoint, int iz, int add, const float2* in, float2* out, hipStream_t st);
+  // Forward steps of a recursion chained through the row spectrum (Kall, all
+  // planes): k_ready = the previous chained step left F_x(taper * in) in Kall
+  // (nothing changed in between), k_next = leave F_x(taper * out) for the next
+  // step.  Falls back to apply() where chaining does not apply (plain path,
+  // masked-input shape).  chain = false turns chaining off (every step whole).
+  DevBuf Kall;
+  bool chain = true;
+  bool chains() const { return chain && !plain && method == FWIX_PSPI; }
+  void apply_chained(int iz, const float2* in, float2* out, bool k_ready, bool k_next, hipStream_t st);
 };
 
 
@@ -154,6 +169,8 @@ struct InjPlan {
              const int* ids, const std::vector<int>* first = nullptr, const std::vector<int>* count = nullptr);
   void forward(int iz, const float2* traces, float2* wfld, hipStream_t st) const;  // wfld += Inj_iz(traces)
   void adjoint(int iz, float2* traces, const float2* wfld, hipStream_t st) const;  // traces += Rec_iz^T wfld
+  // whether forward(iz) changes the wavefield (some trace injects at level iz)
+  bool injects(int iz) const { return iz >= 0 && iz < nlev && seg_lvl[iz + 1] > seg_lvl[iz]; }
 };
 
 }  // namespace fwix

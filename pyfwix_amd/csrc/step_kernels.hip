@@ -281,10 +281,15 @@ __global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
 }
 
 // rows_select: out = (add ? out : 0) + sum_r [labels == r] * scale * F_x^-1(W_r)
-template <int N, bool MAP>
+// KNEXT: also K = scale * F_x(taper * out) (quad-block), the first row pass of
+// a next step that propagates out further: one transform more here instead of
+// a pass that reads out back.
+template <int N, bool MAP, bool KNEXT = false>
 __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
   using C = RowCfg<N>;
   using F = WaveFft<N, C::NB, false, true, PadLds<N>>;
+  using Ff = WaveFft<N, C::NB, false, false, PadLds<N>>;
+  static_assert(F::P::last() == Ff::P::first(), "register hand-off needs mirrored plans");
   __shared__ float2 lds[kWaves * C::LDSW];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nrows = a.np * a.NY;
@@ -368,6 +373,17 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
       float2 y = cscale(acc[e], s);
       if (a.add) y = cadd(y, a.out[idx]);
       a.out[idx] = y;
+      if constexpr (KNEXT) acc[e] = cscale(y, a.tapx[i] * a.tapy[lr % a.NY]);
+    }
+  }
+  if constexpr (KNEXT) {
+    Ff::run(acc, wl, twl, lane);
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      int b, i;
+      edge_pos<N, C::NB, Ff::P::last(), false>(lane, e, b, i);
+      const int lr = rb + b;
+      if (lr < nrows) a.K[qidx(lr, i, N)] = cscale(acc[e], s);
     }
   }
 }
@@ -1241,8 +1257,14 @@ hipError_t launch_rows_fanout(const KArgs& a, hipStream_t st) {
 hipError_t launch_rows_select(const KArgs& a, hipStream_t st) {
   return dispatch_n(a.NX, [&](auto nc) -> hipError_t {
     constexpr int N = decltype(nc)::value;
-    if (a.mplane) hipLaunchKernelGGL((k_rows_select<N, true>), row_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
-    else hipLaunchKernelGGL((k_rows_select<N, false>), row_grid(N, a.np * a.NY), dim3(kThreads), 0, st, a);
+    const dim3 g = row_grid(N, a.np * a.NY);
+    if (a.knext) {
+      if (a.mplane) hipLaunchKernelGGL((k_rows_select<N, true, true>), g, dim3(kThreads), 0, st, a);
+      else hipLaunchKernelGGL((k_rows_select<N, false, true>), g, dim3(kThreads), 0, st, a);
+    } else {
+      if (a.mplane) hipLaunchKernelGGL((k_rows_select<N, true>), g, dim3(kThreads), 0, st, a);
+      else hipLaunchKernelGGL((k_rows_select<N, false>), g, dim3(kThreads), 0, st, a);
+    }
     return hipGetLastError();
   });
 }

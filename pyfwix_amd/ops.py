@@ -440,6 +440,13 @@ class Born:
         grids where the first Taylor term's k^2 / (w^2 s) weight amplifies float32 rounding."""
         call("fwix_born_set_precision", self._h.h, int(precision))
 
+    def set_step_chaining(self, on: bool) -> None:
+        """Chain the background recursion's PSPI steps through the row spectrum (default
+        on): each step's select pass leaves the next step's row spectrum, so the next
+        step skips its first row pass.  Off runs every step whole (same results to
+        float32 rounding)."""
+        call("fwix_born_set_step_chaining", self._h.h, int(bool(on)))
+
     def set_density(self, den):
         d = self.dims
         _check(den, (d.nz, d.NW, d.ny, d.nx), name="density")
