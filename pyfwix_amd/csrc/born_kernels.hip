@@ -102,6 +102,12 @@ __global__ __launch_bounds__(kThreads) void k_rows_ic_fwd(KArgs a) {
   for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = a.twx[i];
   __syncthreads();
   if (rb >= nrows) return;
+  // rows of the padding (y >= ny) carry no product: their spectrum is zero
+  bool any = false;
+  for (int b = 0; b < C::NB; ++b) {
+    const int lr = rb + b;
+    if (lr < nrows && lr % a.NY < a.ny) any = true;
+  }
   float2 v[C::V];
 #pragma unroll
   for (int e = 0; e < C::V; ++e) {
@@ -118,7 +124,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_ic_fwd(KArgs a) {
       }
     }
   }
-  F::run(v, lds + wave * C::LDSW, twl, lane);
+  if (any) F::run(v, lds + wave * C::LDSW, twl, lane);
 #pragma unroll
   for (int e = 0; e < C::V; ++e) {
     int b, i;
@@ -390,6 +396,16 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj_out(KArgs a) {
   for (int i = threadIdx.x; i < N; i += kThreads) twl[i] = twg[i];
   __syncthreads();
   if (rb >= nrows) return;
+  if constexpr (IC) {
+    // the imaging condition writes the model region only: rows of the
+    // padding (y >= ny) have nothing to produce, not even their transform
+    bool any = false;
+    for (int b = 0; b < C::NB; ++b) {
+      const int lr = rb + b;
+      if (lr < nrows && lr % a.NY < a.ny) any = true;
+    }
+    if (!any) return;
+  }
   T x[C::V];
   if (a.nterm > 1) {
 #pragma unroll
