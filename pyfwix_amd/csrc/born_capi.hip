@@ -452,7 +452,10 @@ struct BornOp : fwix_op_s {
     bool kr = false;
     auto step = [&](int z, const float2* in, float2* out) {
       const bool kn = z + 2 < d.nz && !src.injects(z + 1);
-      eng.apply_chained(z, in, out, kr, kn, st);
+      // U_{z+1} itself is read only where it is stored or recorded (the next
+      // chained step starts from its spectrum)
+      const bool need = direct || (z + 1) % ckpt == 0 || (data && rec.records(z + 1));
+      eng.apply_chained(z, in, out, kr, kn, st, need);
       kr = kn && eng.chains();
       ++steps;
     };

@@ -373,7 +373,8 @@ KArgs StepEngine::base(int iz, int c0, int np) const {
 // timers (null: no timers).
 void step_passes(int method, bool adjoint, const fwix_dims& d, const fwix_params& p, const KArgs& proto, int chunk,
                  int add, const float2* in, float2* out, hipStream_t st,
-                 const std::function<double(int, int)>* rows, float2* kall, bool k_ready, bool k_next) {
+                 const std::function<double(int, int)>* rows, float2* kall, bool k_ready, bool k_next,
+                 bool out_needed) {
   const size_t plane = (size_t)d.NX * d.NY;
   const int planes = d.NW * d.NS;
   const float sx = (float)(1.0 / std::sqrt((double)d.NX));
@@ -401,8 +402,9 @@ void step_passes(int method, bool adjoint, const fwix_dims& d, const fwix_params
       }
       a.scale = sy2;
       { auto t = timer("cols_fanout", pb + wb); hip_check(launch_cols_fanout(adjoint ? 1 : 0, a, st), "column fan-out pass"); }
-      a.out = cout; a.add = add; a.scale = sx; a.knext = k_next ? 1 : 0;
-      { auto t = timer("rows_select", wb + (1 + (add ? 1 : 0) + (k_next ? 1 : 0)) * pb + pb / 8); hip_check(launch_rows_select(a, st), "rows select pass"); }
+      a.out = cout; a.add = add; a.scale = sx; a.knext = k_next ? (out_needed ? 1 : 2) : 0;
+      const double ob = (k_next && !out_needed) ? 0.0 : (1 + (add ? 1 : 0)) * pb;
+      { auto t = timer("rows_select", wb + ob + (k_next ? pb : 0.0) + pb / 8); hip_check(launch_rows_select(a, st), "rows select pass"); }
       a.knext = 0;
     } else {  // "masked inputs" shape: PSPI adjoint and NSPS forward
       a.in = cin; a.scale = sx;
@@ -423,7 +425,8 @@ void StepEngine::apply(bool adjoint, int iz, int add, const float2* in, float2* 
   step_passes(method, adjoint, tables.dims, params, base(iz, 0, chunk), chunk, add, in, out, st, &rows);
 }
 
-void StepEngine::apply_chained(int iz, const float2* in, float2* out, bool k_ready, bool k_next, hipStream_t st) {
+void StepEngine::apply_chained(int iz, const float2* in, float2* out, bool k_ready, bool k_next, hipStream_t st,
+                               bool out_needed) {
   if (!chains()) return apply(false, iz, 0, in, out, st);
   check_ready(iz);
   const fwix_dims& d = tables.dims;
@@ -434,7 +437,7 @@ void StepEngine::apply_chained(int iz, const float2* in, float2* out, bool k_rea
   }
   const std::function<double(int, int)> rows = [&](int c0, int np) { return present_rows(iz, c0, np); };
   step_passes(method, false, d, params, base(iz, 0, chunk), chunk, 0, in, out, st, &rows, (float2*)Kall.p, k_ready,
-              k_next);
+              k_next, out_needed || !k_next);
 }
 
 // The PSPI step at the reference's operator granularity, one shot per launch

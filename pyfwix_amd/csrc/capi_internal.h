@@ -97,11 +97,12 @@ int auto_chunk(const fwix_dims& d, int nterm, int requested);
 // shape): with kall the chunks' row spectra live in kall (all planes) instead
 // of the proto's K; k_ready skips the first row pass (kall already holds
 // scale * F_x(taper * in), left by the previous step), and k_next makes the
-// select pass leave scale * F_x(taper * out) there for the next step.
+// select pass leave scale * F_x(taper * out) there for the next step;
+// out_needed = false (with k_next) skips writing out itself.
 void step_passes(int method, bool adjoint, const fwix_dims& d, const fwix_params& p, const KArgs& proto, int chunk,
                  int add, const float2* in, float2* out, hipStream_t st,
                  const std::function<double(int, int)>* rows, float2* kall = nullptr, bool k_ready = false,
-                 bool k_next = false);
+                 bool k_next = false, bool out_needed = true);
 
 // One depth step (PSPI or NSPS) over a whole wavefield, forward or adjoint,
 // run as chunks of planes through the three fused passes.
@@ -144,7 +145,10 @@ struct StepEngine {
   DevBuf Kall;
   bool chain = true;
   bool chains() const { return chain && !plain && method == FWIX_PSPI; }
-  void apply_chained(int iz, const float2* in, float2* out, bool k_ready, bool k_next, hipStream_t st);
+  // out_needed = false (only with k_next): nobody reads out before the next
+  // chained step overwrites it, so it is not written at all
+  void apply_chained(int iz, const float2* in, float2* out, bool k_ready, bool k_next, hipStream_t st,
+                     bool out_needed = true);
 };
 
 
@@ -171,6 +175,8 @@ struct InjPlan {
   void adjoint(int iz, float2* traces, const float2* wfld, hipStream_t st) const;  // traces += Rec_iz^T wfld
   // whether forward(iz) changes the wavefield (some trace injects at level iz)
   bool injects(int iz) const { return iz >= 0 && iz < nlev && seg_lvl[iz + 1] > seg_lvl[iz]; }
+  // whether adjoint(iz) reads the wavefield (some trace records at level iz)
+  bool records(int iz) const { return iz >= 0 && iz < nlev && act_lvl[iz + 1] > act_lvl[iz]; }
 };
 
 }  // namespace fwix

@@ -87,6 +87,7 @@ struct KArgs {
   int f64;                 // the launch runs its transforms in float64 arithmetic
   int qin, qout;           // k_rows: in / out in the quad-block layout (they are K)
   int knext;               // k_rows_select: also K = scale * F_x(taper * out), the next step's spectrum
+                           //   (2: K only, out itself is not needed)
 };
 
 // Quad-block layout of the column-pass scratch (K and the W terms): a 128-byte

@@ -372,7 +372,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
       const unsigned idx = lr * N + i;
       float2 y = cscale(acc[e], s);
       if (a.add) y = cadd(y, a.out[idx]);
-      a.out[idx] = y;
+      if (!KNEXT || a.knext != 2) a.out[idx] = y;
       if constexpr (KNEXT) acc[e] = cscale(y, a.tapx[i] * a.tapy[lr % a.NY]);
     }
   }
