@@ -800,7 +800,11 @@ def bench_fwi(args):
 
     class _W:  # pass -> kernel symbol: a shot batch maps planes to shots (row kernels' MAP flag)
         NX = NY = N
-    kern = lambda name: (pass_kernel(name, _W) or "").replace(", false>", ", true>") or None  # noqa: E731
+
+    def kern(name):
+        k = (pass_kernel(name, _W) or "").replace(", false>", ", true>") or None
+        # the background steps are chained: their select pass also leaves the next step's spectrum
+        return f"fwix::k_rows_select<{N}, true, true>" if name == "rows_select" else k
     roof = _roofline(passes, args.config, kern, copy_gbs, ev_us)
     lab = N * N * b.planes
     m = nx * nx * b.planes
