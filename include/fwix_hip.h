@@ -366,6 +366,15 @@ int fwix_tune_columns_get(int kind, int mode, int n);
  * _set returns 0 or -1 for an invalid argument; _get the layout or -1. */
 int fwix_tune_columns_layout_set(int kind, int mode, int n, int layout);
 int fwix_tune_columns_layout_get(int kind, int mode, int n);
+/* Layout of the fused passes' internal scratch (the spectrum K and the
+ * per-reference / per-term planes W) for column length n (no counterpart in
+ * the reference): 1 = 4 x 4 quad blocks (a 128-byte line holds 4 rows x 4
+ * columns), 0 = row-major.  Default: quad blocks at n = 1024, row-major
+ * elsewhere.  Results are bit-identical in both.  Takes effect for the next
+ * operator call; _set returns 0 or -1 for an invalid argument, _get the
+ * layout or -1. */
+int fwix_tune_scratch_layout_set(int n, int quad);
+int fwix_tune_scratch_layout_get(int n);
 
 /* Device copy of `bytes` (a multiple of 16, 16-byte aligned buffers) with
  * 16-byte loads and stores: the stream-copy kernel that measures the

@@ -131,6 +131,8 @@ _SIGS = {
     "fwix_tune_columns_get": [_I, _I, _I],
     "fwix_tune_columns_layout_set": [_I, _I, _I, _I],
     "fwix_tune_columns_layout_get": [_I, _I, _I],
+    "fwix_tune_scratch_layout_set": [_I, _I],
+    "fwix_tune_scratch_layout_get": [_I],
     "fwix_copy": [_P, _P, C.c_size_t, _P],
     "fwix_timing_overhead": [C.c_void_p, _I, C.POINTER(C.c_float)],
     "fwix_timing_collect": [],

@@ -130,7 +130,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_ic_fwd(KArgs a) {
     int b, i;
     edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
     const int lr = rb + b;
-    if (lr < nrows) a.K[qidx(lr, i, N)] = cscale(v[e], a.scale);
+    if (lr < nrows) a.K[widx(a, lr, i, N)] = cscale(v[e], a.scale);
   }
 }
 
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
       int b, i;
       edge_pos<N, C::NB, Fi::P::first(), false>(lane, e, b, i);
       const int lr = rb + b;
-      x[e] = lr < nrows ? from_f2<T>(W[qidx(lr, i, N)]) : cz<T>();
+      x[e] = lr < nrows ? from_f2<T>(W[widx(a, lr, i, N)]) : cz<T>();
     }
     Fi::run(x, wl, twl, lane);
     const S c = (S)taylor_coef(it) * (S)a.scale;  // the inverse row transform's 1/sqrt(NX)
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
       int b, i;
       edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
       const int lr = rb + b;
-      if (lr < nrows) a.K[qidx(lr, i, N)] = to_f2(cscale(acc[e], a.scale));
+      if (lr < nrows) a.K[widx(a, lr, i, N)] = to_f2(cscale(acc[e], a.scale));
     }
   }
 }
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
       int b, i;
       edge_pos<N, C::NB, Fi::P::first(), false>(lane, e, b, i);
       const int lr = rb + b;
-      u[e] = lr < nrows ? from_f2<T>(a.K[qidx(lr, i, N)]) : cz<T>();
+      u[e] = lr < nrows ? from_f2<T>(a.K[widx(a, lr, i, N)]) : cz<T>();
     }
     Fi::run(u, wl, twl, lane);
 #pragma unroll
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
       int b, i;
       edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
       const int lr = rb + b;
-      if (lr < nrows) a.W[qidx(lr, i, N)] = to_f2(cscale(u[e], a.scale));
+      if (lr < nrows) a.W[widx(a, lr, i, N)] = to_f2(cscale(u[e], a.scale));
     }
     return;
   } else {
@@ -367,7 +367,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter_adj(KArgs a) {
         int b, i;
         edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
         const int lr = rb + b;
-        if (lr < nrows) W[qidx(lr, i, N)] = to_f2(cscale(x[e], a.scale));
+        if (lr < nrows) W[widx(a, lr, i, N)] = to_f2(cscale(x[e], a.scale));
       }
     }
   }
@@ -413,7 +413,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_adj_out(KArgs a) {
       int b, i;
       edge_pos<N, C::NB, Fi::P::first(), false>(lane, e, b, i);
       const int lr = rb + b;
-      x[e] = lr < nrows ? from_f2<T>(a.K[qidx(lr, i, N)]) : cz<T>();
+      x[e] = lr < nrows ? from_f2<T>(a.K[widx(a, lr, i, N)]) : cz<T>();
     }
     Fi::run(x, lds + wave * C::LDSW, twl, lane);
   } else {

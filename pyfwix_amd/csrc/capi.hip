@@ -214,6 +214,7 @@ KArgs Tables::kargs() const {
   a.NX = dims.NX; a.NY = dims.NY; a.NW = dims.NW; a.NS = dims.NS;
   a.nx = dims.nx; a.ny = dims.ny;
   a.dz = dims.dz;
+  a.quad = scratch_quad_get(dims.NY) == 1 ? 1 : 0;
   return a;
 }
 
@@ -779,6 +780,13 @@ int fwix_tune_columns_layout_set(int kind, int mode, int n, int layout) {
   });
 }
 int fwix_tune_columns_layout_get(int kind, int mode, int n) { return col_layout_get(kind, mode, n); }
+int fwix_tune_scratch_layout_set(int n, int quad) {
+  return guard([&] {
+    if (scratch_quad_set(n, quad) != 0)
+      throw Error("tune_scratch_layout: n must be a supported size and quad 0 or 1");
+  });
+}
+int fwix_tune_scratch_layout_get(int n) { return scratch_quad_get(n); }
 int fwix_copy(const void* in, void* out, size_t bytes, fwix_stream s) {
   return guard([&] {
     if (bytes % 16) throw Error("fwix_copy: bytes must be a multiple of 16");

@@ -85,18 +85,24 @@ struct KArgs {
   float scale;             // normalisation applied by the kernel (set per launch)
   int add;
   int f64;                 // the launch runs its transforms in float64 arithmetic
-  int qin, qout;           // k_rows: in / out in the quad-block layout (they are K)
+  int qin, qout;           // k_rows: in / out are K (in the scratch layout below)
+  int quad;                // K and W in the quad-block layout (else row-major)
   int knext;               // k_rows_select: also K = scale * F_x(taper * out), the next step's spectrum
                            //   (2: K only, out itself is not needed)
 };
 
-// Quad-block layout of the column-pass scratch (K and the W terms): a 128-byte
+// Quad-block layout of the column-pass scratch (K and the W terms, where the
+// launch's a.quad says so; row-major otherwise): a 128-byte
 // line holds a 4 x 4 block (rows 4q..4q+3 x columns 4p..4p+3, row-major inside),
 // blocks row-major.  A column tile of 4 or 8 columns then loads and stores
 // whole lines, and the 4 rows of a row-pass workgroup share every line they
 // touch.  y counts rows over all planes of the chunk (NY is a multiple of 4).
 __device__ __forceinline__ unsigned qidx(int y, int x, int NX) {
   return (unsigned)(((((y >> 2) * (NX >> 2)) + (x >> 2)) << 4) + ((y & 3) << 2) + (x & 3));
+}
+// element (y, x) of the K / W scratch in the launch's layout
+__device__ __forceinline__ unsigned widx(const KArgs& a, int y, int x, int NX) {
+  return a.quad ? qidx(y, x, NX) : (unsigned)(y * NX + x);
 }
 
 // twiddle tables of the arithmetic type T (float2 or double2)

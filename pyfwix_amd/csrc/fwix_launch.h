@@ -83,6 +83,9 @@ int col_wpe_set(int kind, int mode, int n, int wpe);
 // per column, 1 two waves per column (n = 1024 only); -1 if invalid.
 int col_layout_get(int kind, int mode, int n);
 int col_layout_set(int kind, int mode, int n, int layout);
+// Layout of the K / W scratch for column length n: 1 quad-block, 0 row-major; -1 if invalid.
+int scratch_quad_get(int n);
+int scratch_quad_set(int n, int quad);
 // out[k] += sum of g over the planes of shot k (first[k], count[k]); m per plane
 hipError_t launch_plane_sum(const float2* g, float2* out, const int* first, const int* count, int nshot, long long m,
                             hipStream_t st);

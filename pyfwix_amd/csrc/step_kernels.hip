@@ -148,7 +148,7 @@ __device__ __forceinline__ void rows_quad_store(const float2* lds, float2* __res
 }
 
 // out = (add ? out : 0) + post(scale * F_x(pre * in))   along x, one row per wave
-// (in / out in the quad-block layout when a.qin / a.qout: staged through LDS)
+// (in / out are K when qin / qout: in the quad-block layout, staged through LDS)
 template <int N, bool INV, bool PRE_TAPER, bool POST_TAPER>
 __global__ __launch_bounds__(kThreads) void k_rows(KArgs a) {
   using C = RowCfg<N>;
@@ -158,12 +158,15 @@ __global__ __launch_bounds__(kThreads) void k_rows(KArgs a) {
   const int nrows = a.np * a.NY;
   const int R0 = blockIdx.x * kWaves * C::NB;
   const int rb = R0 + wave * C::NB;
-  const bool staged = a.qin || a.qout;  // grid-uniform: every wave reaches the barriers
+  // K in the quad-block layout goes through LDS as whole lines; row-major K
+  // is read and written like any wavefield
+  const bool qin = a.qin && a.quad, qout = a.qout && a.quad;
+  const bool staged = qin || qout;  // grid-uniform: every wave reaches the barriers
   if (rb >= nrows && !staged) return;
   float2* wl = lds + wave * C::LDSW;
   const float s = a.scale;  // row normalisation 1/sqrt(NX)
   float2 v[C::V];
-  if (a.qin) {
+  if (qin) {
     rows_quad_load<N, C::NB>(lds, a.in, R0, nrows);
     __syncthreads();
   }
@@ -175,12 +178,12 @@ __global__ __launch_bounds__(kThreads) void k_rows(KArgs a) {
       const int lr = rb + b;
       float2 x = czero();
       if (lr < nrows) {
-        x = a.qin ? wl[PadLds<N>::addr(b, i)] : a.in[(unsigned)(lr * N + i)];
+        x = qin ? wl[PadLds<N>::addr(b, i)] : a.in[(unsigned)(lr * N + i)];
         if (PRE_TAPER) x = cscale(x, a.tapx[i] * a.tapy[lr % a.NY]);
       }
       v[e] = x;
     }
-    if (a.qin) wave_sync();  // the transform reuses the region it was read from
+    if (qin) wave_sync();  // the transform reuses the region it was read from
     F::run(v, wl, a.twx, lane);
   }
   auto post = [&](float2 y, int lr, int i, unsigned idx) {
@@ -189,7 +192,7 @@ __global__ __launch_bounds__(kThreads) void k_rows(KArgs a) {
     if (a.add) y = cadd(y, a.out[idx]);
     return y;
   };
-  if (a.qout) {
+  if (qout) {
     if (rb < nrows) {
       wave_sync();
 #pragma unroll
@@ -275,7 +278,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_fanout(KArgs a) {
       int b, i;
       edge_pos<N, C::NB, F::P::last(), false>(lane, e, b, i);
       const int lr = rb + b;
-      if (lr < nrows) W[qidx(lr, i, N)] = to_f2(cscale(x[e], s));
+      if (lr < nrows) W[widx(a, lr, i, N)] = to_f2(cscale(x[e], s));
     }
   }
 }
@@ -338,7 +341,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
       int b, i;
       edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
       const int lr = rb + b;
-      nxt[e] = lr < nrows ? W[qidx(lr, i, N)] : czero();
+      nxt[e] = lr < nrows ? W[widx(a, lr, i, N)] : czero();
     }
   };
   // absent references contribute nothing to these rows: skip their loads and
@@ -383,7 +386,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
       int b, i;
       edge_pos<N, C::NB, Ff::P::last(), false>(lane, e, b, i);
       const int lr = rb + b;
-      if (lr < nrows) a.K[qidx(lr, i, N)] = cscale(acc[e], s);
+      if (lr < nrows) a.K[widx(a, lr, i, N)] = cscale(acc[e], s);
     }
   }
 }
@@ -686,7 +689,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   col_tile<C>(p, c0);
   const size_t pb = (size_t)p * a.NX * N;
   T* priv = (T*)smem + wave * C::PRIV;
-  const TileOffs<C> to(a.NX, c0, true);
+  const TileOffs<C> to(a.NX, c0, a.quad != 0);
   tile_load<C>(smem, PlaneBuf(a.K + pb, plane_bytes(a)), to);
   const int iw = plane_iw(a, p);
   const T* tw = twy_of(a, (T*)nullptr);
@@ -751,7 +754,7 @@ __global__ __launch_bounds__(ColCfg<N>::THREADS) __attribute__((amdgpu_waves_per
   col_tile<C>(p, c0);
   const size_t pb = (size_t)p * a.NX * N;
   T* priv = (T*)smem + wave * C::PRIV;
-  const TileOffs<C> to(a.NX, c0, true);
+  const TileOffs<C> to(a.NX, c0, a.quad != 0);
   const int iw = plane_iw(a, p);
   const float w2 = a.w2[iw];
   const int colw = c0 + wave * C::NB;
@@ -989,7 +992,7 @@ __global__ __launch_bounds__(SplitCfg::THREADS) __attribute__((amdgpu_waves_per_
   col_tile<C>(p, c0);
   const size_t pb = (size_t)p * a.NX * C::NROW;
   float2* priv = smem + wave * C::PRIV;
-  const TileOffs<C> to(a.NX, c0, true);
+  const TileOffs<C> to(a.NX, c0, a.quad != 0);
   {
     TileRegs<C> t;
     t.fetch(PlaneBuf(a.K + pb, plane_bytes(a)), to);
@@ -1065,7 +1068,7 @@ __global__ __launch_bounds__(SplitCfg::THREADS) __attribute__((amdgpu_waves_per_
   col_tile<C>(p, c0);
   const size_t pb = (size_t)p * a.NX * C::NROW;
   float2* priv = smem + wave * C::PRIV;
-  const TileOffs<C> to(a.NX, c0, true);
+  const TileOffs<C> to(a.NX, c0, a.quad != 0);
   const int iw = plane_iw(a, p);
   const float w2 = a.w2[iw];
   const int im = model_iw_any(a, iw);
@@ -1312,6 +1315,31 @@ int col_wpe_get(int kind, int mode, int n) {
 // and mode: 0 one wave per column (256-thread workgroups, 4 columns), 1 two
 // waves per column (512-thread workgroups, 4 columns, radix-2 split).
 static int g_layout[2][3] = {{1, 1, 1}, {1, 1, 1}};
+// Layout of the K / W scratch per lateral size (the column length NY): 1
+// quad-block, 0 row-major.  The quad-block layout pays where the column tiles
+// are 4 columns wide (1024-point columns, whole lines instead of quarter
+// lines); at 8 columns the row passes' partial-line accesses cost more than
+// the column passes gain.
+static int g_quad[16] = {};
+static bool g_quad_init = false;
+static void quad_init() {
+  if (g_quad_init) return;
+  g_quad[size_slot(1024)] = 1;
+  g_quad_init = true;
+}
+int scratch_quad_get(int n) {
+  const int i = size_slot(n);
+  if (i < 0) return -1;
+  quad_init();
+  return g_quad[i];
+}
+int scratch_quad_set(int n, int quad) {
+  const int i = size_slot(n);
+  if (i < 0 || (quad != 0 && quad != 1)) return -1;
+  quad_init();
+  g_quad[i] = quad;
+  return 0;
+}
 int col_layout_get(int kind, int mode, int n) {
   if (kind < 0 || kind > 1 || mode < 0 || mode > 2 || size_slot(n) < 0) return -1;
   return n == 1024 ? g_layout[kind][mode] : 0;

@@ -147,3 +147,50 @@ def test_1024_column_layouts_match_oracle(dev, nref):
             assert rel(a, refs[name][1]) < TOL_WFLD, (lw, name, "adjoint")
             assert rel(f, outs[(0, 2)][name][0]) < 2e-6, (lw, name, "forward vs one wave")
             assert rel(a, outs[(0, 2)][name][1]) < 2e-6, (lw, name, "adjoint vs one wave")
+
+
+@pytest.mark.parametrize("NX,NY", [(64, 48), (48, 1024)])
+def test_scratch_layouts_give_identical_steps(dev, NX, NY):
+    """The fused passes' scratch in quad blocks or row-major: the same PSPI /
+    NSPS steps and scattering terms bit for bit (only memory placement
+    differs)."""
+    import torch
+    from fwix_testutil import dims_of, grid, smooth_slowness
+    from pyfwix_amd import _lib
+    from pyfwix_amd.ops import FWIX_NSPS, FWIX_PSPI, OneStep, Scatter
+    g = grid(NX, NY, 3, 1, nx=NX - 8, ny=NY - 8, nz=2)
+    rng = np.random.default_rng(NX + NY)
+    nref = 3
+    sref = (1.0 / rng.uniform(1500, 3000, (2, nref, g.NW)) ** 2).astype(np.complex64)
+    labels = rng.integers(0, nref, (2, g.NW, g.NY, g.NX)).astype(np.uint8)
+    slow = smooth_slowness(rng, g, imag=0.01)
+    shape = (g.NS, g.NW, g.NY, g.NX)
+    x = torch.from_numpy(crand(rng, shape).astype(np.complex64)).to(dev)
+    before = _lib.lib.fwix_tune_scratch_layout_get(NY)
+    outs = {}
+    try:
+        for quad in (0, 1):
+            _lib.call("fwix_tune_scratch_layout_set", NY, quad)
+            res = []
+            for method in (FWIX_PSPI, FWIX_NSPS):
+                op = OneStep(method, dims_of(g), nref=nref, taperx=4, tapery=4)
+                op.set_refs(torch.from_numpy(sref).to(dev), torch.from_numpy(labels).to(dev))
+                op.set_depth(1)
+                f, a = torch.zeros_like(x), torch.zeros_like(x)
+                op.forward(False, x, f)
+                op.adjoint(False, a, x)
+                res += [f, a]
+            sc = Scatter(dims_of(g), taylor=2, eps_scat=0.04)
+            sc.set_slow(torch.from_numpy(slow.astype(np.complex64)).to(dev))
+            sc.set_depth(1)
+            f, a = torch.zeros_like(x), torch.zeros_like(x)
+            sc.forward(False, x, f)
+            sc.adjoint(False, a, x)
+            res += [f, a]
+            torch.cuda.synchronize()
+            outs[quad] = [r.cpu().numpy() for r in res]
+    finally:
+        _lib.call("fwix_tune_scratch_layout_set", NY, before)
+    for p, q in zip(outs[0], outs[1]):
+        assert np.abs(p).max() > 0
+        assert np.array_equal(p, q)
