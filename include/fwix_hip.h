@@ -209,6 +209,12 @@ int fwix_oneway_set_frequencies(fwix_op op, const float* f_hz);
  * and (nz, nshot, NY, NX), and must be set again.  No counterpart in the
  * reference, which runs one shot geometry per operator. */
 int fwix_oneway_set_shot_planes(fwix_op op, int nshot, const int* nplanes);
+/* Consecutive steps of the "shared spectrum" shape (PSPI forward, NSPS
+ * adjoint) chain through the row spectrum: a step's select pass also leaves
+ * F_x(taper * U_{z+1}) for the next step, which skips its first row pass.
+ * on = 1 (default) chains, 0 runs every step whole; results agree to float32
+ * rounding.  No counterpart in the reference. */
+int fwix_oneway_set_step_chaining(fwix_op op, int on);
 int fwix_oneway_forward(fwix_op op, int add, const fwix_c64* model, fwix_c64* data, fwix_c64* ckpt, fwix_stream s);
 int fwix_oneway_adjoint(fwix_op op, int add, fwix_c64* model, const fwix_c64* data, fwix_stream s);
 

@@ -95,6 +95,7 @@ _SIGS = {
     "fwix_ic_adjoint": [_P, _I, _P, _P, _P],
     "fwix_oneway_create": [_I, _I, _D, _PP, _OPP],
     "fwix_oneway_set_refs": [_P, _P, _P],
+    "fwix_oneway_set_step_chaining": [_P, _I],
     "fwix_oneway_set_frequencies": [_P, _P],
     "fwix_oneway_set_shot_planes": [_P, _I, _P],
     "fwix_oneway_forward": [_P, _I, _P, _P, _P, _P],

@@ -237,10 +237,15 @@ struct OneWayOp : fwix_op_s {
     const int nz = d.nz;
     // Downward forward and Upward adjoint step z = 0..nz-1; the others z = nz-1..0
     const bool ascending = (direction == 0) != adjoint;
+    // consecutive steps chain through the row spectrum (PSPI forward, NSPS
+    // adjoint); w itself is written where a checkpoint or the result reads it
+    bool kr = false;
     for (int k = 0; k < nz; ++k) {
       const int z = ascending ? k : nz - 1 - k;
       if (ckpt) hip_check(launch_axpy(w, ckpt + (size_t)z * n, n, 1.f, 0, st), "checkpoint");
-      eng.apply(adjoint, z, 0, w, w, st);
+      const bool kn = k + 1 < nz;
+      eng.apply_chained(z, w, w, kr, kn, st, ckpt != nullptr || !kn, adjoint);
+      kr = kn && eng.chains(adjoint);
     }
     hip_check(launch_axpy(w, out, n, 1.f, add, st), "accumulate");
   }
@@ -934,6 +939,13 @@ int fwix_oneway_set_shot_planes(fwix_op op, int nshot, const int* nplanes) {
 }
 int fwix_oneway_set_frequencies(fwix_op op, const float* f_hz) {
   return guard([&] { as<OneWayOp>(op, KIND_ONEWAY)->eng.tables.set_frequencies(f_hz, false); });
+}
+int fwix_oneway_set_step_chaining(fwix_op op, int on) {
+  return guard([&] {
+    auto* p = as<OneWayOp>(op, KIND_ONEWAY);
+    p->eng.chain = on != 0;
+    if (!on) p->eng.Kall.release();
+  });
 }
 int fwix_oneway_set_refs(fwix_op op, const fwix_c64* sref, const uint8_t* labels) {
   return guard([&] {

@@ -427,8 +427,8 @@ void StepEngine::apply(bool adjoint, int iz, int add, const float2* in, float2* 
 }
 
 void StepEngine::apply_chained(int iz, const float2* in, float2* out, bool k_ready, bool k_next, hipStream_t st,
-                               bool out_needed) {
-  if (!chains()) return apply(false, iz, 0, in, out, st);
+                               bool out_needed, bool adjoint) {
+  if (!chains(adjoint)) return apply(adjoint, iz, 0, in, out, st);
   check_ready(iz);
   const fwix_dims& d = tables.dims;
   const size_t bytes = (size_t)d.NX * d.NY * d.NW * d.NS * sizeof(float2);
@@ -437,7 +437,7 @@ void StepEngine::apply_chained(int iz, const float2* in, float2* out, bool k_rea
     Kall.alloc(bytes);
   }
   const std::function<double(int, int)> rows = [&](int c0, int np) { return present_rows(iz, c0, np); };
-  step_passes(method, false, d, params, base(iz, 0, chunk), chunk, 0, in, out, st, &rows, (float2*)Kall.p, k_ready,
+  step_passes(method, adjoint, d, params, base(iz, 0, chunk), chunk, 0, in, out, st, &rows, (float2*)Kall.p, k_ready,
               k_next, out_needed || !k_next);
 }
 

@@ -137,18 +137,21 @@ struct StepEngine {
   KArgs base(int iz, int c0, int np) const;
   void apply(bool adjoint, int iz, int add, const float2* in, float2* out, hipStream_t st);
   void apply_plain(bool adjoint, int iz, int add, const float2* in, float2* out, hipStream_t st);
-  // Forward steps of a recursion chained through the row spectrum (Kall, all
+  // Steps of a recursion chained through the row spectrum (Kall, all
   // planes): k_ready = the previous chained step left F_x(taper * in) in Kall
   // (nothing changed in between), k_next = leave F_x(taper * out) for the next
   // step.  Falls back to apply() where chaining does not apply (plain path,
-  // masked-input shape).  chain = false turns chaining off (every step whole).
+  // the masked-input shape: PSPI adjoint, NSPS forward).  chain = false turns
+  // chaining off (every step whole).
   DevBuf Kall;
   bool chain = true;
-  bool chains() const { return chain && !plain && method == FWIX_PSPI; }
+  // the steps of this direction have the "shared spectrum" shape (PSPI
+  // forward, NSPS adjoint): a row pass, the column fan-out, the select pass
+  bool chains(bool adjoint = false) const { return chain && !plain && ((method == FWIX_PSPI) != adjoint); }
   // out_needed = false (only with k_next): nobody reads out before the next
   // chained step overwrites it, so it is not written at all
   void apply_chained(int iz, const float2* in, float2* out, bool k_ready, bool k_next, hipStream_t st,
-                     bool out_needed = true);
+                     bool out_needed = true, bool adjoint = false);
 };
 
 

@@ -303,6 +303,11 @@ class OneWay:
     def set_frequencies(self, f_hz) -> None:
         call("fwix_oneway_set_frequencies", self._h.h, _vp(_f32(f_hz, self.dims.NW, "frequencies")))
 
+    def set_step_chaining(self, on: bool) -> None:
+        """Chain consecutive PSPI-forward / NSPS-adjoint steps through the row spectrum
+        (default on); off runs every step whole (same results to float32 rounding)."""
+        call("fwix_oneway_set_step_chaining", self._h.h, int(bool(on)))
+
     def forward(self, add, model, data, checkpoints: torch.Tensor | None = None):
         s = _wshape(self.dims)
         _check(model, s, name="model")

@@ -122,8 +122,10 @@ def test_batch_recomputed_segments_and_dot(dev):
     seg = ShotBatch(shots, NZ, D, D, D, PARAMS, ckpt_every=3, device=dev)
     d_seg, g_seg = _run(seg, slows, dss)
     torch.cuda.synchronize()
-    assert rel(d_seg.cpu().numpy(), d_full.cpu().numpy()) < 1e-6
-    assert rel(g_seg.cpu().numpy(), g_full.cpu().numpy()) < 1e-6
+    # the stored run chains all its background steps; a recomputed segment's
+    # first step runs its own row pass: equal up to float32 rounding order
+    assert rel(d_seg.cpu().numpy(), d_full.cpu().numpy()) < 5e-6
+    assert rel(g_seg.cpu().numpy(), g_full.cpu().numpy()) < 5e-6
     # dot test over the whole batched model space (nz, nshot, ny, nx)
     m = torch.from_numpy(crand(rng, tuple(full.dslow.shape)).astype(np.complex64) * 1e-8).to(dev)
     dd = torch.from_numpy(crand(rng, (full.nsamp,)).astype(np.complex64)).to(dev)

@@ -545,3 +545,35 @@ def test_born_step_chaining_matches_whole_steps(dev, ckpt):
     for a, b in zip(out[True], out[False]):
         assert np.abs(b).max() > 0
         assert rel(a, b) < 1e-6
+
+
+@pytest.mark.parametrize("direction", [0, 1])
+@pytest.mark.parametrize("method", ["pspi", "nsps"])
+def test_oneway_step_chaining_matches_whole_steps(dev, direction, method):
+    """Downward / Upward recursions with chained steps (PSPI forward and NSPS
+    adjoint: the select pass leaves the next step's row spectrum) give the
+    whole steps' results, with and without per-depth checkpoints."""
+    import torch
+    from pyfwix_amd.ops import FWIX_NSPS, FWIX_PSPI, OneWay
+    rng = np.random.default_rng(61 + direction)
+    g = grid(48, 64, 3, 2, nx=36, ny=48, nz=6)
+    slow, sref, labels = _refs(g, rng, 3)
+    tap = 6 if method == "pspi" else 0
+    shape = (g.NS, g.NW, g.NY, g.NX)
+    x = _t(crand(rng, shape), dev)
+    out = {}
+    for chain in (True, False):
+        op = OneWay(direction, FWIX_PSPI if method == "pspi" else FWIX_NSPS, dims_of(g), nref=3,
+                    taperx=tap, tapery=tap)
+        op.set_refs(_t(sref, dev), _t(labels, dev))
+        op.set_step_chaining(chain)
+        f, f2, a = _z(shape, dev), _z(shape, dev), _z(shape, dev)
+        ck = _z((g.nz, *shape), dev)
+        op.forward(False, x, f)
+        op.forward(False, x, f2, ck)
+        op.adjoint(False, a, x)
+        torch.cuda.synchronize()
+        out[chain] = [t.cpu().numpy() for t in (f, f2, ck, a)]
+    for p, q in zip(out[True], out[False]):
+        assert np.abs(q).max() > 0
+        assert rel(p, q) < 1e-6
