@@ -650,7 +650,17 @@ __global__ __launch_bounds__(kThreads) void k_plane_sum(const float2* __restrict
     const long long j = t - (long long)k * m;
     const float2* src = g + (size_t)first[k] * m + j;
     float2 acc = out[t];
-    for (int p = 0; p < count[k]; ++p) acc = cadd(acc, src[(size_t)p * m]);
+    // eight planes' loads in flight per thread; the sum keeps plane order
+    const int c = count[k];
+    int p = 0;
+    for (; p + 8 <= c; p += 8) {
+      float2 v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = src[(size_t)(p + q) * m];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) acc = cadd(acc, v[q]);
+    }
+    for (; p < c; ++p) acc = cadd(acc, src[(size_t)p * m]);
     out[t] = acc;
   }
 }
