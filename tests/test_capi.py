@@ -138,3 +138,10 @@ def test_column_budget_table():
     for bad in ((0, 0, 512, 1), (2, 0, 1024, 1), (0, 3, 1024, 0), (0, 0, 1024, 2), (0, 0, 500, 0)):
         with pytest.raises(RuntimeError, match="tune_columns_layout"):
             autotune.set_layout(*bad)
+    # scratch layout: quad blocks at 1024 points by default, row-major below
+    from pyfwix_amd import _lib
+    assert _lib.lib.fwix_tune_scratch_layout_get(1024) == 1 and _lib.lib.fwix_tune_scratch_layout_get(512) == 0
+    assert _lib.lib.fwix_tune_scratch_layout_get(500) == -1
+    for bad in ((500, 0), (512, 2), (1024, -1)):
+        with pytest.raises(RuntimeError, match="tune_scratch_layout"):
+            _lib.call("fwix_tune_scratch_layout_set", *bad)
