@@ -178,10 +178,13 @@ __device__ __forceinline__ void wave_sync() {
 // Butterfly t = lane + 64*q of a radix-R stage owns batch b and index j.
 // BFAST: batch index varies fastest across lanes (column tiles), else the
 // position within the batch does (rows).
+// (Unsigned arithmetic: t is never negative, and the lane index is opaque to
+// the compiler inside a transform, so signed division would cost sign fix-ups.)
 template <int N, int NB, int R, bool BFAST>
 __device__ __forceinline__ void bj(int t, int& b, int& j) {
-  if (BFAST) { b = t % NB; j = t / NB; }
-  else { b = t / (N / R); j = t % (N / R); }
+  const unsigned u = (unsigned)t;
+  if (BFAST) { b = (int)(u % NB); j = (int)(u / NB); }
+  else { b = (int)(u / (N / R)); j = (int)(u % (N / R)); }
 }
 
 // (batch, position) of value v = q*R + r at the input of a stage of radix R
@@ -212,7 +215,7 @@ struct WaveFft {
       if constexpr (Ns > 1) {
         int b, j;
         bj<N, NB, R, BFAST>(lane + 64 * q, b, j);
-        const int k = j % Ns;
+        const int k = (int)((unsigned)j % Ns);
         constexpr int step = N / (Ns * R);
 #pragma unroll
         for (int r = 1; r < R; ++r) {
@@ -231,7 +234,8 @@ struct WaveFft {
     for (int q = 0; q < V / R; ++q) {
       int b, j;
       bj<N, NB, R, BFAST>(lane + 64 * q, b, j);
-      const int base = (j / Ns) * Ns * R + (j % Ns);
+      const unsigned uj = (unsigned)j;
+      const int base = (int)((uj / Ns) * Ns * R + (uj % Ns));
 #pragma unroll
       for (int r = 0; r < R; ++r) lds[L::addr(b, base + r * Ns)] = v[q * R + r];
     }

@@ -46,7 +46,7 @@ constexpr int kWaves = kThreads / 64;
 constexpr int padded(int N) { return N + N / 32; }
 template <int N>
 struct PadLds {  // one wave's NB batches, batch-major, padded
-  __device__ __forceinline__ static int addr(int b, int i) { return b * padded(N) + i + (i >> 5); }
+  __device__ __forceinline__ static int addr(int b, int i) { return b * padded(N) + i + (int)((unsigned)i >> 5); }
 };
 
 // One argument block for every step kernel (passed by value).
@@ -100,9 +100,18 @@ struct KArgs {
 __device__ __forceinline__ unsigned qidx(int y, int x, int NX) {
   return (unsigned)(((((y >> 2) * (NX >> 2)) + (x >> 2)) << 4) + ((y & 3) << 2) + (x & 3));
 }
-// element (y, x) of the K / W scratch in the launch's layout
+// element (y, x) of the K / W scratch in the launch's layout, as a row part
+// (uniform over a row) plus a column part: quad blocks put (y, x) at
+// ((y >> 2) NX + (y & 3)) 4 + ((x >> 2) << 4) + (x & 3)
+__device__ __forceinline__ unsigned wrow(const KArgs& a, int y, int NX) {
+  return a.quad ? (unsigned)(((y >> 2) * NX + (y & 3)) << 2) : (unsigned)(y * NX);
+}
+__device__ __forceinline__ unsigned wcol(const KArgs& a, int x) {
+  const unsigned u = (unsigned)x;
+  return a.quad ? ((u >> 2) << 4) + (u & 3u) : u;
+}
 __device__ __forceinline__ unsigned widx(const KArgs& a, int y, int x, int NX) {
-  return a.quad ? qidx(y, x, NX) : (unsigned)(y * NX + x);
+  return wrow(a, y, NX) + wcol(a, x);
 }
 
 // twiddle tables of the arithmetic type T (float2 or double2)

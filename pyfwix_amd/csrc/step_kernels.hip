@@ -336,10 +336,14 @@ __global__ __launch_bounds__(kThreads) void k_rows_select(KArgs a) {
   float2 nxt[C::V];
   auto fetch = [&](int r) {
     const float2* W = a.W + (size_t)r * a.wstride;
+    // the lane index made opaque here: the per-element offsets are recomputed
+    // per reference (a few integer ops) instead of held in registers
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
 #pragma unroll
     for (int e = 0; e < C::V; ++e) {
       int b, i;
-      edge_pos<N, C::NB, F::P::first(), false>(lane, e, b, i);
+      edge_pos<N, C::NB, F::P::first(), false>(ln, e, b, i);
       const int lr = rb + b;
       nxt[e] = lr < nrows ? W[widx(a, lr, i, N)] : czero();
     }
