@@ -87,12 +87,13 @@ def shot_refs(slow3: np.ndarray, nref: int, NX: int, NY: int):
 def launch_bytes(planes: int, nshot: int, NX: int, NY: int, nx: int, ny: int, nz: int, ckpt: int) -> int:
     """HBM held by one shot-batch launch (Born forward + adjoint): the background
     checkpoints (every ckpt-th depth) and the recomputed segment, three working
-    wavefields, the chunk scratch (at most 4 GiB plus its spectra), the per-plane
-    gradient of one depth, and the per-shot slowness, model, gradient and labels."""
+    wavefields, the chunk scratch (at most 8 GiB plus its spectra) and the
+    chained-step spectrum of all planes, the per-plane gradient of one depth,
+    and the per-shot slowness, model, gradient and labels."""
     slice_ = planes * NX * NY * 8
     nck = -(-nz // ckpt)
     seg = ckpt if ckpt > 1 else 0
-    scratch = min(slice_ * 3, 5 << 30)
+    scratch = min(slice_ * 3, 9 << 30) + slice_
     model = nz * nshot * (3 * nx * ny * 8 + NX * NY + 4 * NY)
     return slice_ * (nck + seg + 3) + scratch + planes * nx * ny * 8 + model
 

@@ -238,14 +238,20 @@ int auto_chunk(const fwix_dims& d, int nterm, int requested) {
   if (requested > 0) return std::min(requested, c_index);
   // Enough planes that a column pass launches >= 4096 workgroups (16 per CU)
   // and a row pass >= 16 waves per SIMD slot, so every launch reaches a
-  // steady state of loads overlapping transforms instead of one burst; at
-  // most 4 GiB of scratch (spectrum + nterm fan-out planes).  Chunks are
+  // steady state of loads overlapping transforms instead of one burst.  The
+  // 1024-point columns (4 columns per workgroup on wave pairs) gain from far
+  // longer launches: 32768 workgroups, 128 planes of 1024^2 (PSPI Born,
+  // 128 frequencies: 16 planes per launch 32.1, 32 planes 33.8, 128 planes
+  // 34.9 Gpts*freq/s), while 768-point NSPS steps with 8 references lose a
+  // little past the 4096 mark (34 planes 21.45, 50 or 100 planes 21.31).
+  // At most 8 GiB of scratch (spectrum + nterm fan-out planes).  Chunks are
   // then balanced so the last one is not a small tail.
   const double per_plane = (double)(1 + nterm) * d.NX * d.NY * 8.0;
   const int cw = fft_size_supported(d.NY) ? col_tile_width(d.NY) : 16;
   const int blocks_per_plane = (d.NX + cw - 1) / cw;
-  int c = (4096 + blocks_per_plane - 1) / blocks_per_plane;
-  const int c_max = (int)(4096.0 * 1024 * 1024 / per_plane);
+  const int target = d.NY == 1024 ? 32768 : 4096;
+  int c = (target + blocks_per_plane - 1) / blocks_per_plane;
+  const int c_max = (int)(8192.0 * 1024 * 1024 / per_plane);
   if (c > c_max) c = c_max;
   if (c < 1) c = 1;
   if (c > c_index) c = c_index;
