@@ -378,7 +378,8 @@ def bench_shot_batch(args):
             _lib.timing_select(None, max(1, args.pass_sample))
             _lib.timing_enable(1 << 15)
         for idx, NX, NY, k in launches:
-            b = ShotBatch([shots[i] for i in idx], nz, d, d, d, params, ckpt_every=k, device=dev, NX=NX, NY=NY)
+            b = ShotBatch([shots[i] for i in idx], nz, d, d, d, params, ckpt_every=k, device=dev, NX=NX, NY=NY,
+                          planes_per_chunk=args.chunk)
             b.set_slowness([models[i][0] for i in idx], [models[i][2] for i in idx])
             b.set_perturbation([models[i][1] for i in idx])
             torch.cuda.synchronize()

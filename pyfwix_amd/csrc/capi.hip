@@ -236,20 +236,22 @@ int auto_chunk(const fwix_dims& d, int nterm, int requested) {
   // (row * NX + x), so one chunk stays below 2^31 complex samples.
   const int c_index = (int)std::min<long long>(planes, ((1LL << 31) - 1) / ((long long)d.NX * d.NY));
   if (requested > 0) return std::min(requested, c_index);
-  // Enough planes that a column pass launches >= 4096 workgroups (16 per CU)
-  // and a row pass >= 16 waves per SIMD slot, so every launch reaches a
-  // steady state of loads overlapping transforms instead of one burst.  The
-  // 1024-point columns (4 columns per workgroup on wave pairs) gain from far
-  // longer launches: 32768 workgroups, 128 planes of 1024^2 (PSPI Born,
-  // 128 frequencies: 16 planes per launch 32.1, 32 planes 33.8, 128 planes
-  // 34.9 Gpts*freq/s), while 768-point NSPS steps with 8 references lose a
-  // little past the 4096 mark (34 planes 21.45, 50 or 100 planes 21.31).
-  // At most 8 GiB of scratch (spectrum + nterm fan-out planes).  Chunks are
-  // then balanced so the last one is not a small tail.
+  // Enough planes that a column pass launches >= 32768 workgroups (128 per
+  // CU), so every launch reaches a steady state of loads overlapping
+  // transforms and the partly filled last round of workgroups is a small part
+  // of it.  Born steps (PSPI, 4 references) per padded size, planes per
+  // launch -> Gpts*freq/s: 1024^2 16 -> 32.1, 32 -> 33.8, 128 -> 34.9;
+  // 768^2 43 -> 37.5, 128 -> 39.4, 256 -> 39.8; 512^2 64 -> 55.1, 256 ->
+  // 57.6; 384^2 171 -> 31.0, 256 -> 32.0.  Steps with more than 4 fan-out
+  // planes keep >= 4096 workgroups (16 per CU): NSPS at 768^2 with 8
+  // references, 100 planes, ran 21.25-21.45 at 34 planes per launch and
+  // 20.84-21.31 at 50 or 100.  At most 8 GiB of scratch (spectrum + nterm
+  // fan-out planes).  Chunks are then balanced so the last one is not a
+  // small tail.
   const double per_plane = (double)(1 + nterm) * d.NX * d.NY * 8.0;
   const int cw = fft_size_supported(d.NY) ? col_tile_width(d.NY) : 16;
   const int blocks_per_plane = (d.NX + cw - 1) / cw;
-  const int target = d.NY == 1024 ? 32768 : 4096;
+  const int target = nterm > 4 ? 4096 : 32768;
   int c = (target + blocks_per_plane - 1) / blocks_per_plane;
   const int c_max = (int)(8192.0 * 1024 * 1024 / per_plane);
   if (c > c_max) c = c_max;
