@@ -203,14 +203,15 @@ def test_oneway_matches_oracle(dev, direction, method):
     assert dot_err(x, out.cpu().numpy(), y, Aty.cpu().numpy()) < TOL_DOT
 
 
-def _born_setup(dev, ckpt=1, seed=5, NS=2, tap=4, nz=6, size=(48, 48, 4, 32, 36), ow=6.0):
+def _born_setup(dev, ckpt=1, seed=5, NS=2, tap=4, nz=6, size=(48, 48, 4, 32, 36), ow=6.0, chunk=0):
     from pyfwix_amd.ops import Born
     rng = np.random.default_rng(seed)
     NX, NY, NW, nx, ny = size
     g = grid(NX, NY, NW, NS, nx=nx, ny=ny, nz=nz, ow=ow, dw=3.0)
     slow, sref, labels = _refs(g, rng, 3)
     src, rec = _geometry(g, rng, nrec=48)
-    born = Born(dims_of(g), nref=3, taperx=tap, tapery=tap, taylor=1, eps_scat=0.04, ckpt_every=ckpt)
+    born = Born(dims_of(g), nref=3, taperx=tap, tapery=tap, taylor=1, eps_scat=0.04, ckpt_every=ckpt,
+                planes_per_chunk=chunk)
     born.set_refs(_t(sref, dev), _t(labels, dev))
     born.set_slow(_t(slow, dev))
     born.set_sources(*src)
@@ -518,17 +519,20 @@ def test_plain_born_equals_fused(dev, ck):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("ckpt", [1, 3])
-def test_born_step_chaining_matches_whole_steps(dev, ckpt):
+@pytest.mark.parametrize("ckpt,chunk", [(1, 0), (3, 0), (3, 3)])
+def test_born_step_chaining_matches_whole_steps(dev, ckpt, chunk):
     """Background steps chained through the row spectrum (a select pass leaves
     the next step's F_x(taper * U)) give the data and the Born forward and
     adjoint of whole steps, with every depth stored and with recomputed
     segments of 3, and a source injecting at depths 2-3 (where the chain must
-    break) besides the one at the top."""
+    break) besides the one at the top.  chunk = 3 runs the chained steps'
+    8 planes as launches of 3, 3 and 2 planes (the spectrum of every chunk
+    kept at its own offset), against whole steps in one launch."""
     import torch
     out = {}
     for chain in (True, False):
-        g, born, slow, sref, labels, src, rec, wav, rng = _born_setup(dev, ckpt=ckpt, seed=33, nz=8)
+        g, born, slow, sref, labels, src, rec, wav, rng = _born_setup(dev, ckpt=ckpt, seed=33, nz=8,
+                                                                      chunk=chunk if chain else 0)
         sz = np.array([g.oz + 0.3 * g.dz, g.oz + 2.6 * g.dz], np.float32)
         born.set_sources(src[0], src[1], sz, src[3])
         born.set_step_chaining(chain)
@@ -547,12 +551,15 @@ def test_born_step_chaining_matches_whole_steps(dev, ckpt):
         assert rel(a, b) < 1e-6
 
 
+@pytest.mark.parametrize("chunk", [0, 4])
 @pytest.mark.parametrize("direction", [0, 1])
 @pytest.mark.parametrize("method", ["pspi", "nsps"])
-def test_oneway_step_chaining_matches_whole_steps(dev, direction, method):
+def test_oneway_step_chaining_matches_whole_steps(dev, direction, method, chunk):
     """Downward / Upward recursions with chained steps (PSPI forward and NSPS
     adjoint: the select pass leaves the next step's row spectrum) give the
-    whole steps' results, with and without per-depth checkpoints."""
+    whole steps' results, with and without per-depth checkpoints; chunk = 4
+    runs the chained steps' 6 planes as launches of 4 and 2 planes, against
+    whole steps in one launch."""
     import torch
     from pyfwix_amd.ops import FWIX_NSPS, FWIX_PSPI, OneWay
     rng = np.random.default_rng(61 + direction)
@@ -564,7 +571,7 @@ def test_oneway_step_chaining_matches_whole_steps(dev, direction, method):
     out = {}
     for chain in (True, False):
         op = OneWay(direction, FWIX_PSPI if method == "pspi" else FWIX_NSPS, dims_of(g), nref=3,
-                    taperx=tap, tapery=tap)
+                    taperx=tap, tapery=tap, planes_per_chunk=chunk if chain else 0)
         op.set_refs(_t(sref, dev), _t(labels, dev))
         op.set_step_chaining(chain)
         f, f2, a = _z(shape, dev), _z(shape, dev), _z(shape, dev)
