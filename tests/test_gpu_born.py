@@ -519,20 +519,24 @@ def test_plain_born_equals_fused(dev, ck):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("ckpt,chunk", [(1, 0), (3, 0), (3, 3)])
-def test_born_step_chaining_matches_whole_steps(dev, ckpt, chunk):
+@pytest.mark.parametrize("ckpt,chunk,size", [(1, 0, None), (3, 0, None), (3, 3, None),
+                                             (3, 4, (1024, 1024, 3, 96, 80))])
+def test_born_step_chaining_matches_whole_steps(dev, ckpt, chunk, size):
     """Background steps chained through the row spectrum (a select pass leaves
     the next step's F_x(taper * U)) give the data and the Born forward and
     adjoint of whole steps, with every depth stored and with recomputed
     segments of 3, and a source injecting at depths 2-3 (where the chain must
     break) besides the one at the top.  chunk = 3 runs the chained steps'
     8 planes as launches of 3, 3 and 2 planes (the spectrum of every chunk
-    kept at its own offset), against whole steps in one launch."""
+    kept at its own offset), against whole steps in one launch; the 1024^2
+    case (6 planes as 4 + 2) runs the split column kernels and the
+    quad-block scratch layout."""
     import torch
     out = {}
     for chain in (True, False):
-        g, born, slow, sref, labels, src, rec, wav, rng = _born_setup(dev, ckpt=ckpt, seed=33, nz=8,
-                                                                      chunk=chunk if chain else 0)
+        kw = {"size": size, "nz": 6} if size else {"nz": 8}
+        g, born, slow, sref, labels, src, rec, wav, rng = _born_setup(dev, ckpt=ckpt, seed=33,
+                                                                      chunk=chunk if chain else 0, **kw)
         sz = np.array([g.oz + 0.3 * g.dz, g.oz + 2.6 * g.dz], np.float32)
         born.set_sources(src[0], src[1], sz, src[3])
         born.set_step_chaining(chain)
