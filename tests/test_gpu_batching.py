@@ -56,14 +56,17 @@ def _run(batch, slows, dss, data_in=None):
     return data, grad
 
 
-def test_batch_equals_shots_alone(dev):
+@pytest.mark.parametrize("chunk", [0, 3])
+def test_batch_equals_shots_alone(dev, chunk):
+    """chunk = 3: the batch's 16 planes run as launches of 3 planes, which
+    straddle the shots' plane ranges (5 + 8 + 3)."""
     import torch
     from pyfwix_amd.batching import ShotBatch, padded_size
     rng = np.random.default_rng(4)
     shots = _shots(rng)
     assert len({padded_size(s.nx) for s in shots} | {padded_size(s.ny) for s in shots}) == 1
     slows, dss = _models(rng, shots)
-    b = ShotBatch(shots, NZ, D, D, D, PARAMS, device=dev)
+    b = ShotBatch(shots, NZ, D, D, D, PARAMS, device=dev, planes_per_chunk=chunk)
     data, grad = _run(b, slows, dss)
     torch.cuda.synchronize()
     assert float(data.abs().max()) > 0
