@@ -8,7 +8,9 @@ slowness / references / model, ragged traces).
 * a batch with recomputed background segments equals one with every depth kept;
 * the batched Born pair passes the dot-product test;
 * a batch of one-way recursions (set_shot_planes on Downward) equals its shots
-  run alone.
+  run alone;
+* torch.autograd through a batch operator (BornFunction) gives the batch's
+  migration adjoint.
 """
 import numpy as np
 import pytest
@@ -233,3 +235,39 @@ def test_batch_equals_shots_alone_on_a_large_grid(dev):
         d1, g1 = _run(one, [slows[k]], [dss[k]])
         assert rel(b.shot_data(data, k).cpu().numpy(), one.shot_data(d1, 0).cpu().numpy()) < 1e-6
         assert rel(b.shot_grad(grad, k).cpu().numpy(), one.shot_grad(g1, 0).cpu().numpy()) < 1e-6
+
+
+def test_born_function_on_a_shot_batch(dev):
+    """torch.autograd through a shot-batch operator: BornFunction returns the
+    batch's ragged traces, and x.grad of 0.5 ||B ds - d||^2 is the batch's
+    migration adjoint of the residual, (nz, nshot, ny, nx)."""
+    import torch
+    from pyfwix_amd.autograd import born
+    from pyfwix_amd.batching import ShotBatch
+    rng = np.random.default_rng(9)
+    shots = _shots(rng)
+    slows, dss = _models(rng, shots)
+    b = ShotBatch(shots, NZ, D, D, D, PARAMS, device=dev)
+    b.set_slowness(slows)
+    b.set_perturbation(dss)
+    b.model()
+    d_ref = b.forward()
+    x = b.dslow.clone().requires_grad_(True)
+    assert x.shape == b.op.model_shape()
+    d = born(b.op, x)
+    assert d.shape == d_ref.shape
+    assert torch.equal(d.detach(), d_ref)
+    d_obs = torch.from_numpy(crand(rng, tuple(d.shape)).astype(np.complex64)).to(dev) * float(d_ref.abs().max())
+    r = (d_ref - d_obs).contiguous()
+    d.backward(r)
+    g_ref = b.adjoint(r)
+    torch.cuda.synchronize()
+    assert float(g_ref.abs().max()) > 0
+    assert rel(x.grad.cpu().numpy(), g_ref.cpu().numpy()) < 1e-6
+    # through a loss: torch forms the residual from abs and pow with float32
+    # rounding of its own, so the gradient agrees to a few float32 ulps
+    x2 = b.dslow.clone().requires_grad_(True)
+    loss = 0.5 * (born(b.op, x2) - d_obs).abs().pow(2).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    assert rel(x2.grad.cpu().numpy(), g_ref.cpu().numpy()) < 1e-5
