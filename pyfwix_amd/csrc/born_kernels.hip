@@ -141,7 +141,12 @@ __global__ __launch_bounds__(kThreads) void k_rows_ic_fwd(KArgs a) {
 // BORN = true : K = scale * F_x( taper * (y + addin) )   (feeds the PSPI column pass)
 // BORN = false: out = (add ? out : 0) + y                (standalone Scatter forward)
 // (T: arithmetic type, float2 or double2; loads and stores stay complex64)
-template <int N, bool BORN, bool MAP, class T = float2>
+// ONE: the launch has exactly one Taylor order (nterm == 2).  Its factors
+// s~^(-1/2), s~^(-3/2) and the source v are then formed after the row
+// transform, so that only the transformed row is live across it (the
+// general form holds the running sum and two factor arrays across every
+// transform).
+template <int N, bool BORN, bool MAP, class T = float2, bool ONE = false>
 __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
   using C = BRowCfg<N>;
   using S = real_of<T>;
@@ -160,47 +165,83 @@ __global__ __launch_bounds__(kThreads) void k_rows_scatter(KArgs a) {
   __syncthreads();
   if (rb >= nrows) return;
   T* wl = lds + wave * C::LDSW;
-  T acc[C::V], pw[C::V], inv[C::V];
-#pragma unroll
-  for (int e = 0; e < C::V; ++e) {
-    int b, i;
-    edge_pos<N, C::NB, Fi::P::last(), false>(lane, e, b, i);
-    const int lr = rb + b;
-    acc[e] = cz<T>();
-    pw[e] = cz<T>();
-    inv[e] = cz<T>();
-    if (lr < nrows) {
-      const int p = lr / a.NY, iy = lr - p * a.NY;
-      scatter_factors(a, model_iw<MAP>(a, plane_iw(a, p)), iy, i, pw[e], inv[e]);
-      T v;
-      if constexpr (BORN) {
-        v = cz<T>();
-        if (iy < a.ny && i < a.nx)
-          v = cmul(from_f2<T>(a.bg[(unsigned)(lr * N + i)]),
-                   from_f2<T>(a.dslow[(unsigned)((model_iw<MAP>(a, plane_iw(a, p)) * a.ny + iy) * a.nx + i)]));
-      } else {
-        v = from_f2<T>(a.in[(unsigned)(lr * N + i)]);
-      }
-      acc[e] = cmul(v, pw[e]);                 // c_0 = 1
-      pw[e] = cmul(pw[e], inv[e]);
+  // scattering source v of node (lr, i): U_z * ds_z on the model region
+  // (BORN) or the input row
+  auto source = [&](int lr, int p, int iy, int i) -> T {
+    if constexpr (BORN) {
+      if (iy < a.ny && i < a.nx)
+        return cmul(from_f2<T>(a.bg[(unsigned)(lr * N + i)]),
+                    from_f2<T>(a.dslow[(unsigned)((model_iw<MAP>(a, plane_iw(a, p)) * a.ny + iy) * a.nx + i)]));
+      return cz<T>();
+    } else {
+      return from_f2<T>(a.in[(unsigned)(lr * N + i)]);
     }
-  }
-  for (int it = 1; it < a.nterm; ++it) {
-    const float2* W = a.W + (size_t)(it - 1) * a.wstride;
-    T x[C::V];
+  };
+  T acc[C::V];
+  if constexpr (ONE) {
 #pragma unroll
     for (int e = 0; e < C::V; ++e) {
       int b, i;
       edge_pos<N, C::NB, Fi::P::first(), false>(lane, e, b, i);
       const int lr = rb + b;
-      x[e] = lr < nrows ? from_f2<T>(W[widx(a, lr, i, N)]) : cz<T>();
+      acc[e] = lr < nrows ? from_f2<T>(a.W[widx(a, lr, i, N)]) : cz<T>();
     }
-    Fi::run(x, wl, twl, lane);
-    const S c = (S)taylor_coef(it) * (S)a.scale;  // the inverse row transform's 1/sqrt(NX)
+    Fi::run(acc, wl, twl, lane);
+    const S c = (S)taylor_coef(1) * (S)a.scale;  // the inverse row transform's 1/sqrt(NX)
+    // the lane index made opaque here: the per-element offsets are formed
+    // after the transform instead of held in registers across it
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
 #pragma unroll
     for (int e = 0; e < C::V; ++e) {
-      acc[e] = cadd(acc[e], cscale(cmul(x[e], pw[e]), c));   // c_it s~^(-1/2-it) * term
-      pw[e] = cmul(pw[e], inv[e]);
+      int b, i;
+      edge_pos<N, C::NB, Fi::P::last(), false>(ln, e, b, i);
+      const int lr = rb + b;
+      if (lr < nrows) {
+        const int p = lr / a.NY, iy = lr - p * a.NY;
+        T pw, inv;
+        scatter_factors(a, model_iw<MAP>(a, plane_iw(a, p)), iy, i, pw, inv);
+        const T v = source(lr, p, iy, i);
+        const T pw1 = cmul(pw, inv);
+        acc[e] = cadd(cmul(v, pw), cscale(cmul(acc[e], pw1), c));  // c_0 = 1
+      } else {
+        acc[e] = cz<T>();
+      }
+    }
+  } else {
+    T pw[C::V], inv[C::V];
+#pragma unroll
+    for (int e = 0; e < C::V; ++e) {
+      int b, i;
+      edge_pos<N, C::NB, Fi::P::last(), false>(lane, e, b, i);
+      const int lr = rb + b;
+      acc[e] = cz<T>();
+      pw[e] = cz<T>();
+      inv[e] = cz<T>();
+      if (lr < nrows) {
+        const int p = lr / a.NY, iy = lr - p * a.NY;
+        scatter_factors(a, model_iw<MAP>(a, plane_iw(a, p)), iy, i, pw[e], inv[e]);
+        acc[e] = cmul(source(lr, p, iy, i), pw[e]);  // c_0 = 1
+        pw[e] = cmul(pw[e], inv[e]);
+      }
+    }
+    for (int it = 1; it < a.nterm; ++it) {
+      const float2* W = a.W + (size_t)(it - 1) * a.wstride;
+      T x[C::V];
+#pragma unroll
+      for (int e = 0; e < C::V; ++e) {
+        int b, i;
+        edge_pos<N, C::NB, Fi::P::first(), false>(lane, e, b, i);
+        const int lr = rb + b;
+        x[e] = lr < nrows ? from_f2<T>(W[widx(a, lr, i, N)]) : cz<T>();
+      }
+      Fi::run(x, wl, twl, lane);
+      const S c = (S)taylor_coef(it) * (S)a.scale;  // the inverse row transform's 1/sqrt(NX)
+#pragma unroll
+      for (int e = 0; e < C::V; ++e) {
+        acc[e] = cadd(acc[e], cscale(cmul(x[e], pw[e]), c));   // c_it s~^(-1/2-it) * term
+        pw[e] = cmul(pw[e], inv[e]);
+      }
     }
   }
   // acc *= -i * (w dz / 2)     (propagator/src/scatter.cu:155-195)
@@ -712,6 +753,9 @@ static hipError_t launch_mapped(const KArgs& a, hipStream_t st) {
 }
 template <int N, bool F, bool M, class T> struct KIcFwd { static constexpr auto fn = k_rows_ic_fwd<N, M>; };
 template <int N, bool F, bool M, class T> struct KScatter { static constexpr auto fn = k_rows_scatter<N, F, M, T>; };
+template <int N, bool F, bool M, class T> struct KScatter1 {
+  static constexpr auto fn = k_rows_scatter<N, F, M, T, true>;
+};
 template <int N, bool F, bool M, class T> struct KScatterAdj { static constexpr auto fn = k_rows_scatter_adj<N, F, M, T>; };
 template <int N, bool F, bool M, class T> struct KScatterAdj1 {
   static constexpr auto fn = k_rows_scatter_adj<N, F, M, T, true>;
@@ -724,6 +768,8 @@ hipError_t launch_rows_ic_fwd(const KArgs& a, hipStream_t st) {
   return launch_mapped<KIcFwd, true>(b, st);
 }
 hipError_t launch_rows_scatter(bool born, const KArgs& a, hipStream_t st) {
+  if (a.nterm == 2)
+    return born ? launch_mapped<KScatter1, true, true>(a, st) : launch_mapped<KScatter1, false>(a, st);
   return born ? launch_mapped<KScatter, true, true>(a, st) : launch_mapped<KScatter, false>(a, st);
 }
 hipError_t launch_rows_scatter_adj(bool fused, const KArgs& a, hipStream_t st) {
